@@ -1,0 +1,124 @@
+"""ctypes binding of libehip.so (the C ABI declared in include/ehip.h).
+
+The library is built in-tree by elmerfem_amd.build.  There is no fallback:
+if the shared object is missing or no GPU is present, calls fail loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libehip.so")
+
+# names and codes mirror include/ehip.h
+EHIP_OK = 0
+CG, BICGSTAB = 0, 1
+PRECOND_NONE, PRECOND_DIAGONAL = 0, 1
+FP64, MIXED_IR = 0, 1
+FAITHFUL, FAST = 0, 1
+PHYS_MODELPDE, PHYS_ELASTIC = 0, 1
+
+HUTI_CONVERGENCE = 1
+HUTI_MAXITER = 2
+HUTI_DIVERGENCE = 3
+HUTI_CG_RHO = 20
+HUTI_BICGSTAB_RHO = 35
+HUTI_BICGSTAB_SNORM = 36
+HUTI_BICGSTAB_OMEGA = 37
+
+
+class Params(C.Structure):
+    _fields_ = [("method", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
+                ("scaling", C.c_int), ("precision", C.c_int), ("mode", C.c_int), ("inner_tol", C.c_double),
+                ("inner_maxit", C.c_int), ("check_every", C.c_int), ("divtol", C.c_double),
+                ("record_history", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("info", C.c_int), ("iters", C.c_int), ("outer_iters", C.c_int), ("spmv_count", C.c_int),
+                ("res_scaled", C.c_double), ("res_true", C.c_double), ("norm", C.c_double),
+                ("t_total_ms", C.c_double), ("t_krylov_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# every exported symbol and its signature; tests check that each is present
+_vp = C.c_void_p
+_ip = C.POINTER(C.c_int)
+_i64 = C.c_int64
+SIGNATURES = {
+    "ehip_init": (C.c_int, [C.c_int]),
+    "ehip_last_error": (C.c_char_p, []),
+    "ehip_version": (C.c_int, []),
+    "ehip_matvec": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_matvec_free": (None, [_vp]),
+    "ehip_system_create": (C.c_int, [C.c_int, _i64, _vp, _vp, _vp, _vp]),
+    "ehip_system_from_mesh": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int,
+                                        _vp]),
+    "ehip_system_free": (C.c_int, [_vp]),
+    "ehip_system_info": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_system_get_pattern": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "ehip_system_set_values": (C.c_int, [_vp, _vp, _vp]),
+    "ehip_system_get_values": (C.c_int, [_vp, _vp, _vp]),
+    "ehip_system_bytes": (_i64, [_vp]),
+    "ehip_assemble": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp]),
+    "ehip_assemble_traction": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp]),
+    "ehip_dirichlet": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int]),
+    "ehip_solve": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(Report), _vp]),
+    "ehip_get_solution": (C.c_int, [_vp, _vp]),
+    "ehip_solve_csr": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params), C.POINTER(Report)]),
+    "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
+    "ehip_spmv_config": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "ehip_spmv_tune": (C.c_int, [_vp, C.c_int]),
+    "ehip_batch_create": (C.c_int, [C.c_int, _vp, _vp]),
+    "ehip_batch_free": (C.c_int, [_vp]),
+    "ehip_batch_solve": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
+    "ehip_batch_get_solution": (C.c_int, [_vp, C.c_int, _vp]),
+}
+
+_lib = None
+
+
+class EhipError(RuntimeError):
+    pass
+
+
+def load(require_all: bool = True):
+    """Loads libehip.so and binds every declared symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EhipError("libehip.so is not built (run python -m elmerfem_amd.build): %s" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        if not hasattr(L, name):
+            if require_all:
+                raise EhipError("libehip.so lacks %s" % name)
+            continue
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != EHIP_OK:
+        msg = _lib.ehip_last_error().decode() if _lib is not None else ""
+        raise EhipError("%s failed (%d): %s" % (what, rc, msg))
+
+
+_inited = False
+
+
+def init(device: int = 0):
+    global _inited
+    L = load()
+    if not _inited:
+        check(L.ehip_init(device), "ehip_init")
+        _inited = True
+    return L

@@ -1,0 +1,655 @@
+// Element matrices and the deterministic CRS gather of libehip.
+//
+// Element side: ElementInfo (fem/src/ElementDescription.src:2586-3505) at
+// the default Gauss points (fem/src/Integration.src:1490-1596) of each
+// element type, with the ModelPDE local matrix (modules/ModelPDE.src:75-166)
+// or the isotropic StressCompose matrix (fem/src/Stress.src:59-521).
+// Reference-element basis tables come from the same monomial description as
+// elements.def (inverse Vandermonde, AddElementDescription :73-305).
+//
+// Glue side: instead of CRS_GlueLocalMatrix's searched atomic adds
+// (CRSMatrix.src:507-590) every CRS slot gathers its contributions from a
+// precomputed list sorted by element index, so Values are the reference's
+// serial element-order sums and identical run to run (no float atomics).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ehip_internal.h"
+#include "linalg_kernels.h"
+#include "pattern.h"
+
+extern "C" int ehip_internal_upload_pattern(ehip_system *s);
+
+namespace ehip {
+
+// ---------------------------------------------------------------- element descriptions
+
+struct EType {
+  int code = 0, dim = 0, npe = 0, ngp = 0;
+  std::vector<double> U, V, W;
+  std::vector<int> p, q, r;
+  std::vector<double> coeff;  // coeff[term*npe + node]
+  std::vector<double> gu, gv, gw, gs;
+  std::vector<double> basis;  // [ngp][npe]
+  std::vector<double> dlb;    // [ngp][npe][3]
+};
+
+// ComputeGaussPoints1D (Integration.src:406-570) for n <= 3, where the
+// companion matrix is 1x1; Newton refinement and weight renormalisation as
+// in the reference.
+static void gauss1d(int n, double *pts, double *wts) {
+  if (n <= 1) { pts[0] = 0.0; wts[0] = 2.0; return; }
+  double P[8] = {0}, P0[8] = {0}, P1[8] = {0}, Q[8] = {0};
+  P0[0] = 1; P1[0] = 1; P1[1] = 0;
+  for (int i = 1; i <= n - 1; i++) {
+    for (int k = 0; k < i + 1; k++) P[k] = (double)(2 * i + 1) * P1[k] / (double)(i + 1);
+    for (int k = 2; k < i + 2; k++) P[k] = P[k] - (double)i * P0[k - 2] / (double)(i + 1);
+    for (int k = 0; k < i + 1; k++) P0[k] = P1[k];
+    for (int k = 0; k < i + 2; k++) P1[k] = P[k];
+  }
+  int np = (n - (n % 2)) / 2;
+  for (int i = 1; i <= np + 1; i++) P[i - 1] = P[2 * i - 2];
+  const double eig = -P[1] / P[0];
+  double Qs[8];
+  for (int i = 0; i < np + 1; i++) Qs[i] = P[i];
+  std::memset(P, 0, sizeof(P));
+  for (int i = 1; i <= np + 1; i++) P[2 * i - 2] = Qs[i - 1];
+  pts[0] = +std::sqrt(eig);
+  pts[1] = -std::sqrt(eig);
+  if (n % 2 == 1) pts[n - 1] = 0.0;
+  auto ev = [](int m, const double *c, double x) { double s = 0; for (int i = 0; i < m + 1; i++) s = s * x + c[i]; return s; };
+  for (int i = 0; i < n; i++) Q[i] = P[i] * (double)(n - i);
+  for (int i = 0; i < n; i++) {
+    double x = pts[i], s;
+    for (int j = 0; j < 100; j++) {
+      s = ev(n, P, x) / ev(n - 1, Q, x);
+      x = x - s;
+      if (std::fabs(s) <= std::fabs(x) * DBL_EPSILON) break;
+    }
+    if (std::fabs(ev(n, P, x)) < std::fabs(ev(n, P, pts[i])) && std::fabs(x - pts[i]) < 1.0e-8) pts[i] = x;
+  }
+  double sum = 0;
+  for (int i = 0; i < n; i++) {
+    const double s = ev(n - 1, Q, pts[i]);
+    wts[i] = 2.0 / ((1.0 - pts[i] * pts[i]) * (s * s));
+  }
+  for (int i = 0; i < n; i++) sum += wts[i];
+  for (int i = 0; i < n; i++) wts[i] = 2.0 * wts[i] / sum;
+}
+
+static bool make_etype(int code, EType &t) {
+  static const double H27U[27] = {-1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, 0, 0, 0};
+  static const double H27V[27] = {-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, 0, 0, 0};
+  static const double H27W[27] = {-1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0, 1, 1, 1, 1, 0, 0, 0, 0, -1, 1, 0};
+  static const int H27B[27] = {1, 2, 3, 5, 6, 7, 9, 10, 11, 17, 18, 19, 21, 22, 23, 25, 26, 27, 33, 34, 35, 37, 38, 39, 41, 42, 43};
+  static const double Q9U[9] = {-1, 1, 1, -1, 0, 1, 0, -1, 0}, Q9V[9] = {-1, -1, 1, 1, -1, 0, 1, 0, 0};
+  static const int Q9B[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
+  static const int B404[4] = {1, 2, 5, 6}, B504[4] = {1, 2, 5, 17}, B808[8] = {1, 2, 5, 6, 17, 18, 21, 22}, B202[2] = {1, 2};
+  const int *B = nullptr;
+  t.code = code;
+  switch (code) {
+    case 202: t.dim = 1; t.npe = 2; B = B202; t.U = {-1, 1}; t.V = {0, 0}; t.W = {0, 0}; break;
+    case 404: t.dim = 2; t.npe = 4; B = B404; t.U = {-1, 1, 1, -1}; t.V = {-1, -1, 1, 1}; t.W = {0, 0, 0, 0}; break;
+    case 409: t.dim = 2; t.npe = 9; B = Q9B; t.U.assign(Q9U, Q9U + 9); t.V.assign(Q9V, Q9V + 9); t.W.assign(9, 0.0); break;
+    case 504: t.dim = 3; t.npe = 4; B = B504; t.U = {0, 1, 0, 0}; t.V = {0, 0, 1, 0}; t.W = {0, 0, 0, 1}; break;
+    case 808:
+      t.dim = 3; t.npe = 8; B = B808;
+      t.U = {-1, 1, 1, -1, -1, 1, 1, -1}; t.V = {-1, -1, 1, 1, -1, -1, 1, 1}; t.W = {-1, -1, -1, -1, 1, 1, 1, 1};
+      break;
+    case 827:
+      t.dim = 3; t.npe = 27; B = H27B;
+      t.U.assign(H27U, H27U + 27); t.V.assign(H27V, H27V + 27); t.W.assign(H27W, H27W + 27);
+      break;
+    default: return false;
+  }
+  const int n = t.npe;
+  t.p.resize(n); t.q.resize(n); t.r.resize(n);
+  for (int k = 0; k < n; k++) { int m = B[k] - 1; t.p[k] = m % 4; t.q[k] = (m / 4) % 4; t.r[k] = m / 16; }
+  auto ipow = [](double x, int e) { double v = 1; for (int i = 0; i < e; i++) v *= x; return v; };
+  // Vandermonde V(node, term), inverted by Gauss-Jordan with partial pivoting
+  std::vector<double> a((size_t)n * n), inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < n; k++) a[(size_t)i * n + k] = ipow(t.U[i], t.p[k]) * ipow(t.V[i], t.q[k]) * ipow(t.W[i], t.r[k]);
+  for (int i = 0; i < n; i++) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; c++) {
+    int pr = c;
+    for (int r = c + 1; r < n; r++) if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)pr * n + c])) pr = r;
+    if (a[(size_t)pr * n + c] == 0.0) return false;
+    if (pr != c)
+      for (int k = 0; k < n; k++) { std::swap(a[(size_t)c * n + k], a[(size_t)pr * n + k]); std::swap(inv[(size_t)c * n + k], inv[(size_t)pr * n + k]); }
+    const double d = a[(size_t)c * n + c];
+    for (int k = 0; k < n; k++) { a[(size_t)c * n + k] /= d; inv[(size_t)c * n + k] /= d; }
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; k++) { a[(size_t)r * n + k] -= f * a[(size_t)c * n + k]; inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k]; }
+    }
+  }
+  // V * C = I with C(term, node) = inv(term, node)
+  t.coeff.assign((size_t)n * n, 0.0);
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < n; i++) t.coeff[(size_t)k * n + i] = inv[(size_t)k * n + i];
+  // default Gauss rule
+  double pts[3], wts[3];
+  if (t.dim == 1) {
+    gauss1d(2, pts, wts);
+    for (int i = 0; i < 2; i++) { t.gu.push_back(pts[i]); t.gv.push_back(0); t.gw.push_back(0); t.gs.push_back(wts[i]); }
+  } else if (t.dim == 2) {
+    const int m = code == 409 ? 3 : 2;
+    gauss1d(m, pts, wts);
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++) { t.gu.push_back(pts[j]); t.gv.push_back(pts[i]); t.gw.push_back(0); t.gs.push_back(wts[i] * wts[j]); }
+  } else if (code == 504) {
+    static const double u4[4] = {0.1757281246520584, 0.2445310270213291, 0.5556470949048655, 0.0240937534217468};
+    static const double v4[4] = {0.5656137776620919, 0.0501800797762026, 0.1487681308666864, 0.2354380116950194};
+    static const double w4[4] = {0.2180665126782654, 0.5635595064952189, 0.0350112499848832, 0.1833627308416330};
+    for (int i = 0; i < 4; i++) { t.gu.push_back(u4[i]); t.gv.push_back(v4[i]); t.gw.push_back(w4[i]); t.gs.push_back(0.25 / 6.0); }
+  } else {
+    const int m = code == 827 ? 3 : 2;
+    gauss1d(m, pts, wts);
+    for (int i = 0; i < m; i++)
+      for (int j = 0; j < m; j++)
+        for (int k = 0; k < m; k++) {
+          t.gu.push_back(pts[k]); t.gv.push_back(pts[j]); t.gw.push_back(pts[i]);
+          t.gs.push_back(wts[i] * wts[j] * wts[k]);
+        }
+  }
+  t.ngp = (int)t.gu.size();
+  t.basis.assign((size_t)t.ngp * n, 0.0);
+  t.dlb.assign((size_t)t.ngp * n * 3, 0.0);
+  for (int g = 0; g < t.ngp; g++)
+    for (int i = 0; i < n; i++) {
+      double s = 0, du = 0, dv = 0, dw = 0;
+      const double u = t.gu[g], v = t.gv[g], w = t.gw[g];
+      for (int k = 0; k < n; k++) {
+        const double c = t.coeff[(size_t)k * n + i];
+        if (c == 0.0) continue;
+        const int P = t.p[k], Qd = t.q[k], R = t.r[k];
+        s += c * ipow(u, P) * ipow(v, Qd) * ipow(w, R);
+        if (P >= 1) du += c * P * ipow(u, P - 1) * ipow(v, Qd) * ipow(w, R);
+        if (Qd >= 1) dv += c * Qd * ipow(u, P) * ipow(v, Qd - 1) * ipow(w, R);
+        if (R >= 1) dw += c * R * ipow(u, P) * ipow(v, Qd) * ipow(w, R - 1);
+      }
+      t.basis[(size_t)g * n + i] = s;
+      t.dlb[((size_t)g * n + i) * 3 + 0] = du;
+      t.dlb[((size_t)g * n + i) * 3 + 1] = dv;
+      t.dlb[((size_t)g * n + i) * 3 + 2] = dw;
+    }
+  return true;
+}
+
+// ElementInfo's metric for one Gauss point (ElementMetric :6757-6884 and
+// InvertMatrix3x3 :8536-8556).  dx[c][k] = sum_a X_c(a) dLB(a,k);
+// G = dx^T dx over cdim; returns LtoG (cdim x dim) and sqrt(det G).
+template <int DIM, int CDIM>
+__device__ __forceinline__ bool metric(const double (&dx)[3][3], double (&L)[3][3], double &detj) {
+  double G[3][3], M[3][3];
+#pragma unroll
+  for (int i = 0; i < DIM; i++)
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < CDIM; k++) s = s + dx[k][i] * dx[k][j];
+      G[i][j] = s;
+    }
+  double detg;
+  if constexpr (DIM == 1) {
+    detg = G[0][0];
+    if (!(detg > DBL_MIN)) return false;
+    M[0][0] = 1.0 / detg;
+  } else if constexpr (DIM == 2) {
+    detg = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+    if (!(detg > DBL_MIN)) return false;
+    M[0][0] = G[1][1] / detg;
+    M[0][1] = -G[0][1] / detg;
+    M[1][0] = -G[1][0] / detg;
+    M[1][1] = G[0][0] / detg;
+  } else {
+    detg = G[0][0] * (G[1][1] * G[2][2] - G[1][2] * G[2][1]) + G[0][1] * (G[1][2] * G[2][0] - G[1][0] * G[2][2]) +
+           G[0][2] * (G[1][0] * G[2][1] - G[1][1] * G[2][0]);
+    if (!(detg > DBL_MIN)) return false;
+    const double s = 1.0 / detg;
+    M[0][0] = s * (G[1][1] * G[2][2] - G[2][1] * G[1][2]);
+    M[1][0] = -s * (G[1][0] * G[2][2] - G[2][0] * G[1][2]);
+    M[2][0] = s * (G[1][0] * G[2][1] - G[2][0] * G[1][1]);
+    M[0][1] = -s * (G[0][1] * G[2][2] - G[2][1] * G[0][2]);
+    M[1][1] = s * (G[0][0] * G[2][2] - G[2][0] * G[0][2]);
+    M[2][1] = -s * (G[0][0] * G[2][1] - G[2][0] * G[0][1]);
+    M[0][2] = s * (G[0][1] * G[1][2] - G[1][1] * G[0][2]);
+    M[1][2] = -s * (G[0][0] * G[1][2] - G[1][0] * G[0][2]);
+    M[2][2] = s * (G[0][0] * G[1][1] - G[1][0] * G[0][1]);
+  }
+  detj = sqrt(detg);
+#pragma unroll
+  for (int i = 0; i < CDIM; i++)
+#pragma unroll
+    for (int j = 0; j < DIM; j++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < DIM; k++) s = s + dx[i][k] * M[k][j];
+      L[i][j] = s;
+    }
+  return true;
+}
+
+struct ElemArgs {
+  int nelem;
+  const int *el;        // 0-based node ids, nelem*npe
+  const double *xyz;    // nnodes*3
+  const double *basis;  // [ngp][npe]
+  const double *dlb;    // [ngp][npe][3]
+  const double *gs;     // [ngp]
+  const double *coef, *conv, *velo, *load;  // per element (may be null)
+  double poisson;
+  double *emat;         // nelem * ld * ld, row-major local matrices
+  double *evec;         // nelem * ld
+  int *bad;             // degenerate element counter
+};
+
+// One workgroup handles EPB elements.  Phase 1: per-Gauss-point metric and
+// material values; phase 2: global basis gradients; phase 3: one lane per
+// (p, q) node pair accumulates over the Gauss points in Gauss-point order.
+template <int NPE, int NGP, int DIM, int CDIM, int PHYS, int EPB, int BS>
+__global__ __launch_bounds__(BS) void k_element(ElemArgs a) {
+  constexpr int DOFS = PHYS == EHIP_PHYS_ELASTIC ? 3 : 1;
+  constexpr int LD = NPE * DOFS;
+  __shared__ double X[EPB][NPE][3];
+  __shared__ double Lg[EPB][NGP][3][3];
+  __shared__ double Wg[EPB][NGP];
+  __shared__ double Mat[EPB][NGP][6];  // D, C, a0, a1, a2, load  (ModelPDE) | Young, Poisson (elasticity)
+  __shared__ double dB[EPB][NGP][NPE][3];
+  __shared__ double Bs[NGP][NPE];
+  const int e0 = blockIdx.x * EPB;
+  for (int k = threadIdx.x; k < NGP * NPE; k += BS) Bs[k / NPE][k % NPE] = a.basis[k];
+  for (int k = threadIdx.x; k < EPB * NPE; k += BS) {
+    const int le = k / NPE, nd = k % NPE, e = e0 + le;
+    if (e < a.nelem) {
+      const int g = a.el[(size_t)e * NPE + nd];
+      X[le][nd][0] = a.xyz[3 * (size_t)g];
+      X[le][nd][1] = a.xyz[3 * (size_t)g + 1];
+      X[le][nd][2] = a.xyz[3 * (size_t)g + 2];
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < EPB * NGP; k += BS) {
+    const int le = k / NGP, gp = k % NGP, e = e0 + le;
+    if (e >= a.nelem) continue;
+    double dx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+    for (int i = 0; i < DIM; i++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double s = 0.0;
+        for (int nd = 0; nd < NPE; nd++) s += X[le][nd][c] * a.dlb[((size_t)gp * NPE + nd) * 3 + i];
+        dx[c][i] = s;
+      }
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, detj = 0.0;
+    if (!metric<DIM, CDIM>(dx, L, detj)) {
+      atomicAdd(a.bad, 1);
+      detj = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) Lg[le][gp][i][j] = L[i][j];
+    Wg[le][gp] = a.gs[gp] * detj;
+    // interpolate per-element (nodal-constant) coefficients with the basis, as GetReal + SUM(Basis*...)
+    double sb[6] = {0, 0, 0, 0, 0, 0};
+    const double c0 = a.coef ? a.coef[e] : 0.0;
+    const double c1 = PHYS == EHIP_PHYS_ELASTIC ? a.poisson : (a.conv ? a.conv[e] : 0.0);
+    const double v0 = a.velo ? a.velo[3 * (size_t)e] : 0.0, v1 = a.velo ? a.velo[3 * (size_t)e + 1] : 0.0,
+                 v2 = a.velo ? a.velo[3 * (size_t)e + 2] : 0.0;
+    const double ld = a.load ? a.load[e] : 0.0;
+    for (int nd = 0; nd < NPE; nd++) {
+      const double b = Bs[gp][nd];
+      sb[0] += b * c0;
+      sb[1] += b * c1;
+      sb[2] += v0 * b;
+      sb[3] += v1 * b;
+      sb[4] += v2 * b;
+      sb[5] += b * ld;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) Mat[le][gp][i] = sb[i];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < EPB * NGP * NPE; k += BS) {
+    const int le = k / (NGP * NPE), rem = k % (NGP * NPE), gp = rem / NPE, nd = rem % NPE;
+    const double *d = a.dlb + ((size_t)gp * NPE + nd) * 3;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      double s = 0.0;
+      if (j < CDIM)
+#pragma unroll
+        for (int kk = 0; kk < DIM; kk++) s = s + d[kk] * Lg[le][gp][j][kk];
+      dB[le][gp][nd][j] = s;
+    }
+  }
+  __syncthreads();
+  if constexpr (PHYS == EHIP_PHYS_MODELPDE) {
+    for (int k = threadIdx.x; k < EPB * NPE * NPE; k += BS) {
+      const int le = k / (NPE * NPE), rem = k % (NPE * NPE), p = rem / NPE, q = rem % NPE, e = e0 + le;
+      if (e >= a.nelem) continue;
+      double acc = 0.0;
+      for (int gp = 0; gp < NGP; gp++) {
+        const double W = Wg[le][gp];
+        const double WD = W * Mat[le][gp][0];
+        const double m = dB[le][gp][p][0] * dB[le][gp][q][0] + dB[le][gp][p][1] * dB[le][gp][q][1] +
+                         dB[le][gp][p][2] * dB[le][gp][q][2];
+        acc = acc + WD * m;
+        const double C = Mat[le][gp][1];
+        if (C != 0.0) {
+          double sa = 0.0;
+#pragma unroll
+          for (int kk = 0; kk < CDIM; kk++) sa += Mat[le][gp][2 + kk] * dB[le][gp][q][kk];
+          acc = acc + W * C * sa * Bs[gp][p];
+        }
+      }
+      a.emat[(size_t)e * LD * LD + p * LD + q] = acc;
+    }
+    for (int k = threadIdx.x; k < EPB * NPE; k += BS) {
+      const int le = k / NPE, p = k % NPE, e = e0 + le;
+      if (e >= a.nelem) continue;
+      double f = 0.0;
+      for (int gp = 0; gp < NGP; gp++) f = f + Wg[le][gp] * Mat[le][gp][5] * Bs[gp][p];
+      a.evec[(size_t)e * LD + p] = f;
+    }
+  } else {
+    for (int k = threadIdx.x; k < EPB * NPE * NPE; k += BS) {
+      const int le = k / (NPE * NPE), rem = k % (NPE * NPE), p = rem / NPE, q = rem % NPE, e = e0 + le;
+      if (e >= a.nelem) continue;
+      double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int gp = 0; gp < NGP; gp++) {
+        const double s = Wg[le][gp];
+        const double E = Mat[le][gp][0], nu = Mat[le][gp][1];
+        const double den = (1 + nu) * (1 - 2 * nu);
+        const double c11 = (1.0 - nu) * E / den, c12 = nu * E / den, c44 = (0.5 - nu) * E / den;
+        const double px = dB[le][gp][p][0], py = dB[le][gp][p][1], pz = dB[le][gp][p][2];
+        const double qx = dB[le][gp][q][0], qy = dB[le][gp][q][1], qz = dB[le][gp][q][2];
+        // G_p C (rows i = 1..3 over the 6 Voigt strains), then times B_q
+        const double g00 = px * c11, g01 = px * c12, g02 = px * c12, g03 = py * c44, g05 = pz * c44;
+        const double g10 = py * c12, g11 = py * c11, g12 = py * c12, g13 = px * c44, g14 = pz * c44;
+        const double g20 = pz * c12, g21 = pz * c12, g22 = pz * c11, g24 = py * c44, g25 = px * c44;
+        A[0][0] = A[0][0] + s * (g00 * qx + g03 * qy + g05 * qz);
+        A[0][1] = A[0][1] + s * (g01 * qy + g03 * qx);
+        A[0][2] = A[0][2] + s * (g02 * qz + g05 * qx);
+        A[1][0] = A[1][0] + s * (g10 * qx + g13 * qy);
+        A[1][1] = A[1][1] + s * (g11 * qy + g13 * qx + g14 * qz);
+        A[1][2] = A[1][2] + s * (g12 * qz + g14 * qy);
+        A[2][0] = A[2][0] + s * (g20 * qx + g25 * qz);
+        A[2][1] = A[2][1] + s * (g21 * qy + g24 * qz);
+        A[2][2] = A[2][2] + s * (g22 * qz + g24 * qy + g25 * qx);
+      }
+      double *out = a.emat + (size_t)e * LD * LD;
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) out[(3 * p + i) * LD + 3 * q + j] = A[i][j];
+    }
+    for (int k = threadIdx.x; k < EPB * LD; k += BS) {
+      const int le = k / LD, i = k % LD, e = e0 + le;
+      if (e < a.nelem) a.evec[(size_t)e * LD + i] = 0.0;
+    }
+  }
+}
+
+// Values[s] = sum over the slot's contributions in element order, from 0
+__global__ void k_gather(int64_t nslots, const int *__restrict__ start, const int *__restrict__ contrib,
+                         const double *__restrict__ emat, double *__restrict__ vals) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
+    const int a = start[s], b = start[s + 1];
+    double v = 0.0;
+    for (int k = a; k < b; ++k) v = v + emat[(uint32_t)contrib[k]];
+    vals[s] = v;
+  }
+}
+
+struct DevTables {
+  double *basis = nullptr, *dlb = nullptr, *gs = nullptr;
+};
+
+static int upload_tables(ehip_system *s, const EType &t, DevTables &d) {
+  int rc;
+  if ((rc = dmalloc(nullptr, (void **)&d.basis, sizeof(double) * t.basis.size()))) return rc;
+  if ((rc = dmalloc(nullptr, (void **)&d.dlb, sizeof(double) * t.dlb.size()))) return rc;
+  if ((rc = dmalloc(nullptr, (void **)&d.gs, sizeof(double) * t.gs.size()))) return rc;
+  EHIP_HIP_TRY(hipMemcpyAsync(d.basis, t.basis.data(), sizeof(double) * t.basis.size(), hipMemcpyHostToDevice, s->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(d.dlb, t.dlb.data(), sizeof(double) * t.dlb.size(), hipMemcpyHostToDevice, s->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(d.gs, t.gs.data(), sizeof(double) * t.gs.size(), hipMemcpyHostToDevice, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  return EHIP_OK;
+}
+
+static void free_tables(DevTables &d) {
+  dfree(nullptr, d.basis, 0);
+  dfree(nullptr, d.dlb, 0);
+  dfree(nullptr, d.gs, 0);
+}
+
+template <int NPE, int NGP, int DIM, int CDIM, int PHYS, int EPB, int BS>
+static void launch_element(ehip_system *s, const ElemArgs &a) {
+  const int grid = (a.nelem + EPB - 1) / EPB;
+  hipLaunchKernelGGL((k_element<NPE, NGP, DIM, CDIM, PHYS, EPB, BS>), dim3(grid), dim3(BS), 0, s->stream, a);
+}
+
+static int dispatch_element(ehip_system *s, int physics, int code, int cdim, const ElemArgs &a) {
+  if (physics == EHIP_PHYS_MODELPDE) {
+    if (code == 808 && cdim == 3) { launch_element<8, 8, 3, 3, 0, 4, 256>(s, a); return EHIP_OK; }
+    if (code == 504 && cdim == 3) { launch_element<4, 4, 3, 3, 0, 16, 256>(s, a); return EHIP_OK; }
+    if (code == 404 && cdim == 2) { launch_element<4, 4, 2, 2, 0, 16, 256>(s, a); return EHIP_OK; }
+    if (code == 827 && cdim == 3) { launch_element<27, 27, 3, 3, 0, 1, 256>(s, a); return EHIP_OK; }
+  } else {
+    if (code == 808) { launch_element<8, 8, 3, 3, 1, 4, 256>(s, a); return EHIP_OK; }
+    if (code == 827) { launch_element<27, 27, 3, 3, 1, 1, 256>(s, a); return EHIP_OK; }
+  }
+  set_error("ehip_assemble: element %d / physics %d / cdim %d not supported", code, physics, cdim);
+  return EHIP_EUNSUP;
+}
+
+}  // namespace ehip
+
+using namespace ehip;
+
+extern "C" int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *elements, int nbnd, int npe_bnd,
+                                     const int *bnd_elements, int dofs, int optimize, ehip_system **out) {
+  if (!elements || !out || nnodes <= 0 || nelem <= 0 || npe <= 0 || npe > 27 || dofs < 1 || dofs > 3) {
+    set_error("ehip_system_from_mesh: bad arguments");
+    return EHIP_EARG;
+  }
+  HostPattern P;
+  int prc = build_host_pattern(nnodes, nelem, npe, elements, bnd_elements ? nbnd : 0, npe_bnd, bnd_elements, dofs,
+                               optimize, P);
+  if (prc) {
+    set_error("ehip_system_from_mesh: pattern build failed (%d): node ids out of range, unused nodes or nnz >= 2^31", prc);
+    return prc == -3 ? EHIP_ECAP : EHIP_EARG;
+  }
+  std::vector<int> sstart, scontrib, rstart, rcontrib;
+  prc = build_slot_maps(P, nelem, npe, elements, sstart, scontrib, rstart, rcontrib);
+  if (prc) {
+    set_error("ehip_system_from_mesh: element-to-slot map exceeds int32 (%d)", prc);
+    return EHIP_ECAP;
+  }
+  ehip_system *s = new ehip_system();
+  EHIP_HIP_TRY(hipGetDevice(&s->dev));
+  EHIP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  s->n = P.n;
+  s->nnz = P.nnz;
+  s->dofs = dofs;
+  s->nnodes = nnodes;
+  s->h_rows = std::move(P.rows);
+  s->h_cols = std::move(P.cols);
+  s->h_diag = std::move(P.diag);
+  s->h_perm = std::move(P.perm);
+  s->bw_before = P.bw_before;
+  s->bw_after = P.bw_after;
+  s->bw_accepted = P.accepted;
+  int rc = ehip_internal_upload_pattern(s);
+  if (rc) { ehip_system_free(s); return rc; }
+  s->nelem = nelem;
+  s->npe = npe;
+  const int ld = npe * dofs;
+  s->ncontrib = (int64_t)scontrib.size();
+  std::vector<int> el0((size_t)nelem * npe);
+  for (size_t k = 0; k < el0.size(); ++k) el0[k] = elements[k] - 1;
+  if ((rc = dmalloc(s, (void **)&s->elements, sizeof(int) * el0.size())) ||
+      (rc = dmalloc(s, (void **)&s->slot_start, sizeof(int) * sstart.size())) ||
+      (rc = dmalloc(s, (void **)&s->slot_contrib, sizeof(int) * scontrib.size())) ||
+      (rc = dmalloc(s, (void **)&s->rhs_start, sizeof(int) * rstart.size())) ||
+      (rc = dmalloc(s, (void **)&s->rhs_contrib, sizeof(int) * rcontrib.size())) ||
+      (rc = dmalloc(s, (void **)&s->emat, sizeof(double) * (size_t)nelem * ld * ld)) ||
+      (rc = dmalloc(s, (void **)&s->evec, sizeof(double) * (size_t)nelem * ld))) {
+    ehip_system_free(s);
+    return rc;
+  }
+  EHIP_HIP_TRY(hipMemcpy(s->elements, el0.data(), sizeof(int) * el0.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->slot_start, sstart.data(), sizeof(int) * sstart.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->slot_contrib, scontrib.data(), sizeof(int) * scontrib.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->rhs_start, rstart.data(), sizeof(int) * rstart.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->rhs_contrib, rcontrib.data(), sizeof(int) * rcontrib.size(), hipMemcpyHostToDevice));
+  *out = s;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdim, const double *coords,
+                             const double *elem_coef, const double *elem_conv, const double *elem_velo,
+                             const double *elem_load, double poisson, double *t_ms) {
+  if (!s || !coords || !s->emat) {
+    set_error("ehip_assemble: system was not created from a mesh");
+    return EHIP_ESTATE;
+  }
+  const int ld = s->npe * s->dofs;
+  if ((physics == EHIP_PHYS_ELASTIC) != (s->dofs == 3) || (physics != EHIP_PHYS_ELASTIC && s->dofs != 1)) {
+    set_error("ehip_assemble: physics %d does not match %d dofs per node", physics, s->dofs);
+    return EHIP_EARG;
+  }
+  EType t;
+  if (!make_etype(elem_code, t) || t.npe != s->npe) {
+    set_error("ehip_assemble: element type %d does not match the mesh (%d nodes)", elem_code, s->npe);
+    return EHIP_EUNSUP;
+  }
+  const int ne = s->nelem;
+  int rc;
+  DevTables tab;
+  if ((rc = upload_tables(s, t, tab))) return rc;
+  // inputs to the device
+  std::vector<void *> tmp;
+  auto up = [&](const double *h, size_t cnt) -> double * {
+    if (!h) return nullptr;
+    double *d = nullptr;
+    if (dmalloc(nullptr, (void **)&d, sizeof(double) * cnt)) return nullptr;
+    (void)hipMemcpyAsync(d, h, sizeof(double) * cnt, hipMemcpyHostToDevice, s->stream);
+    tmp.push_back(d);
+    return d;
+  };
+  ElemArgs a{};
+  a.nelem = ne;
+  a.el = s->elements;
+  a.xyz = up(coords, (size_t)s->nnodes * 3);
+  a.basis = tab.basis;
+  a.dlb = tab.dlb;
+  a.gs = tab.gs;
+  a.coef = up(elem_coef, ne);
+  a.conv = up(elem_conv, ne);
+  a.velo = up(elem_velo, (size_t)ne * 3);
+  a.load = up(elem_load, ne);
+  a.poisson = poisson;
+  a.emat = s->emat;
+  a.evec = s->evec;
+  int *bad = nullptr;
+  if ((rc = dmalloc(nullptr, (void **)&bad, sizeof(int)))) return rc;
+  EHIP_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
+  a.bad = bad;
+  hipEvent_t e0, e1;
+  EHIP_HIP_TRY(hipEventCreate(&e0));
+  EHIP_HIP_TRY(hipEventCreate(&e1));
+  EHIP_HIP_TRY(hipEventRecord(e0, s->stream));
+  rc = dispatch_element(s, physics, elem_code, cdim, a);
+  if (!rc) {
+    const int g = 2048;
+    hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, s->stream, s->nnz, s->slot_start, s->slot_contrib, s->emat,
+                       s->vals);
+    hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, s->stream, (int64_t)s->n, s->rhs_start, s->rhs_contrib,
+                       s->evec, s->rhs);
+  }
+  EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
+  EHIP_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (t_ms) *t_ms = ms;
+  int hbad = 0;
+  EHIP_HIP_TRY(hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
+  for (void *p : tmp) dfree(nullptr, p, 0);
+  dfree(nullptr, bad, 0);
+  free_tables(tab);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  EHIP_HIP_TRY(hipGetLastError());
+  if (rc) return rc;
+  if (hbad) {
+    set_error("ehip_assemble: %d degenerate Gauss-point metrics (ElementMetric)", hbad);
+    return EHIP_EARG;
+  }
+  (void)ld;
+  s->has_values = true;
+  return EHIP_OK;
+}
+
+// StressBoundary traction part (Stress.src:836-1000), evaluated on the host
+// in boundary-element order and added to the device RHS.
+extern "C" int ehip_assemble_traction(ehip_system *s, int bnd_code, const double *coords, int nb, int npe,
+                                      const int *belems, const double *load3) {
+  if (!s || !coords || !belems || !load3 || s->dofs != 3) return EHIP_EARG;
+  EType t;
+  if (!make_etype(bnd_code, t) || t.npe != npe || t.dim != 2) return EHIP_EUNSUP;
+  std::vector<double> rhs(s->n);
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  EHIP_HIP_TRY(hipMemcpy(rhs.data(), s->rhs, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+  std::vector<double> force(3 * npe);
+  for (int e = 0; e < nb; ++e) {
+    const int *nd = belems + (size_t)e * npe;
+    std::fill(force.begin(), force.end(), 0.0);
+    for (int g = 0; g < t.ngp; ++g) {
+      double dx[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+      for (int i = 0; i < 2; ++i)
+        for (int c = 0; c < 3; ++c) {
+          double sm = 0;
+          for (int a = 0; a < npe; ++a) sm += coords[3 * (size_t)(nd[a] - 1) + c] * t.dlb[((size_t)g * npe + a) * 3 + i];
+          dx[c][i] = sm;
+        }
+      double G[2][2];
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+          double sm = 0;
+          for (int k = 0; k < 3; ++k) sm = sm + dx[k][i] * dx[k][j];
+          G[i][j] = sm;
+        }
+      const double detg = G[0][0] * G[1][1] - G[0][1] * G[1][0];
+      if (!(detg > DBL_MIN)) return EHIP_EARG;
+      const double sw = std::sqrt(detg) * t.gs[g];
+      double L[3];
+      for (int i = 0; i < 3; ++i) {
+        double sm = 0;
+        for (int k = 0; k < npe; ++k) sm += load3[i] * t.basis[(size_t)g * npe + k];
+        L[i] = sm;
+      }
+      for (int q = 0; q < npe; ++q)
+        for (int i = 0; i < 3; ++i) force[3 * q + i] = force[3 * q + i] + sw * t.basis[(size_t)g * npe + q] * L[i];
+    }
+    for (int a = 0; a < npe; ++a) {
+      const int id = s->h_perm[nd[a] - 1];
+      for (int j = 0; j < 3; ++j) {
+        const int k = 3 * (id - 1) + j;
+        rhs[k] = rhs[k] + force[3 * a + j];
+      }
+    }
+  }
+  EHIP_HIP_TRY(hipMemcpy(s->rhs, rhs.data(), sizeof(double) * s->n, hipMemcpyHostToDevice));
+  return EHIP_OK;
+}

@@ -1,0 +1,119 @@
+// Internal declarations of libehip (MI355X / gfx950).  Not part of the ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ehip.h"
+
+namespace ehip {
+
+void set_error(const char *fmt, ...);
+
+#define EHIP_HIP_TRY(call)                                                          \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::ehip::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return EHIP_EHIP;                                                             \
+    }                                                                               \
+  } while (0)
+
+// SpMV schedule: rows are cut into chunks of at most `nnz_per_block`
+// nonzeros (a longer row gets a chunk of its own).  One workgroup streams a
+// chunk's values/cols coalesced into LDS as products, then one lane per row
+// sums its row left to right (the reference's summation order).
+struct SpmvPlan {
+  int block = 256;          // workgroup size
+  int nb = 2048;            // nnz per chunk (LDS capacity in elements)
+  int nchunks = 0;
+  int grid = 0;             // launched workgroups (multiple of 8, XCD-aware)
+  int *chunk_row = nullptr; // device, nchunks+1 (0-based first row of each chunk)
+};
+
+// Device-resident Krylov scalars.  Reductions finish inside the producing
+// kernel: the last workgroup to arrive (integer ticket) sums the per-block
+// partials in a fixed order, so results are run-to-run deterministic.
+template <typename T>
+struct KStateT {
+  T rho, rho_new, alpha, beta, omega, pq, rr, rhsnorm, res, ts, tt, snorm, res_true, scale;
+  int iter, info, done, maxit, snorm_exit, rec_conv, pad0, pad1;
+  T tol, divtol;
+  unsigned int ticket[4];
+};
+using KState = KStateT<double>;
+using KState32 = KStateT<float>;
+
+struct ehip_system_impl;
+
+}  // namespace ehip
+
+struct ehip_system {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int n = 0;
+  int64_t nnz = 0;
+  int dofs = 1;
+  int nnodes = 0;
+  // host copy of the pattern, 1-based like Matrix_t
+  std::vector<int> h_rows, h_cols, h_diag, h_perm;
+  int bw_before = 0, bw_after = 0, bw_accepted = 0;
+  // device CRS, 0-based
+  int *rows = nullptr, *cols = nullptr, *diag = nullptr;
+  double *vals = nullptr, *rhs = nullptr, *x = nullptr;
+  float *vals32 = nullptr;
+  double *dscale = nullptr;  // ScaleLinearSystem's D (n)
+  bool has_values = false, has_x = false;
+  ehip::SpmvPlan plan64, plan32;
+  bool tuned64 = false, tuned32 = false;
+  // Krylov workspace
+  double *work = nullptr;  // n * nwork doubles
+  int nwork = 0;
+  float *work32 = nullptr;
+  int nwork32 = 0;
+  double *partials = nullptr;  // per-block partial sums (4 * maxgrid), FP64
+  float *partials32 = nullptr; // per-block partial sums (4 * maxgrid), FP32
+  ehip::KState *st = nullptr;    // FP64 solver scalars
+  ehip::KState32 *st32 = nullptr;// FP32 inner-solver scalars
+  ehip::KState *h_st = nullptr;  // pinned mirror
+  ehip::KState32 *h_st32 = nullptr;
+  double *hist = nullptr;        // device residual history (hist_cap)
+  int hist_cap = 0;
+  // element-to-slot gather maps (from ehip_system_from_mesh)
+  int nelem = 0, npe = 0;
+  int *elements = nullptr;        // nelem*npe, 0-based node ids (device)
+  int64_t ncontrib = 0;
+  int *slot_start = nullptr;      // nnz+1
+  int *slot_contrib = nullptr;    // ncontrib: e*ld*ld + local entry
+  int *rhs_start = nullptr;       // n+1
+  int *rhs_contrib = nullptr;     // nelem*ld: e*ld + local row
+  double *emat = nullptr;         // nelem*ld*ld element matrices (row-major)
+  double *evec = nullptr;         // nelem*ld element force vectors
+  int64_t bytes = 0;
+};
+
+namespace ehip {
+
+// allocation helpers that account for device bytes per system
+int dmalloc(ehip_system *s, void **p, size_t bytes);
+void dfree(ehip_system *s, void *p, size_t bytes);
+
+int ensure_work(ehip_system *s, int nvec);
+int ensure_work32(ehip_system *s, int nvec);
+int ensure_state(ehip_system *s);
+
+// SpMV
+int plan_spmv(ehip_system *s, SpmvPlan &plan, int block, int nb);
+int spmv_f64(ehip_system *s, const double *x, double *y);
+int spmv_f32(ehip_system *s, const float *x, float *y);
+
+// scaling (device): D from the diagonal, values *= D_i D_j, b *= D
+int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double *bnorm_out,
+                 bool make_f32);
+int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale);
+
+int max_grid();
+
+}  // namespace ehip

@@ -1,0 +1,1182 @@
+// Scaling, Dirichlet rows and the Krylov solvers of libehip.
+//
+// The solve follows SolveLinearSystem's iterative branch
+// (fem/src/SolverUtils.src:7151-7525): zero-RHS exit, ScaleLinearSystem
+// (:6470-6640), x = 1e-8 when x = 0 (IterSolve.src:358), HUTI CG
+// (hutiter/src/huti_cg.src:73-244) or BiCGStab (huti_bicgstab.src:85-273)
+// with the TRESID_SCALED_BYB stop test, BackScaleLinearSystem (:6754-6866)
+// and ComputeNorm (:4887-5006).
+//
+// Every reduction finishes inside its producing kernel (last-workgroup
+// ticket, fixed summation order), every kernel reads the device-side `done`
+// flag first and returns when the solve has ended, so the host enqueues
+// blocks of iterations and polls the scalars only every `check_every`
+// iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ehip_internal.h"
+#include "linalg_kernels.h"
+#include "spmv_core.h"
+
+namespace ehip {
+
+// ---------------------------------------------------------------- scaling
+
+// D_i = 1/sqrt(|a_ii|); a zero diagonal falls back to the row's absolute
+// sum (ScaleLinearSystem, SolverUtils.src:6520-6550).
+__global__ void k_diag_scale(int n, const int *__restrict__ rows, const int *__restrict__ diag,
+                             const double *__restrict__ vals, double *__restrict__ D) {
+  const double tiny = DBL_MIN;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double d = vals[diag[i]];
+    if (fabs(d) <= tiny) {
+      double s = 0.0;
+      for (int j = rows[i]; j < rows[i + 1]; ++j) s += fabs(vals[j]);
+      d = s;
+    }
+    D[i] = fabs(d) > tiny ? 1.0 / sqrt(fabs(d)) : 1.0;
+  }
+}
+
+struct ScaleValues {
+  double *vals;
+  const int *cols;
+  const double *D;
+  float *v32;  // optional FP32 copy of the scaled values
+  __device__ void operator()(int r, int j) {
+    double v = vals[j] * (D[r] * D[cols[j]]);
+    vals[j] = v;
+    if (v32) v32[j] = (float)v;
+  }
+};
+
+struct BackscaleValues {
+  double *vals;
+  const int *cols;
+  const double *D;
+  __device__ void operator()(int r, int j) { vals[j] = vals[j] / (D[r] * D[cols[j]]); }
+};
+
+// b *= D and the sum of squares of the scaled b (ScaleLinearSystem's bnorm)
+__global__ __launch_bounds__(kVecBlock) void k_scale_rhs(int n, double *__restrict__ b,
+                                                         const double *__restrict__ D, double *partials,
+                                                         unsigned *ticket, double *out) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double v = b[i] * D[i];
+    b[i] = v;
+    acc[0] += v * v;
+  }
+  double tot[1];
+  if (last_block_reduce<double, 1>(acc, partials, gridDim.x, ticket, tot))
+    if (threadIdx.x == 0) *out = tot[0];
+}
+
+// D *= bnorm; b /= bnorm; x /= D
+__global__ void k_scale_rhs2(int n, double *b, double *D, double *x, double bnorm) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double d = D[i] * bnorm;
+    D[i] = d;
+    b[i] = b[i] / bnorm;
+    if (x) x[i] = x[i] / d;
+  }
+}
+
+// BackScaleLinearSystem vector part: x *= D; D /= bnorm; b = b / D * bnorm
+__global__ void k_backscale_vec(int n, double *b, double *D, double *x, double bnorm) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (x) x[i] = x[i] * D[i];
+    double d = D[i] / bnorm;
+    D[i] = d;
+    b[i] = b[i] / d * bnorm;
+  }
+}
+
+int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double *bnorm_out, bool make_f32) {
+  const int n = s->n;
+  int rc;
+  if ((rc = ensure_state(s))) return rc;
+  if (make_f32 && !s->vals32)
+    if ((rc = dmalloc(s, (void **)&s->vals32, sizeof(float) * s->nnz))) return rc;
+  hipLaunchKernelGGL(k_diag_scale, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rows, s->diag,
+                     s->vals, s->dscale);
+  ScaleValues f{s->vals, s->cols, s->dscale, make_f32 ? s->vals32 : nullptr};
+  hipLaunchKernelGGL(k_nonzeros<ScaleValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
+                     s->plan64.chunk_row, s->plan64.nchunks, f);
+  double bnorm = 1.0;
+  if (b) {
+    double *dsum = reinterpret_cast<double *>(&s->st->ts);
+    const int g = vec_grid(n);
+    hipLaunchKernelGGL(k_scale_rhs, dim3(g), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, s->partials,
+                       &s->st->ticket[0], dsum);
+    if (rhs_scaling) {
+      double ss = 0;
+      EHIP_HIP_TRY(hipMemcpyAsync(&ss, dsum, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+      EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+      bnorm = sqrt(ss);
+    }
+    hipLaunchKernelGGL(k_scale_rhs2, dim3(g), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, x, bnorm);
+  }
+  if (bnorm_out) *bnorm_out = bnorm;
+  EHIP_HIP_TRY(hipGetLastError());
+  return EHIP_OK;
+}
+
+int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale) {
+  const int n = s->n;
+  if (b)
+    hipLaunchKernelGGL(k_backscale_vec, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, x,
+                       rhs_scale);
+  BackscaleValues f{s->vals, s->cols, s->dscale};
+  hipLaunchKernelGGL(k_nonzeros<BackscaleValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
+                     s->plan64.chunk_row, s->plan64.nchunks, f);
+  EHIP_HIP_TRY(hipGetLastError());
+  return EHIP_OK;
+}
+
+// ---------------------------------------------------------------- Dirichlet
+
+// DefaultDirichletBCs on a scaled system without RHS scaling: every value is
+// taken through v*(D_i D_j) and back through /(D_i D_j) like the reference's
+// scale / back-scale round trip; Dirichlet rows become a unit diagonal in
+// the scaled system (ZeroRow + SetMatrixElement, SolverUtils.src:2632-2657)
+// or, in symmetric mode, their columns are eliminated into b
+// (CRS_SetSymmDirichlet, CRSMatrix.src:674-735).
+struct DirichletValues {
+  double *vals;
+  const int *cols;
+  const double *D;
+  const signed char *isbc;
+  int symmetric;
+  __device__ void operator()(int r, int j) {
+    const int c = cols[j];
+    double v = vals[j] * (D[r] * D[c]);
+    if (isbc[r]) v = (c == r) ? 1.0 : 0.0;
+    else if (symmetric && isbc[c]) v = 0.0;
+    vals[j] = v / (D[r] * D[c]);
+  }
+};
+
+// rhs: b*D, Dirichlet b = val/D, symmetric elimination b -= a_ij val_j/D_j,
+// then b/D.  `vals` are still the unscaled, pre-Dirichlet values here.
+__global__ void k_dirichlet_rhs(int n, const int *__restrict__ rows, const int *__restrict__ cols,
+                                const double *__restrict__ vals, double *b, const double *__restrict__ D,
+                                const signed char *__restrict__ isbc, const double *__restrict__ bcval,
+                                int symmetric) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double bi = b[i] * D[i];
+    if (isbc[i]) {
+      bi = bcval[i] / D[i];
+    } else if (symmetric) {
+      for (int j = rows[i]; j < rows[i + 1]; ++j) {
+        const int c = cols[j];
+        if (isbc[c]) bi = bi - vals[j] * (D[i] * D[c]) * (bcval[c] / D[c]);
+      }
+    }
+    b[i] = bi / D[i];
+  }
+}
+
+}  // namespace ehip
+
+using namespace ehip;
+
+extern "C" int ehip_dirichlet(ehip_system *s, int nbc, const int *bc_rows, const double *bc_vals, int symmetric,
+                              int scaling) {
+  if (!s || nbc < 0 || (nbc > 0 && !bc_rows)) return EHIP_EARG;
+  const int n = s->n;
+  std::vector<signed char> flag(n, 0);
+  std::vector<double> val(n, 0.0);
+  for (int k = 0; k < nbc; ++k) {
+    int r = bc_rows[k];
+    if (r < 1 || r > n) {
+      set_error("ehip_dirichlet: row %d out of range", r);
+      return EHIP_EARG;
+    }
+    flag[r - 1] = 1;
+    val[r - 1] = bc_vals ? bc_vals[k] : 0.0;
+  }
+  int rc;
+  if ((rc = ensure_work(s, 2))) return rc;
+  signed char *dflag = nullptr;
+  if ((rc = dmalloc(s, (void **)&dflag, n))) return rc;
+  double *dval = s->work;  // reuse workspace for the values
+  double *D = s->dscale;
+  EHIP_HIP_TRY(hipMemcpyAsync(dflag, flag.data(), n, hipMemcpyHostToDevice, s->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(dval, val.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+  if (scaling) {
+    hipLaunchKernelGGL(k_diag_scale, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rows, s->diag,
+                       s->vals, D);
+  } else {
+    std::vector<double> one(n, 1.0);
+    EHIP_HIP_TRY(hipMemcpyAsync(D, one.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+    EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  hipLaunchKernelGGL(k_dirichlet_rhs, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rows, s->cols,
+                     s->vals, s->rhs, D, dflag, dval, symmetric);
+  DirichletValues f{s->vals, s->cols, D, dflag, symmetric};
+  hipLaunchKernelGGL(k_nonzeros<DirichletValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
+                     s->plan64.chunk_row, s->plan64.nchunks, f);
+  EHIP_HIP_TRY(hipGetLastError());
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  dfree(s, dflag, n);
+  return EHIP_OK;
+}
+
+
+namespace ehip {
+
+// ================================================================ Krylov kernels
+
+template <typename T>
+struct Vecs {
+  T *r, *z, *p, *q, *t, *dg, *b, *x;
+  T *rtld, *v, *sv, *t1v, *t2v;  // BiCGStab
+};
+
+template <typename T>
+__device__ __forceinline__ bool solve_done(const KStateT<T> *st) {
+  return __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+#define EHIP_EPI_MEMBERS(T) \
+  Vecs<T> v;                \
+  KStateT<T> *st;           \
+  T *partials;              \
+  int stride;               \
+  unsigned *ticket;
+
+// r = b - A x ; z = r/dg ; partials r.z, r.r, b.b  (start of every solve)
+template <typename T>
+struct EpiInit {
+  static constexpr int NV = 3;
+  EHIP_EPI_MEMBERS(T)
+  __device__ void init() {}
+  __device__ void row(int i, T s, T (&acc)[3]) {
+    const T bi = v.b[i];
+    const T ri = bi - s;
+    v.r[i] = ri;
+    const T zi = ri / v.dg[i];
+    v.z[i] = zi;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+    acc[2] += bi * bi;
+  }
+  __device__ void finish(T (&tot)[3]) {
+    if (threadIdx.x == 0) {
+      st->rhsnorm = sqrt(tot[2]);
+      st->rho = tot[0];
+      st->rr = tot[1];
+      st->res = sqrt(tot[1]) / st->rhsnorm;
+      st->beta = (T)0;
+    }
+  }
+};
+
+template <typename T, int BS, int NB, class Epi>
+__global__ __launch_bounds__(BS) void k_spmv_epi(const int *__restrict__ rows, const int *__restrict__ cols,
+                                                 const T *__restrict__ vals, const T *__restrict__ xin,
+                                                 const int *__restrict__ chunk_row, int nchunks, Epi epi,
+                                                 const KStateT<T> *st) {
+  if (st && solve_done(st)) return;
+  spmv_stream_body<T, T, BS, NB>(rows, cols, vals, xin, chunk_row, nchunks, epi);
+}
+
+// ---- fast CG, 1 SpMV per iteration
+// step 1: q = A z + beta q ; p = z + beta p ; partial p.q -> alpha
+template <typename T>
+struct EpiCgFast1 {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(T)
+  T beta;
+  __device__ void init() { beta = st->beta; }
+  __device__ void row(int i, T s, T (&acc)[1]) {
+    const T qi = s + beta * v.q[i];
+    const T pi = v.z[i] + beta * v.p[i];
+    v.q[i] = qi;
+    v.p[i] = pi;
+    acc[0] += pi * qi;
+  }
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x == 0) {
+      st->pq = tot[0];
+      st->alpha = st->rho / tot[0];
+    }
+  }
+};
+
+// step 2: x += alpha p ; r -= alpha q ; z = r/dg ; partials r.z, r.r ; stop tests
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, T *partials,
+                                                        double *hist) {
+  if (solve_done(st)) return;
+  const T alpha = st->alpha;
+  T acc[2] = {(T)0, (T)0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.x[i] = v.x[i] + alpha * v.p[i];
+    const T ri = v.r[i] - alpha * v.q[i];
+    v.r[i] = ri;
+    const T zi = ri / v.dg[i];
+    v.z[i] = zi;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+  }
+  T tot[2];
+  if (last_block_reduce<T, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+    const T rho_old = st->rho;
+    st->rho = tot[0];
+    st->beta = tot[0] / rho_old;
+    st->rr = tot[1];
+    const T res = sqrt(tot[1]) / st->rhsnorm;
+    st->res = res;
+    const int it = st->iter + 1;  // iterations done
+    st->iter = it;
+    if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
+    if (res < st->tol) {
+      st->rec_conv = 1;
+      st->done = 1;
+    } else if (!(res == res) || (st->divtol > (T)0 && res > st->divtol)) {
+      st->info = 3;
+      st->done = 1;
+    } else if (tot[0] == (T)0) {
+      st->info = 20;
+      st->done = 1;
+    } else if (it >= st->maxit) {
+      st->info = 2;
+      st->done = 1;
+    }
+  }
+}
+
+// true residual t = b - A x ; restart r = t, z = t/dg ; partials t.t, t.z
+template <typename T>
+struct EpiTrue {
+  static constexpr int NV = 2;
+  EHIP_EPI_MEMBERS(T)
+  __device__ void init() {}
+  __device__ void row(int i, T s, T (&acc)[2]) {
+    const T ti = v.b[i] - s;
+    v.r[i] = ti;
+    const T zi = ti / v.dg[i];
+    v.z[i] = zi;
+    acc[0] += ti * ti;
+    acc[1] += ti * zi;
+  }
+  __device__ void finish(T (&tot)[2]) {
+    if (threadIdx.x == 0) {
+      const T rt = sqrt(tot[0]) / st->rhsnorm;
+      st->res_true = rt;
+      st->rho = tot[1];
+      st->rr = tot[0];
+      st->beta = (T)0;
+      st->done = 0;
+      st->rec_conv = 0;
+      if (rt < st->tol) {
+        st->info = 1;
+        st->done = 1;
+      } else if (st->iter >= st->maxit) {
+        st->info = 2;
+        st->done = 1;
+      }
+    }
+  }
+};
+
+// ---- faithful CG in huti_cg.src order; per iteration k:
+//   Kc : q = A p ; alpha = rho / (p.q)
+//   Kda: x += alpha p ; r -= alpha q ; z = r/dg ; rho_new = r.z
+//   Ke : t = A x - b ; residual ||t||/||b|| ; p = z + (rho_new/rho) p
+template <typename T>
+struct EpiCgQ {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(T)
+  __device__ void init() {}
+  __device__ void row(int i, T s, T (&acc)[1]) {
+    v.q[i] = s;
+    acc[0] += v.p[i] * s;
+  }
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x == 0) {
+      st->pq = tot[0];
+      st->alpha = st->rho / tot[0];
+    }
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T> *st, T *partials) {
+  if (solve_done(st)) return;
+  const T alpha = st->alpha;
+  T acc[1] = {(T)0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.x[i] = v.x[i] + alpha * v.p[i];
+    const T ri = v.r[i] - alpha * v.q[i];
+    v.r[i] = ri;
+    const T zi = ri / v.dg[i];
+    v.z[i] = zi;
+    acc[0] += ri * zi;
+  }
+  T tot[1];
+  if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0)
+    st->rho_new = tot[0];
+}
+
+template <typename T>
+struct EpiCgResidual {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(T)
+  double *hist;
+  T beta;
+  __device__ void init() { beta = st->rho_new / st->rho; }
+  __device__ void row(int i, T s, T (&acc)[1]) {
+    const T ti = s - v.b[i];
+    acc[0] += ti * ti;
+    v.p[i] = v.z[i] + beta * v.p[i];
+  }
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x != 0) return;
+    const T res = sqrt(tot[0]) / st->rhsnorm;
+    st->res = res;
+    const int it = st->iter;  // HUTI iter_count, 1-based
+    if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
+    if (res < st->tol) {
+      st->info = 1;
+      st->done = 1;
+      return;
+    }
+    st->rho = st->rho_new;
+    st->iter = it + 1;
+    if (it + 1 > st->maxit) {
+      st->info = 2;
+      st->done = 1;
+    } else if (st->rho_new == (T)0) {
+      st->info = 20;
+      st->done = 1;
+    }
+  }
+};
+
+// ---- BiCGStab in huti_bicgstab.src order; per iteration k:
+//   B3 : v = A t1v ; alpha = rho / (rtld.v)
+//   B45: s = r - alpha v ; t2v = s/dg ; ||s|| (s-norm exit below HUTI_EPSILON)
+//   B6 : t = A t2v ; omega = (t.s)/(t.t)
+//   B71: x += alpha t1v + omega t2v ; r = s - omega t ; rho_new = rtld.r
+//   B82: faithful: residual ||A x - b||/||b|| fused with p = r + beta (p - omega v), t1v = p/dg
+//        fast    : residual ||r||/||b|| inside B71, p update as a vector kernel
+template <typename T>
+struct EpiBiV {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(T)
+  __device__ void init() {}
+  __device__ void row(int i, T s, T (&acc)[1]) {
+    v.v[i] = s;
+    acc[0] += v.rtld[i] * s;
+  }
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x == 0) st->alpha = st->rho / tot[0];
+  }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bi_s(int n, Vecs<T> v, KStateT<T> *st, T *partials) {
+  if (solve_done(st)) return;
+  const T alpha = st->alpha;
+  T acc[1] = {(T)0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const T si = v.r[i] - alpha * v.v[i];
+    v.sv[i] = si;
+    v.t2v[i] = si / v.dg[i];
+    acc[0] += si * si;
+  }
+  T tot[1];
+  if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+    const T sn = sqrt(tot[0]);
+    st->snorm = sn;
+    if (sn < (T)1.17549435E-38f) {  // HUTI_EPSILON (huti_fdefs.h)
+      st->snorm_exit = 1;
+      st->info = 36;
+    }
+  }
+}
+
+template <typename T>
+struct EpiBiT {
+  static constexpr int NV = 2;
+  EHIP_EPI_MEMBERS(T)
+  int active;
+  __device__ void init() { active = !st->snorm_exit; }
+  __device__ void row(int i, T s, T (&acc)[2]) {
+    v.t[i] = s;
+    acc[0] += s * v.sv[i];
+    acc[1] += s * s;
+  }
+  __device__ void finish(T (&tot)[2]) {
+    if (threadIdx.x == 0) {
+      st->ts = tot[0];
+      st->tt = tot[1];
+      st->omega = tot[0] / tot[1];
+    }
+  }
+};
+
+// s-norm exit: x += alpha t1v and finish
+template <typename T>
+__global__ void k_bi_snorm_exit(int n, Vecs<T> v, KStateT<T> *st) {
+  if (solve_done(st) || !st->snorm_exit) return;
+  const T alpha = st->alpha;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v.x[i] = v.x[i] + alpha * v.t1v[i];
+}
+template <typename T>
+__global__ void k_bi_snorm_mark(KStateT<T> *st) {
+  if (!st->done && st->snorm_exit) st->done = 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bi_xr(int n, Vecs<T> v, KStateT<T> *st, T *partials, int fast,
+                                                     double *hist) {
+  if (solve_done(st)) return;
+  const T alpha = st->alpha, omega = st->omega;
+  T acc[2] = {(T)0, (T)0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.x[i] = v.x[i] + alpha * v.t1v[i] + omega * v.t2v[i];
+    const T ri = v.sv[i] - omega * v.t[i];
+    v.r[i] = ri;
+    acc[0] += v.rtld[i] * ri;
+    acc[1] += ri * ri;
+  }
+  T tot[2];
+  if (last_block_reduce<T, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+    st->rho_new = tot[0];
+    st->rr = tot[1];
+    if (fast) {
+      const T res = sqrt(tot[1]) / st->rhsnorm;
+      st->res = res;
+      const int it = st->iter;
+      if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
+      if (res < st->tol) {
+        st->rec_conv = 1;
+        st->done = 1;
+        return;
+      }
+      if (!(res == res) || (st->divtol > (T)0 && res > st->divtol)) {
+        st->info = 3;
+        st->done = 1;
+        return;
+      }
+      if (omega == (T)0) {
+        st->info = 37;
+        st->done = 1;
+        return;
+      }
+      st->iter = it + 1;
+      if (it + 1 > st->maxit) {
+        st->info = 2;
+        st->done = 1;
+      } else if (tot[0] == (T)0) {
+        st->info = 35;
+        st->done = 1;
+      } else {
+        st->beta = (tot[0] * alpha) / (st->rho * omega);
+        st->rho = tot[0];
+      }
+    }
+  }
+}
+
+template <typename T>
+struct EpiBiResidual {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(T)
+  double *hist;
+  T beta, omega;
+  __device__ void init() {
+    omega = st->omega;
+    beta = (st->rho_new * st->alpha) / (st->rho * omega);
+  }
+  __device__ void row(int i, T s, T (&acc)[1]) {
+    const T ti = s - v.b[i];
+    acc[0] += ti * ti;
+    const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
+    v.p[i] = pi;
+    v.t1v[i] = pi / v.dg[i];
+  }
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x != 0) return;
+    const T res = sqrt(tot[0]) / st->rhsnorm;
+    st->res = res;
+    const int it = st->iter;
+    if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
+    if (res < st->tol) {
+      st->info = 1;
+      st->done = 1;
+      return;
+    }
+    if (st->omega == (T)0) {
+      st->info = 37;
+      st->done = 1;
+      return;
+    }
+    st->iter = it + 1;
+    if (it + 1 > st->maxit) {
+      st->info = 2;
+      st->done = 1;
+    } else if (st->rho_new == (T)0) {
+      st->info = 35;
+      st->done = 1;
+    } else {
+      st->rho = st->rho_new;
+    }
+  }
+};
+
+// fast BiCGStab p update: p = r + beta (p - omega v) ; t1v = p/dg
+template <typename T>
+__global__ void k_bi_p(int n, Vecs<T> v, KStateT<T> *st) {
+  if (solve_done(st)) return;
+  const T beta = st->beta, omega = st->omega;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
+    v.p[i] = pi;
+    v.t1v[i] = pi / v.dg[i];
+  }
+}
+
+// BiCGStab start: rtld = r ; first p = r (beta = 0, p = v = 0) ; t1v = p/dg ; rho = rtld.r
+template <typename T>
+__global__ __launch_bounds__(kVecBlock) void k_bi_start(int n, Vecs<T> v, KStateT<T> *st, T *partials) {
+  if (solve_done(st)) return;
+  T acc[1] = {(T)0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const T ri = v.r[i];
+    v.rtld[i] = ri;
+    v.p[i] = ri;
+    v.v[i] = (T)0;
+    v.t1v[i] = ri / v.dg[i];
+    acc[0] += ri * ri;
+  }
+  T tot[1];
+  if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+    st->rho = tot[0];
+    st->omega = (T)1;
+    st->alpha = (T)0;
+    if (tot[0] == (T)0) {
+      st->info = 35;
+      st->done = 1;
+    }
+  }
+}
+
+// dg = a_ii, or 1 when |a_ii| <= AEPS or without preconditioning (CRS_DiagPrecondition)
+template <typename T>
+__global__ void k_make_dg(int n, const int *__restrict__ diag, const double *__restrict__ vals, T *dg, int precond) {
+  const double AEPS = 10.0 * DBL_EPSILON;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double d = vals[diag[i]];
+    dg[i] = (T)((precond && fabs(d) > AEPS) ? d : 1.0);
+  }
+}
+
+template <typename T>
+__global__ void k_fill(int n, T *x, T val) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = val;
+}
+
+template <typename T>
+__global__ void k_zero2(int n, T *a, T *b) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    a[i] = (T)0;
+    b[i] = (T)0;
+  }
+}
+
+template <typename T>
+__global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int iter0) {
+  st->tol = tol;
+  st->maxit = maxit;
+  st->divtol = divtol;
+  st->iter = iter0;
+  st->info = 0;
+  st->done = 0;
+  st->snorm_exit = 0;
+  st->rec_conv = 0;
+  st->beta = (T)0;
+  st->res_true = (T)-1;
+  st->pad0 = 0;
+  st->pad1 = 0;
+}
+
+// ---- mixed precision
+// r32 = t/||t|| ; z32 = r32/dg32 ; d32 = 0 ; rho, rr in FP32
+__global__ __launch_bounds__(kVecBlock) void k_to32(int n, const double *__restrict__ t, double inv, float *r,
+                                                    float *z, float *d, const float *__restrict__ dg,
+                                                    KState32 *st, float *partials) {
+  float acc[2] = {0.f, 0.f};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float ri = (float)(t[i] * inv);
+    r[i] = ri;
+    const float zi = ri / dg[i];
+    z[i] = zi;
+    d[i] = 0.f;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+  }
+  float tot[2];
+  if (last_block_reduce<float, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+    st->rho = tot[0];
+    st->rr = tot[1];
+    st->rhsnorm = sqrtf(tot[1]);
+    st->beta = 0.f;
+    st->iter = 0;
+    st->done = 0;
+    st->info = 0;
+    st->rec_conv = 0;
+  }
+}
+
+// x64 += scale * d32
+__global__ void k_correct(int n, double *x, const float *__restrict__ d, double scale) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    x[i] = x[i] + scale * (double)d[i];
+}
+
+// FP64 outer residual: t = b - A x ; partial t.t -> res_true, scale = ||t||
+struct EpiOuter {
+  static constexpr int NV = 1;
+  double *b, *t;
+  KState *st;
+  double *partials;
+  int stride;
+  unsigned *ticket;
+  double *hist;
+  __device__ void init() {}
+  __device__ void row(int i, double s, double (&acc)[1]) {
+    const double ti = b[i] - s;
+    t[i] = ti;
+    acc[0] += ti * ti;
+  }
+  __device__ void finish(double (&tot)[1]) {
+    if (threadIdx.x != 0) return;
+    const double nt = sqrt(tot[0]);
+    st->scale = nt;
+    const double res = nt / st->rhsnorm;
+    st->res_true = res;
+    st->res = res;
+    const int k = st->pad0;  // outer step
+    if (hist && k < st->maxit) hist[k] = res;
+    st->pad0 = k + 1;
+    if (res < st->tol) {
+      st->info = 1;
+      st->done = 1;
+    } else if (!(res == res)) {
+      st->info = 3;
+      st->done = 1;
+    }
+  }
+};
+
+// sum of squares of b and of b - A x (unscaled report residual)
+struct EpiResid {
+  static constexpr int NV = 2;
+  const double *b;
+  double *partials;
+  int stride;
+  unsigned *ticket;
+  double *out;
+  __device__ void init() {}
+  __device__ void row(int i, double s, double (&acc)[2]) {
+    const double t = b[i] - s;
+    acc[0] += t * t;
+    acc[1] += b[i] * b[i];
+  }
+  __device__ void finish(double (&tot)[2]) {
+    if (threadIdx.x == 0) {
+      out[0] = tot[0];
+      out[1] = tot[1];
+    }
+  }
+};
+
+__global__ __launch_bounds__(kVecBlock) void k_sumsq(int n, const double *x, double *partials, unsigned *ticket,
+                                                     double *out) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc[0] += x[i] * x[i];
+  double tot[1];
+  if (last_block_reduce<double, 1>(acc, partials, gridDim.x, ticket, tot) && threadIdx.x == 0) *out = tot[0];
+}
+
+// ================================================================ host drivers
+
+template <typename T, class Epi>
+static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
+                       const KStateT<T> *st) {
+#define EHIP_EPI_CASE(BS, NB)                                                                              \
+  if (p.block == BS && p.nb == NB) {                                                                       \
+    hipLaunchKernelGGL((k_spmv_epi<T, BS, NB, Epi>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows, s->cols, \
+                       vals, xin, p.chunk_row, p.nchunks, e, st);                                          \
+    return;                                                                                                \
+  }
+  EHIP_EPI_CASE(256, 2048)
+  EHIP_EPI_CASE(256, 1024)
+  EHIP_EPI_CASE(128, 1024)
+  EHIP_EPI_CASE(512, 4096)
+  EHIP_EPI_CASE(256, 4096)
+#undef EHIP_EPI_CASE
+}
+
+template <typename T>
+static int read_state(ehip_system *s, KStateT<T> *dst, KStateT<T> *hst) {
+  EHIP_HIP_TRY(hipMemcpyAsync(hst, dst, sizeof(KStateT<T>), hipMemcpyDeviceToHost, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  return EHIP_OK;
+}
+
+struct SolveCtx {
+  int method, precond, maxit, check, mode;
+  double tol, divtol;
+  double *hist;  // device, may be null
+  int spmv;
+};
+
+template <typename T>
+static Vecs<T> make_vecs(T *w, int n, int method, T *x, T *b) {
+  Vecs<T> v{};
+  const size_t N = (size_t)n;
+  v.x = x;
+  v.b = b;
+  if (method == EHIP_CG) {
+    v.r = w; v.z = w + N; v.p = w + 2 * N; v.q = w + 3 * N; v.t = w + 4 * N; v.dg = w + 5 * N;
+  } else {
+    v.r = w; v.rtld = w + N; v.p = w + 2 * N; v.t1v = w + 3 * N; v.v = w + 4 * N;
+    v.sv = w + 5 * N; v.t2v = w + 6 * N; v.t = w + 7 * N; v.dg = w + 8 * N; v.z = w + 9 * N;
+  }
+  return v;
+}
+static int nvecs(int method) { return method == EHIP_CG ? 6 : 10; }
+
+// Krylov iterations on a scaled system in precision T.  For T = double the
+// vectors are s->x / s->rhs; for the FP32 inner solve of the mixed form they
+// are the caller's.  Returns with the final state in hst.
+template <typename T>
+static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *vals, Vecs<T> v,
+                  KStateT<T> *st, KStateT<T> *hst, T *part, bool init_residual) {
+  const int n = s->n;
+  const int g = vec_grid(n);
+  const int pg = max_grid();
+  int rc;
+  const int iter0 = (c.mode == EHIP_FAITHFUL || c.method == EHIP_BICGSTAB) ? 1 : 0;
+  if (init_residual) {
+    hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0);
+    EpiInit<T> e{v, st, part, pg, &st->ticket[0]};
+    launch_epi<T>(s, plan, vals, v.x, e, (const KStateT<T> *)nullptr);
+    c.spmv++;
+  } else {
+    // mixed inner solve: k_to32 prepared r, z, rho; x (= d) is zero
+    hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0);
+  }
+  if (c.method == EHIP_CG) {
+    if (c.mode == EHIP_FAITHFUL) {
+      EHIP_HIP_TRY(hipMemcpyAsync(v.p, v.z, sizeof(T) * n, hipMemcpyDeviceToDevice, s->stream));
+      for (;;) {
+        for (int k = 0; k < c.check; ++k) {
+          EpiCgQ<T> e1{v, st, part, pg, &st->ticket[0]};
+          launch_epi<T>(s, plan, vals, v.p, e1, st);
+          hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, (T)0};
+          launch_epi<T>(s, plan, vals, v.x, e2, st);
+          c.spmv += 2;
+        }
+        if ((rc = read_state(s, st, hst))) return rc;
+        if (hst->done) break;
+      }
+    } else {
+      hipLaunchKernelGGL(k_zero2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v.p, v.q);
+      for (;;) {
+        for (int k = 0; k < c.check; ++k) {
+          EpiCgFast1<T> e1{v, st, part, pg, &st->ticket[0], (T)0};
+          launch_epi<T>(s, plan, vals, v.z, e1, st);
+          hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part, c.hist);
+          c.spmv++;
+        }
+        if ((rc = read_state(s, st, hst))) return rc;
+        if (!hst->done) continue;
+        if (!hst->rec_conv || !init_residual) break;  // inner solves stop on the recurrence
+        // recurrence converged: confirm with the true residual, restart from it otherwise
+        EpiTrue<T> et{v, st, part, pg, &st->ticket[0]};
+        launch_epi<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
+        c.spmv++;
+        if ((rc = read_state(s, st, hst))) return rc;
+        if (hst->done) break;
+      }
+    }
+  } else {
+    hipLaunchKernelGGL(k_bi_start<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+    for (;;) {
+      for (int k = 0; k < c.check; ++k) {
+        EpiBiV<T> e3{v, st, part, pg, &st->ticket[0]};
+        launch_epi<T>(s, plan, vals, v.t1v, e3, st);
+        hipLaunchKernelGGL(k_bi_s<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+        hipLaunchKernelGGL(k_bi_snorm_exit<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
+        hipLaunchKernelGGL(k_bi_snorm_mark<T>, dim3(1), dim3(1), 0, s->stream, st);
+        EpiBiT<T> e6{v, st, part, pg, &st->ticket[0], 1};
+        launch_epi<T>(s, plan, vals, v.t2v, e6, st);
+        hipLaunchKernelGGL(k_bi_xr<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part,
+                           c.mode == EHIP_FAST ? 1 : 0, c.hist);
+        c.spmv += 2;
+        if (c.mode == EHIP_FAITHFUL) {
+          EpiBiResidual<T> e8{v, st, part, pg, &st->ticket[2], c.hist, (T)0, (T)0};
+          launch_epi<T>(s, plan, vals, v.x, e8, st);
+          c.spmv++;
+        } else {
+          hipLaunchKernelGGL(k_bi_p<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
+        }
+      }
+      if ((rc = read_state(s, st, hst))) return rc;
+      if (!hst->done) continue;
+      if (c.mode == EHIP_FAITHFUL || !hst->rec_conv || !init_residual) break;
+      EpiTrue<T> et{v, st, part, pg, &st->ticket[0]};
+      launch_epi<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
+      c.spmv++;
+      if ((rc = read_state(s, st, hst))) return rc;
+      if (hst->done) break;
+      hipLaunchKernelGGL(k_bi_start<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+    }
+  }
+  EHIP_HIP_TRY(hipGetLastError());
+  return EHIP_OK;
+}
+
+// Mixed precision: FP32 values and vectors, FP32-accumulated inner Krylov
+// (fast mode) inside an FP64 iterative refinement on the scaled system.
+// FP64 stays on the device only for x, the true residual and its norm.
+static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_maxit, int *inner_total) {
+  const int n = s->n;
+  int rc;
+  const int nv = nvecs(c.method);
+  if ((rc = ensure_work(s, 1))) return rc;
+  if ((rc = ensure_work32(s, nv + 1))) return rc;
+  double *t64 = s->work;
+  Vecs<float> v = make_vecs<float>(s->work32, n, c.method, s->work32 + (size_t)nv * n, nullptr);
+  v.b = v.r;  // the inner right-hand side is the normalised residual held in r
+  const int g = vec_grid(n);
+  const int pg = max_grid();
+  hipLaunchKernelGGL(k_make_dg<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, v.dg,
+                     c.precond);
+  KState *st = s->st;
+  // ||b|| of the scaled system and the outer state
+  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0);
+  hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->rhs, s->partials, &st->ticket[1],
+                     &st->rr);
+  if ((rc = read_state(s, st, s->h_st))) return rc;
+  const double bn = sqrt(s->h_st->rr);
+  EHIP_HIP_TRY(hipMemcpyAsync(&st->rhsnorm, &bn, sizeof(double), hipMemcpyHostToDevice, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  *inner_total = 0;
+  int outer = 0;
+  SolveCtx ci = c;
+  ci.mode = EHIP_FAST;
+  ci.hist = nullptr;
+  ci.tol = inner_tol;
+  ci.divtol = 0;
+  for (;;) {
+    EpiOuter eo{s->rhs, t64, st, s->partials, pg, &st->ticket[0], c.hist};
+    launch_epi<double>(s, s->plan64, s->vals, s->x, eo, (const KState *)nullptr);
+    c.spmv++;
+    if ((rc = read_state(s, st, s->h_st))) return rc;
+    if (s->h_st->done) break;
+    if (*inner_total >= c.maxit) break;
+    const double scale = s->h_st->scale;
+    hipLaunchKernelGGL(k_to32, dim3(g), dim3(kVecBlock), 0, s->stream, n, t64, 1.0 / scale, v.r, v.z, v.x, v.dg,
+                       s->st32, s->partials32);
+    ci.maxit = std::min(inner_maxit, c.maxit - *inner_total);
+    ci.spmv = 0;
+    if ((rc = krylov<float>(s, ci, s->plan32, s->vals32, v, s->st32, s->h_st32, s->partials32, false))) return rc;
+    c.spmv += ci.spmv;
+    *inner_total += s->h_st32->iter;
+    if (s->h_st32->info == 3 || s->h_st32->iter == 0) {
+      // inner breakdown without progress: stop with the outer residual
+      break;
+    }
+    hipLaunchKernelGGL(k_correct, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->x, v.x, scale);
+    ++outer;
+  }
+  if ((rc = read_state(s, st, s->h_st))) return rc;
+  if (!s->h_st->done) {
+    // ran out of iterations
+    KState h = *s->h_st;
+    h.info = 2;
+    h.done = 1;
+    *s->h_st = h;
+  }
+  s->h_st->pad1 = outer;
+  s->h_st->iter = *inner_total;
+  return EHIP_OK;
+}
+
+}  // namespace ehip
+
+// ================================================================ C ABI
+
+extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist) {
+  if (!s || !p) return EHIP_EARG;
+  if (!s->has_values) {
+    set_error("ehip_solve: the system has no values (assemble or set them first)");
+    return EHIP_ESTATE;
+  }
+  if (p->method != EHIP_CG && p->method != EHIP_BICGSTAB) return EHIP_EARG;
+  if (p->maxit < 1 || !(p->tol > 0)) {
+    set_error("ehip_solve: maxit must be >= 1 and tol > 0");
+    return EHIP_EARG;
+  }
+  const int n = s->n;
+  int rc;
+  if ((rc = ensure_state(s))) return rc;
+  hipEvent_t e0, e1, e2, e3;
+  EHIP_HIP_TRY(hipEventCreate(&e0));
+  EHIP_HIP_TRY(hipEventCreate(&e1));
+  EHIP_HIP_TRY(hipEventCreate(&e2));
+  EHIP_HIP_TRY(hipEventCreate(&e3));
+  ehip_report rep{};
+  EHIP_HIP_TRY(hipEventRecord(e0, s->stream));
+  // initial guess
+  bool xzero = true;
+  if (x) {
+    for (int i = 0; i < n && xzero; ++i) xzero = (x[i] == 0.0);
+    EHIP_HIP_TRY(hipMemcpyAsync(s->x, x, sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+  } else {
+    EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
+  }
+  // bnorm and the zero right-hand side exit (SolverUtils.src:7324-7341)
+  hipLaunchKernelGGL(k_sumsq, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rhs, s->partials,
+                     &s->st->ticket[1], &s->st->rr);
+  if ((rc = read_state(s, s->st, s->h_st))) return rc;
+  const double b2 = s->h_st->rr;
+  if (!(sqrt(b2) > DBL_MIN)) {
+    EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
+    if (x) std::memset(x, 0, sizeof(double) * n);
+    rep.info = 1;
+    rep.iters = 0;
+    rep.norm = 0.0;
+    if (r) *r = rep;
+    s->has_x = true;
+    return EHIP_OK;
+  }
+  const bool scaling = p->scaling != 0;
+  const bool mixed = p->precision == EHIP_MIXED_IR;
+  double bscale = 1.0;
+  // keep the unscaled b for the report residual
+  if ((rc = ensure_work(s, mixed ? 2 : nvecs(p->method) + 1))) return rc;
+  double *b0 = s->work + (size_t)(s->nwork - 1) * n;
+  EHIP_HIP_TRY(hipMemcpyAsync(b0, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+  if (scaling) {
+    if ((rc = scale_system(s, s->rhs, true, s->x, &bscale, mixed))) return rc;
+  } else if (mixed) {
+    if (!s->vals32)
+      if ((rc = dmalloc(s, (void **)&s->vals32, sizeof(float) * s->nnz))) return rc;
+    std::vector<double> one(n, 1.0);
+    EHIP_HIP_TRY(hipMemcpy(s->dscale, one.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    ScaleValues f{s->vals, s->cols, s->dscale, s->vals32};
+    hipLaunchKernelGGL(k_nonzeros<ScaleValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
+                       s->plan64.chunk_row, s->plan64.nchunks, f);
+  }
+  if (xzero)
+    hipLaunchKernelGGL(k_fill<double>, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, 1.0e-8);
+  // history buffer
+  double *dhist = nullptr;
+  if (p->record_history) {
+    if (s->hist_cap < p->maxit + 2) {
+      if (s->hist) dfree(s, s->hist, sizeof(double) * s->hist_cap);
+      s->hist = nullptr;
+      s->hist_cap = 0;
+      if ((rc = dmalloc(s, (void **)&s->hist, sizeof(double) * (p->maxit + 2)))) return rc;
+      s->hist_cap = p->maxit + 2;
+    }
+    dhist = s->hist;
+  }
+  SolveCtx c{};
+  c.method = p->method;
+  c.precond = p->precond;
+  c.maxit = p->maxit;
+  c.mode = p->mode;
+  c.tol = p->tol;
+  c.divtol = p->divtol;
+  c.check = p->check_every > 0 ? p->check_every : (n > 200000 ? 25 : 10);
+  c.hist = dhist;
+  EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
+  int inner_total = 0;
+  if (mixed) {
+    const double itol = p->inner_tol > 0 ? p->inner_tol : 1e-4;
+    const int imax = p->inner_maxit > 0 ? p->inner_maxit : 1000;
+    // FP32 vectors are the workspace; the FP64 workspace holds the residual
+    if ((rc = mixed_ir(s, c, itol, imax, &inner_total))) return rc;
+    rep.outer_iters = s->h_st->pad1;
+  } else {
+    Vecs<double> v = make_vecs<double>(s->work, n, p->method, s->x, s->rhs);
+    hipLaunchKernelGGL(k_make_dg<double>, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals,
+                       v.dg, p->precond);
+    if ((rc = krylov<double>(s, c, s->plan64, s->vals, v, s->st, s->h_st, s->partials, true))) return rc;
+  }
+  EHIP_HIP_TRY(hipEventRecord(e2, s->stream));
+  const KState hs = *s->h_st;
+  rep.info = hs.info;
+  rep.iters = hs.iter;
+  rep.res_scaled = hs.res_true >= 0 ? hs.res_true : hs.res;
+  if (p->record_history && hist) {
+    int k = mixed ? rep.outer_iters + 1 : std::min(hs.iter, p->maxit);
+    if (k > 0) EHIP_HIP_TRY(hipMemcpy(hist, dhist, sizeof(double) * k, hipMemcpyDeviceToHost));
+  }
+  if (scaling)
+    if ((rc = backscale_system(s, s->rhs, s->x, bscale))) return rc;
+  // unscaled FP64 true residual ||b - A x|| / ||b|| and ComputeNorm
+  {
+    double *out = reinterpret_cast<double *>(&s->st->ts);
+    EpiResid er{b0, s->partials, max_grid(), &s->st->ticket[0], out};
+    launch_epi<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
+    hipLaunchKernelGGL(k_sumsq, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, s->partials,
+                       &s->st->ticket[1], &s->st->rr);
+    if ((rc = read_state(s, s->st, s->h_st))) return rc;
+    rep.res_true = sqrt(s->h_st->ts) / sqrt(s->h_st->tt);
+    rep.norm = sqrt(s->h_st->rr / (double)n);
+  }
+  EHIP_HIP_TRY(hipEventRecord(e3, s->stream));
+  if (x) EHIP_HIP_TRY(hipMemcpyAsync(x, s->x, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  float tk = 0, tt = 0;
+  (void)hipEventElapsedTime(&tk, e1, e2);
+  (void)hipEventElapsedTime(&tt, e0, e3);
+  rep.t_krylov_ms = tk;
+  rep.t_total_ms = tt;
+  rep.spmv_count = c.spmv + 1;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  (void)hipEventDestroy(e3);
+  s->has_x = true;
+  if (r) *r = rep;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_get_solution(ehip_system *s, double *x) {
+  if (!s || !x) return EHIP_EARG;
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  EHIP_HIP_TRY(hipMemcpy(x, s->x, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+  return EHIP_OK;
+}
+
+extern "C" int ehip_solve_csr(int n, const int *rows, const int *cols, const int *diag, const double *values,
+                              const double *b, double *x, const ehip_params *p, ehip_report *r) {
+  ehip_system *s = nullptr;
+  int rc = ehip_system_create(n, (int64_t)rows[n] - 1, rows, cols, diag, &s);
+  if (rc) return rc;
+  rc = ehip_system_set_values(s, values, b);
+  if (!rc) rc = ehip_solve(s, p, x, r, nullptr);
+  ehip_system_free(s);
+  return rc;
+}

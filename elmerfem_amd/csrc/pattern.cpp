@@ -1,0 +1,330 @@
+// Host CRS topology for libehip: node adjacency, bandwidth ordering, CRS
+// rows/cols/diag and the element-to-slot gather maps.  Integer work on the
+// host, parallel over nodes with OpenMP; runs once per mesh.
+#include "pattern.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <numeric>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace ehip {
+
+namespace {
+
+struct Adjacency {
+  std::vector<int64_t> start;  // nnodes+2, entries for node i (1-based) at [start[i], start[i+1])
+  std::vector<int> idx;        // sorted unique neighbour nodes, self included (1-based)
+};
+
+// node -> elements incidence (elements in ascending order)
+struct Incidence {
+  std::vector<int64_t> start;  // nnodes+2
+  std::vector<int> elem;       // element ids (0-based), ascending per node
+  std::vector<int> local;      // local node index inside that element
+};
+
+Incidence incidence(int nnodes, int nelem, int npe, const int *el) {
+  Incidence I;
+  I.start.assign((size_t)nnodes + 2, 0);
+  for (int64_t k = 0; k < (int64_t)nelem * npe; ++k) I.start[el[k] + 1]++;
+  for (int i = 1; i <= nnodes + 1; ++i) I.start[i] += I.start[i - 1];
+  I.elem.resize((size_t)nelem * npe);
+  I.local.resize((size_t)nelem * npe);
+  std::vector<int64_t> fill(I.start.begin(), I.start.end());
+  for (int e = 0; e < nelem; ++e)
+    for (int a = 0; a < npe; ++a) {
+      const int nd = el[(int64_t)e * npe + a];
+      const int64_t pos = fill[nd]++;
+      I.elem[pos] = e;
+      I.local[pos] = a;
+    }
+  // start is indexed by 1-based node id: node i occupies [start[i], start[i+1])
+  return I;
+}
+
+// MakeListMatrix (ElementUtils.src:279-620): every pair of nodes sharing an
+// element, each list sorted and without duplicates (List_GetMatrixIndex).
+Adjacency adjacency(int nnodes, int nelem, int npe, const int *el, int nbnd, int npe_b, const int *bel) {
+  Incidence I = incidence(nnodes, nelem, npe, el);
+  Incidence B;
+  if (nbnd > 0) B = incidence(nnodes, nbnd, npe_b, bel);
+  Adjacency A;
+  std::vector<std::vector<int>> lists((size_t)nnodes + 1);
+  std::vector<int64_t> deg((size_t)nnodes + 2, 0);
+#pragma omp parallel
+  {
+    std::vector<int> buf;
+#pragma omp for schedule(dynamic, 1024)
+    for (int i = 1; i <= nnodes; ++i) {
+      buf.clear();
+      for (int64_t k = I.start[i]; k < I.start[i + 1]; ++k) {
+        const int *nd = el + (int64_t)I.elem[k] * npe;
+        buf.insert(buf.end(), nd, nd + npe);
+      }
+      if (nbnd > 0)
+        for (int64_t k = B.start[i]; k < B.start[i + 1]; ++k) {
+          const int *nd = bel + (int64_t)B.elem[k] * npe_b;
+          buf.insert(buf.end(), nd, nd + npe_b);
+        }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      lists[i] = buf;
+      deg[i + 1] = (int64_t)buf.size();
+    }
+  }
+  A.start.assign((size_t)nnodes + 2, 0);
+  for (int i = 1; i <= nnodes; ++i) A.start[i + 1] = A.start[i] + deg[i + 1];
+  A.idx.resize((size_t)A.start[nnodes + 1]);
+#pragma omp parallel for schedule(static)
+  for (int i = 1; i <= nnodes; ++i) std::copy(lists[i].begin(), lists[i].end(), A.idx.begin() + A.start[i]);
+  return A;
+}
+
+// ComputeBandwidth (BandwidthOptimize.src:60-90)
+int bandwidth(int n, const Adjacency &A, const std::vector<int> *reorder) {
+  int hb = 0;
+#pragma omp parallel for reduction(max : hb) schedule(static)
+  for (int i = 1; i <= n; ++i)
+    for (int64_t c = A.start[i]; c < A.start[i + 1]; ++c) {
+      const int k = A.idx[c];
+      const int d = reorder ? std::abs((*reorder)[i] - (*reorder)[k]) : std::abs(i - k);
+      hb = std::max(hb, d);
+    }
+  return hb;
+}
+
+// Levelize (BandwidthOptimize.src:290-350): the reference's depth-first walk
+// with an explicit stack of list positions; `level` records the depth at
+// which each node is first reached and maxlevel the deepest one.
+void levelize(int root, const Adjacency &A, std::vector<int> &level, std::vector<char> &done, int &maxlevel) {
+  struct Pos { int64_t p, end; };
+  std::vector<Pos> stack;
+  stack.reserve(512);
+  int node = root, lev = 0;
+  int64_t end = A.start[node + 1];
+  int64_t p = A.start[node] < end ? A.start[node] : -1;
+  while (p >= 0) {
+    stack.push_back({p, end});
+    level[node] = lev;
+    done[node] = 1;
+    maxlevel = std::max(maxlevel, lev);
+    end = A.start[node + 1];
+    p = A.start[node] < end ? A.start[node] : -1;
+    for (;;) {
+      if (p >= 0) {
+        node = A.idx[p];
+        if (!done[node]) { ++lev; break; }
+      } else if (!stack.empty()) {
+        p = stack.back().p;
+        end = stack.back().end;
+        stack.pop_back();
+        --lev;
+      } else {
+        break;
+      }
+      p = (p + 1 < end) ? p + 1 : -1;
+    }
+  }
+}
+
+// OptimizeBandwidth (BandwidthOptimize.src:108-285): minimum-degree start,
+// pseudo-peripheral root search over Levelize depths, Cuthill-McKee
+// numbering over the sorted lists, reversal, and rejection when the half
+// bandwidth grows.  Also reproduces the reference's assignment of the old
+// depth to StartNode when a better root is found (:170-176).
+void optimize_bandwidth(int n, const Adjacency &A, std::vector<int> &perm, int &before, int &after, int &accepted) {
+  std::vector<int> degree(n + 1), level(n + 1, 0);
+  std::vector<char> done(n + 1, 0);
+  for (int i = 1; i <= n; ++i) degree[i] = (int)(A.start[i + 1] - A.start[i]);
+  before = bandwidth(n, A, nullptr) + 1;
+  int start = 1, mindeg = degree[1];
+  for (int i = 1; i <= n; ++i)
+    if (degree[i] < mindeg) { start = i; mindeg = degree[i]; }
+  int maxlevel = 0;
+  levelize(start, A, level, done, maxlevel);
+  bool newroot = true;
+  while (newroot) {
+    newroot = false;
+    mindeg = degree[start];
+    int k = start;
+    for (int i = 1; i <= n; ++i)
+      if (level[i] == maxlevel && degree[i] < mindeg) { k = i; mindeg = degree[i]; }
+    if (k != start) {
+      const int j = maxlevel;
+      maxlevel = 0;
+      std::fill(done.begin(), done.end(), 0);
+      levelize(k, A, level, done, maxlevel);
+      if (j > maxlevel) { newroot = true; start = j; }
+    }
+  }
+  std::vector<int> order(n + 2, 0), doneidx(n + 1, 0);
+  int indx = 1;
+  order[indx] = start;
+  doneidx[start] = indx++;
+  int scan = 1;  // first node that may still be unnumbered
+  for (int i = 1; i <= n; ++i) {
+    if (order[i] == 0) {
+      while (scan <= n && doneidx[scan] != 0) ++scan;
+      if (scan <= n) { order[indx] = scan; doneidx[scan] = indx++; }
+    }
+    const int node = order[i];
+    for (int64_t c = A.start[node]; c < A.start[node + 1]; ++c) {
+      const int k = A.idx[c];
+      if (doneidx[k] == 0) { order[indx] = k; doneidx[k] = indx++; }
+    }
+  }
+  for (int i = 1; i <= n; ++i) doneidx[order[i]] = n - i + 1;
+  std::vector<int> saved(perm);
+  for (int i = 1; i <= n; ++i) perm[i] = saved[i] > 0 ? doneidx[saved[i]] : 0;
+  after = bandwidth(n, A, &perm) + 1;
+  accepted = 1;
+  if (before < after) {
+    perm = saved;
+    accepted = 0;
+  }
+}
+
+}  // namespace
+
+int build_host_pattern(int nnodes, int nelem, int npe, const int *el, int nbnd, int npe_b, const int *bel, int dofs,
+                       int optimize, HostPattern &P) {
+  if (nnodes <= 0 || nelem <= 0 || npe <= 0 || dofs <= 0) return -1;
+  for (int64_t k = 0; k < (int64_t)nelem * npe; ++k)
+    if (el[k] < 1 || el[k] > nnodes) return -1;
+  for (int64_t k = 0; k < (int64_t)nbnd * npe_b; ++k)
+    if (bel[k] < 1 || bel[k] > nnodes) return -1;
+  Adjacency A = adjacency(nnodes, nelem, npe, el, nbnd, npe_b, bel);
+  for (int i = 1; i <= nnodes; ++i)
+    if (A.start[i + 1] == A.start[i]) return -2;  // node outside every element: not supported
+  P.nnodes = nnodes;
+  P.dofs = dofs;
+  P.n = nnodes * dofs;
+  std::vector<int> perm(nnodes + 1);
+  std::iota(perm.begin(), perm.end(), 0);
+  P.bw_before = P.bw_after = bandwidth(nnodes, A, nullptr) + 1;
+  P.accepted = 0;
+  if (optimize) optimize_bandwidth(nnodes, A, perm, P.bw_before, P.bw_after, P.accepted);
+  P.perm.assign(perm.begin() + 1, perm.end());
+  std::vector<int> inv(nnodes + 1);
+  for (int i = 1; i <= nnodes; ++i) inv[perm[i]] = i;
+  const int n = P.n;
+  P.rows.assign((size_t)n + 1, 0);
+  int64_t pos = 0;
+  for (int j = 1; j <= nnodes; ++j) {
+    const int node = inv[j];
+    const int64_t deg = (A.start[node + 1] - A.start[node]) * dofs;
+    for (int l = 0; l < dofs; ++l) {
+      P.rows[(size_t)dofs * (j - 1) + l] = (int)(pos + 1);
+      pos += deg;
+    }
+  }
+  P.rows[n] = (int)(pos + 1);
+  P.nnz = pos;
+  if (P.nnz >= INT32_MAX) return -3;
+  P.cols.resize((size_t)P.nnz);
+  P.diag.resize((size_t)n);
+#pragma omp parallel
+  {
+    std::vector<int> buf;
+#pragma omp for schedule(dynamic, 1024)
+    for (int j = 1; j <= nnodes; ++j) {
+      const int node = inv[j];
+      buf.clear();
+      for (int64_t c = A.start[node]; c < A.start[node + 1]; ++c) {
+        const int k = perm[A.idx[c]];
+        for (int d = 1; d <= dofs; ++d) buf.push_back(dofs * (k - 1) + d);
+      }
+      std::sort(buf.begin(), buf.end());
+      for (int l = 0; l < dofs; ++l) {
+        const int r = dofs * (j - 1) + l;  // 0-based row
+        const int64_t base = P.rows[r] - 1;
+        std::copy(buf.begin(), buf.end(), P.cols.begin() + base);
+        const auto it = std::lower_bound(buf.begin(), buf.end(), r + 1);
+        P.diag[r] = (int)(base + (it - buf.begin()) + 1);
+      }
+    }
+  }
+  return 0;
+}
+
+int build_slot_maps(const HostPattern &P, int nelem, int npe, const int *el, std::vector<int> &slot_start,
+                    std::vector<int> &slot_contrib, std::vector<int> &rhs_start, std::vector<int> &rhs_contrib) {
+  const int dofs = P.dofs, n = P.n, nnodes = P.nnodes;
+  const int ld = npe * dofs;
+  if ((int64_t)nelem * ld * ld >= INT32_MAX) return -3;
+  Incidence I = incidence(nnodes, nelem, npe, el);
+  // count per slot: every element touching row node a adds one contribution
+  // to the slots of its npe*dofs columns
+  slot_start.assign((size_t)P.nnz + 1, 0);
+  rhs_start.assign((size_t)n + 1, 0);
+  std::vector<int> inv(nnodes + 1);
+  for (int i = 0; i < nnodes; ++i) inv[P.perm[i]] = i + 1;
+  // pass 1: counts
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int j = 1; j <= nnodes; ++j) {
+    const int a = inv[j];
+    for (int l = 0; l < dofs; ++l) {
+      const int r = dofs * (j - 1) + l;
+      const int64_t b0 = P.rows[r] - 1, b1 = P.rows[r + 1] - 1;
+      const int *cb = P.cols.data() + b0;
+      const int len = (int)(b1 - b0);
+      rhs_start[r + 1] = (int)(I.start[a + 1] - I.start[a]);
+      for (int64_t k = I.start[a]; k < I.start[a + 1]; ++k) {
+        const int *nd = el + (int64_t)I.elem[k] * npe;
+        for (int bl = 0; bl < npe; ++bl) {
+          const int kb = P.perm[nd[bl] - 1];
+          for (int m = 1; m <= dofs; ++m) {
+            const int col = dofs * (kb - 1) + m;
+            const int pos = (int)(std::lower_bound(cb, cb + len, col) - cb);
+            // counts accumulate per slot; slots of one row are private to this thread
+            slot_start[b0 + pos + 1]++;
+          }
+        }
+      }
+    }
+  }
+  for (int64_t s = 0; s < P.nnz; ++s) slot_start[s + 1] += slot_start[s];
+  for (int r = 0; r < n; ++r) rhs_start[r + 1] += rhs_start[r];
+  slot_contrib.resize((size_t)slot_start[P.nnz]);
+  rhs_contrib.resize((size_t)rhs_start[n]);
+  // pass 2: fill in ascending element order (incidence lists are ascending)
+#pragma omp parallel
+  {
+    std::vector<int> fillpos;
+#pragma omp for schedule(dynamic, 1024)
+    for (int j = 1; j <= nnodes; ++j) {
+      const int a = inv[j];
+      for (int l = 0; l < dofs; ++l) {
+        const int r = dofs * (j - 1) + l;
+        const int64_t b0 = P.rows[r] - 1, b1 = P.rows[r + 1] - 1;
+        const int *cb = P.cols.data() + b0;
+        const int len = (int)(b1 - b0);
+        fillpos.assign(len, 0);
+        int rp = rhs_start[r];
+        for (int64_t k = I.start[a]; k < I.start[a + 1]; ++k) {
+          const int e = I.elem[k];
+          const int la = I.local[k];
+          const int li = la * dofs + l;
+          const int *nd = el + (int64_t)e * npe;
+          rhs_contrib[rp++] = e * ld + li;
+          for (int bl = 0; bl < npe; ++bl) {
+            const int kb = P.perm[nd[bl] - 1];
+            for (int m = 1; m <= dofs; ++m) {
+              const int col = dofs * (kb - 1) + m;
+              const int pos = (int)(std::lower_bound(cb, cb + len, col) - cb);
+              const int lj = bl * dofs + (m - 1);
+              slot_contrib[slot_start[b0 + pos] + fillpos[pos]++] = e * ld * ld + li * ld + lj;
+            }
+          }
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+}  // namespace ehip

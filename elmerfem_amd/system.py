@@ -1,0 +1,204 @@
+"""Host-side mirror of the reference's Default* solver API over libehip.
+
+A `System` is a device-resident Matrix_t (CRS Rows/Cols/Diag/Values/RHS)
+with the entry points a solver module calls (fem/src/DefUtils.src):
+
+    DefaultInitialize + element loop + DefaultUpdateEquations -> assemble()
+    DefaultDirichletBCs                                      -> dirichlet()
+    DefaultSolve                                             -> solve()
+
+All work runs in libehip (HIP on the GPU); this module only moves host
+arrays across the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .keywords import LinearSystemKeywords
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a, n=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+class System:
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        L = _lib.load()
+        n = C.c_int()
+        nnz = C.c_int64()
+        dofs = C.c_int()
+        bb, ba, acc = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(L.ehip_system_info(self._h, C.byref(n), C.byref(nnz), C.byref(dofs), C.byref(bb), C.byref(ba),
+                                      C.byref(acc)), "ehip_system_info")
+        self.n, self.nnz, self.dofs = n.value, nnz.value, dofs.value
+        self.bandwidth = (bb.value, ba.value, bool(acc.value))
+        self.nnodes = self.n // self.dofs
+
+    # ------------------------------------------------------------ creation
+    @classmethod
+    def from_csr(cls, rows, cols, diag):
+        L = _lib.init()
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        diag = np.ascontiguousarray(diag, np.int32)
+        h = C.c_void_p()
+        _lib.check(L.ehip_system_create(len(rows) - 1, len(cols), _ptr(rows), _ptr(cols), _ptr(diag), C.byref(h)),
+                   "ehip_system_create")
+        return cls(h)
+
+    @classmethod
+    def from_mesh(cls, mesh, dofs=1, optimize_bandwidth=True, with_boundary=True):
+        L = _lib.init()
+        el = np.ascontiguousarray(mesh.elements, np.int32)
+        be = np.ascontiguousarray(mesh.bnd_elements, np.int32) if with_boundary and len(mesh.bnd_elements) else None
+        h = C.c_void_p()
+        _lib.check(L.ehip_system_from_mesh(mesh.nnodes, el.shape[0], el.shape[1], _ptr(el),
+                                           0 if be is None else be.shape[0], 0 if be is None else be.shape[1],
+                                           _ptr(be), dofs, int(bool(optimize_bandwidth)), C.byref(h)),
+                   "ehip_system_from_mesh")
+        s = cls(h)
+        s.mesh = mesh
+        return s
+
+    def free(self):
+        if self._h:
+            _lib.load().ehip_system_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ pattern / values
+    def pattern(self):
+        perm = np.zeros(self.nnodes, np.int32)
+        rows = np.zeros(self.n + 1, np.int32)
+        cols = np.zeros(self.nnz, np.int32)
+        diag = np.zeros(self.n, np.int32)
+        _lib.check(_lib.load().ehip_system_get_pattern(self._h, _ptr(perm), _ptr(rows), _ptr(cols), _ptr(diag)),
+                   "ehip_system_get_pattern")
+        return perm, rows, cols, diag
+
+    def set_values(self, values=None, rhs=None):
+        v = _f64(values, self.nnz)
+        b = _f64(rhs, self.n)
+        _lib.check(_lib.load().ehip_system_set_values(self._h, _ptr(v), _ptr(b)), "ehip_system_set_values")
+
+    def values(self):
+        v = np.zeros(self.nnz)
+        b = np.zeros(self.n)
+        _lib.check(_lib.load().ehip_system_get_values(self._h, _ptr(v), _ptr(b)), "ehip_system_get_values")
+        return v, b
+
+    def device_bytes(self) -> int:
+        return int(_lib.load().ehip_system_bytes(self._h))
+
+    # ------------------------------------------------------------ assembly
+    def assemble(self, physics, coords, elem_coef=None, elem_conv=None, elem_velo=None, elem_load=None,
+                 poisson=0.3, elem_code=None, cdim=None) -> float:
+        """Element loop + DefaultUpdateEquations; returns device time in ms."""
+        mesh = getattr(self, "mesh", None)
+        code = elem_code if elem_code is not None else mesh.elem_code
+        cd = cdim if cdim is not None else (mesh.dim if mesh is not None else 3)
+        ne = mesh.nelem
+        xyz = _f64(coords)
+        t = C.c_double()
+        arr = lambda a, m: None if a is None else _f64(np.broadcast_to(a, m).copy() if np.ndim(a) == 0 or np.shape(a) != m else a)
+        coef = arr(elem_coef, (ne,))
+        conv = arr(elem_conv, (ne,))
+        velo = arr(elem_velo, (ne, 3))
+        load = arr(elem_load, (ne,))
+        _lib.check(_lib.load().ehip_assemble(self._h, physics, code, cd, _ptr(xyz), _ptr(coef), _ptr(conv), _ptr(velo),
+                                             _ptr(load), float(poisson), C.byref(t)), "ehip_assemble")
+        return t.value
+
+    def assemble_traction(self, bnd_code, coords, belems, load3):
+        be = np.ascontiguousarray(belems, np.int32)
+        _lib.check(_lib.load().ehip_assemble_traction(self._h, bnd_code, _ptr(_f64(coords)), be.shape[0], be.shape[1],
+                                                      _ptr(be), _ptr(_f64(load3, 3))), "ehip_assemble_traction")
+
+    def dirichlet(self, rows, vals=None, symmetric=False, scaling=True):
+        r = np.ascontiguousarray(rows, np.int32)
+        v = None if vals is None else _f64(vals, r.size)
+        _lib.check(_lib.load().ehip_dirichlet(self._h, r.size, _ptr(r), _ptr(v), int(symmetric), int(scaling)),
+                   "ehip_dirichlet")
+
+    def dirichlet_nodes(self, nodes, dof=None, vals=None, symmetric=False, scaling=True):
+        """Rows of the given 1-based mesh nodes (all dofs, or one 1-based dof)."""
+        perm, _, _, _ = self.pattern() if not hasattr(self, "_perm") else (self._perm, None, None, None)
+        self._perm = perm
+        k = perm[np.asarray(nodes) - 1].astype(np.int64)
+        dofs = range(1, self.dofs + 1) if dof is None else [dof]
+        rows = np.concatenate([self.dofs * (k - 1) + d for d in dofs]).astype(np.int32)
+        self.dirichlet(rows, vals, symmetric, scaling)
+        return rows
+
+    # ------------------------------------------------------------ solve
+    def solve(self, kw: LinearSystemKeywords | _lib.Params, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL,
+              inner_tol=0.0, inner_maxit=0, check_every=0, history=False, return_x=True):
+        p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every,
+                                                              history)
+        if isinstance(kw, _lib.Params):
+            p.record_history = int(history)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        rep = _lib.Report()
+        hist = np.zeros(p.maxit + 2) if history else None
+        _lib.check(_lib.load().ehip_solve(self._h, C.byref(p), _ptr(x) if return_x or x0 is not None else None,
+                                          C.byref(rep), _ptr(hist)), "ehip_solve")
+        out = rep.as_dict()
+        if history:
+            k = rep.outer_iters + 1 if p.precision == _lib.MIXED_IR else min(rep.iters, p.maxit)
+            out["history"] = hist[:k].copy()
+        return (x if return_x else None), out
+
+    # ------------------------------------------------------------ SpMV
+    def spmv(self, u, precision=0):
+        u = _f64(u, self.n)
+        v = np.zeros(self.n)
+        _lib.check(_lib.load().ehip_spmv(self._h, precision, _ptr(u), _ptr(v)), "ehip_spmv")
+        return v
+
+    def spmv_bench(self, precision=0, reps=50) -> float:
+        us = C.c_double()
+        _lib.check(_lib.load().ehip_spmv_bench(self._h, precision, reps, C.byref(us)), "ehip_spmv_bench")
+        return us.value
+
+    def spmv_tune(self, precision=0):
+        _lib.check(_lib.load().ehip_spmv_tune(self._h, precision), "ehip_spmv_tune")
+        return self.spmv_config()
+
+    def spmv_config(self):
+        b, nb, ch = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.load().ehip_spmv_config(self._h, C.byref(b), C.byref(nb), C.byref(ch)), "ehip_spmv_config")
+        return {"block": b.value, "nnz_per_block": nb.value, "chunks": ch.value}
+
+
+def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL):
+    """One-shot drop-in (the Before-Linsolve hook path): host CRS in, x out."""
+    L = _lib.init()
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    diag = np.ascontiguousarray(diag, np.int32)
+    v = _f64(values)
+    bb = _f64(b)
+    x = np.zeros(len(rows) - 1) if x0 is None else np.array(x0, dtype=np.float64)
+    p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode)
+    rep = _lib.Report()
+    _lib.check(L.ehip_solve_csr(len(rows) - 1, _ptr(rows), _ptr(cols), _ptr(diag), _ptr(v), _ptr(bb), _ptr(x),
+                                C.byref(p), C.byref(rep)), "ehip_solve_csr")
+    return x, rep.as_dict()

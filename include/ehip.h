@@ -1,0 +1,188 @@
+/*
+ * ehip.h -- C ABI of the MI355X-native assemble-then-iterate library
+ * (libehip.so).  Plain pointers and sizes only; callable from C, from
+ * Fortran through ISO_C_BINDING (elmerfem_amd/fortran/ehip.F90) and from
+ * Python through ctypes.
+ *
+ * Conventions shared with the reference's Matrix_t (fem/src/Types.src:155-228):
+ *  - CRS arrays are 1-based as Elmer holds them: rows[n+1], cols[nnz],
+ *    diag[n] (int32), values[nnz] and rhs[n] (double).
+ *  - Unknowns are interleaved per node: row = dofs*(perm(node)-1)+dof
+ *    (fem/src/CRSMatrix.src:553-583).
+ *  - Status codes: every entry point returns EHIP_OK (0) or a negative
+ *    EHIP_E* code and never aborts; ehip_last_error() describes the failure.
+ *    Solver outcomes come back in ehip_report.info with HUTI's codes
+ *    (hutiter/src/huti_fdefs.h:19-56): 1 converged, 2 max iterations,
+ *    3 diverged, 20 CG rho = 0, 35/36/37 BiCGStab rho/s-norm/omega.
+ *  - Threading: a system is used by one host thread at a time; each system
+ *    owns one HIP stream.  Calls on different systems may overlap.
+ */
+#ifndef EHIP_H
+#define EHIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EHIP_OK 0
+#define EHIP_EARG (-1)     /* invalid argument */
+#define EHIP_EHIP (-2)     /* HIP runtime error */
+#define EHIP_ENOMEM (-3)   /* device or host allocation failed */
+#define EHIP_EUNSUP (-4)   /* unsupported element type / option */
+#define EHIP_ECAP (-5)     /* request exceeds a capacity limit */
+#define EHIP_ESTATE (-6)   /* call out of order (e.g. solve before values) */
+
+/* Linear System Iterative Method (IterSolve.src:221-253) */
+#define EHIP_CG 0
+#define EHIP_BICGSTAB 1
+/* Linear System Preconditioning (IterSolve.src:372-411) */
+#define EHIP_PRECOND_NONE 0
+#define EHIP_PRECOND_DIAGONAL 1
+/* precision forms */
+#define EHIP_FP64 0
+#define EHIP_MIXED_IR 1 /* FP32 inner Krylov inside FP64 iterative refinement */
+/* iteration modes */
+#define EHIP_FAITHFUL 0 /* HUTI loop order: true residual every iteration */
+#define EHIP_FAST 1     /* fused kernels, recurrence residual, true-residual exit check */
+
+/* HUTI_INFO codes */
+#define EHIP_HUTI_CONVERGENCE 1
+#define EHIP_HUTI_MAXITER 2
+#define EHIP_HUTI_DIVERGENCE 3
+#define EHIP_HUTI_CG_RHO 20
+#define EHIP_HUTI_BICGSTAB_RHO 35
+#define EHIP_HUTI_BICGSTAB_SNORM 36
+#define EHIP_HUTI_BICGSTAB_OMEGA 37
+
+typedef struct ehip_system ehip_system;
+
+/* Solver parameters: the Linear System keywords IterSolver and
+ * SolveLinearSystem read (IterSolve.src:221-366, SolverUtils.src:7256). */
+typedef struct {
+  int method;        /* EHIP_CG | EHIP_BICGSTAB ("Linear System Iterative Method") */
+  int precond;       /* EHIP_PRECOND_* ("Linear System Preconditioning") */
+  double tol;        /* "Linear System Convergence Tolerance" */
+  int maxit;         /* "Linear System Max Iterations" */
+  int scaling;       /* "Linear System Scaling", default 1 */
+  int precision;     /* EHIP_FP64 | EHIP_MIXED_IR */
+  int mode;          /* EHIP_FAITHFUL | EHIP_FAST */
+  double inner_tol;  /* mixed form: relative reduction per inner solve (0 = default 1e-4) */
+  int inner_maxit;   /* mixed form: inner iteration cap per refinement step (0 = default) */
+  int check_every;   /* host polls convergence every k iterations (0 = default) */
+  double divtol;     /* "Linear System Divergence Tolerance" (0 = none) */
+  int record_history;/* 1: keep the residual of every iteration */
+} ehip_params;
+
+typedef struct {
+  int info;             /* HUTI_INFO */
+  int iters;            /* HUTI_ITERS (sum of inner iterations for mixed IR) */
+  int outer_iters;      /* refinement steps (mixed IR), 0 otherwise */
+  int spmv_count;       /* SpMV launches in the solve */
+  double res_scaled;    /* final ||D(b - A x)|| / ||D b|| (the stop criterion) */
+  double res_true;      /* final ||b - A x|| / ||b|| on the unscaled system, FP64 */
+  double norm;          /* ComputeNorm: sqrt(sum x^2 / n) */
+  double t_total_ms;    /* wall time of the solve call on the device stream */
+  double t_krylov_ms;   /* time inside the Krylov iteration */
+} ehip_report;
+
+/* ---- library ---- */
+int ehip_init(int device);
+const char *ehip_last_error(void);
+int ehip_version(void);
+
+/* ---- exact "Matrix Vector Proc" ABI ----
+ * Replaces CRS_MatrixVectorProd's plugin branch (CRSMatrix.src:3549-3552,
+ * reached through matvecsubr, fem/src/Load.c:734-747): v = A u with host
+ * arrays.  *spmv is Matrix_t%SpMV (Types.src:210): the library keeps its
+ * device copy of the matrix there and re-uploads Values when reinit != 0 or
+ * when their checksum changes. */
+void ehip_matvec(void **spmv, int *n, int *rows, int *cols, double *vals, double *u, double *v,
+                 int *reinit);
+/* Releases what ehip_matvec cached under *spmv. */
+void ehip_matvec_free(void **spmv);
+
+/* ---- device-resident systems ---- */
+/* From an existing 1-based CRS pattern (Matrix_t Rows/Cols/Diag). */
+int ehip_system_create(int n, int64_t nnz, const int *rows, const int *cols, const int *diag,
+                       ehip_system **out);
+/* From a mesh: the CreateMatrix chain (ElementUtils.src:698-935) with
+ * InitialPermutation identity, MakeListMatrix, OptimizeBandwidth
+ * (BandwidthOptimize.src:108-355) when optimize != 0, CRS sort/Diag, plus
+ * the element-to-CRS-slot map used by ehip_assemble.  elements: nelem x npe
+ * 1-based node ids; bnd_elements (may be NULL) take part in the pattern as
+ * the reference's boundary elements do. */
+int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *elements, int nbnd,
+                          int npe_bnd, const int *bnd_elements, int dofs, int optimize,
+                          ehip_system **out);
+int ehip_system_free(ehip_system *s);
+int ehip_system_info(const ehip_system *s, int *n, int64_t *nnz, int *dofs, int *bw_before,
+                     int *bw_after, int *bw_accepted);
+/* Copies of the host-side pattern (1-based); any pointer may be NULL. */
+int ehip_system_get_pattern(const ehip_system *s, int *perm, int *rows, int *cols, int *diag);
+int ehip_system_set_values(ehip_system *s, const double *values, const double *rhs);
+int ehip_system_get_values(ehip_system *s, double *values, double *rhs);
+/* Device memory the system holds, in bytes. */
+int64_t ehip_system_bytes(const ehip_system *s);
+
+/* ---- assembly ---- */
+#define EHIP_PHYS_MODELPDE 0   /* ModelPDE: diffusion + convection + source (ModelPDE.src:75-166) */
+#define EHIP_PHYS_ELASTIC 1    /* StressSolve isotropic elasticity (Stress.src:59-521) */
+/* Bulk element loop + DefaultUpdateEquations glue, in element order, as a
+ * deterministic gather over the element-to-slot map (no float atomics).
+ * coords: nnodes x 3 (row-major).  Per-element arrays (length nelem, or
+ * 3*nelem for velo) may be NULL (= 0).  elem_code: 404, 504, 808, 827.
+ * Values and RHS are zeroed first (DefaultInitialize). */
+int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdim, const double *coords,
+                  const double *elem_coef, const double *elem_conv, const double *elem_velo,
+                  const double *elem_load, double poisson, double *t_ms);
+/* Boundary traction (StressBoundary, Stress.src:814-1020) added to RHS in
+ * boundary-element order: bnd_code 404 or 409, belems nb x npe 1-based. */
+int ehip_assemble_traction(ehip_system *s, int bnd_code, const double *coords, int nb, int npe,
+                           const int *belems, const double *load3);
+/* DefaultDirichletBCs for nodal Dirichlet rows (DefUtils.src:3530-4030):
+ * scale, ZeroRow + unit diagonal + b = val/D (or CRS_SetSymmDirichlet when
+ * symmetric != 0), back-scale.  bc_rows are 1-based matrix rows. */
+int ehip_dirichlet(ehip_system *s, int nbc, const int *bc_rows, const double *bc_vals,
+                   int symmetric, int scaling);
+
+/* ---- solve ----
+ * SolveLinearSystem's iterative branch (SolverUtils.src:7151-7525):
+ * zero-RHS exit, ScaleLinearSystem, x = 1e-8 when x = 0 (IterSolve.src:358),
+ * HUTI CG/BiCGStab with TRESID_SCALED_BYB, back-scale, ComputeNorm.  x is the
+ * host initial guess on entry and the solution on exit (may be NULL: zero
+ * guess, solution left on the device).  hist (may be NULL, length maxit)
+ * receives the per-iteration residual when params->record_history. */
+int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist);
+/* Solution currently on the device. */
+int ehip_get_solution(ehip_system *s, double *x);
+/* One-shot drop-in for a Before-Linsolve hook: host CRS + b in, x out. */
+int ehip_solve_csr(int n, const int *rows, const int *cols, const int *diag, const double *values,
+                   const double *b, double *x, const ehip_params *p, ehip_report *r);
+
+/* ---- measurement helpers ---- */
+/* v = A u on the device-resident system (host vectors), precision 0 = FP64
+ * values, 1 = FP32 copy of the values. */
+int ehip_spmv(ehip_system *s, int precision, const double *u, double *v);
+/* Times `reps` SpMV launches on device-resident vectors; returns the mean
+ * kernel time in microseconds (HIP events on the system's stream). */
+int ehip_spmv_bench(ehip_system *s, int precision, int reps, double *us_per_call);
+/* Chosen SpMV schedule: block size and nnz per block (autotuned). */
+int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks);
+int ehip_spmv_tune(ehip_system *s, int precision);
+
+/* ---- batching: many independent systems in one multi-system launch ----
+ * Systems keep their own device memory; the batch concatenates their
+ * scaled matrices into one block-diagonal CRS and runs per-system scalars
+ * (alpha, beta, omega, rho, convergence) in every launch of the iteration,
+ * replayed from a captured hipGraph. */
+typedef struct ehip_batch ehip_batch;
+int ehip_batch_create(int nsys, ehip_system **systems, ehip_batch **out);
+int ehip_batch_free(ehip_batch *b);
+/* reports[nsys]; solutions stay on the device (ehip_batch_get_solution). */
+int ehip_batch_solve(ehip_batch *b, const ehip_params *p, ehip_report *reports, double *t_ms);
+int ehip_batch_get_solution(ehip_batch *b, int i, double *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -98,7 +98,7 @@ int ref_huti_solve(int method, int precond, int n, const int *rows, const int *c
   double *bb = malloc(sizeof(double) * n);
   memcpy(bb, b, sizeof(double) * n);
   G.n = n; G.rows = rows; G.cols = cols; G.diag = diag; G.values = values; G.precond = precond;
-  G.cap = maxit + 2;
+  G.cap = 2 * maxit + 4;
   G.norms = malloc(sizeof(double) * G.cap);
   G.nnorm = 0;
   int nd = n, wd = wrk;
@@ -111,6 +111,7 @@ int ref_huti_solve(int method, int precond, int n, const int *rows, const int *c
                         (void (*)())cb_pcond, (void (*)())cb_copy, (double (*)())ddot_,
                         (double (*)())cb_norm, (double (*)())cb_stopc);
   *iters = ipar[30];
+  if (G.nnorm > G.cap) G.nnorm = G.cap;
   if (hist) {
     /* CG: norms = [||b||, r1, r2, ...].  BiCGStab: per iteration ||s|| then
        ||Ax-b||, so residuals sit at the even positions after ||b||. */
