@@ -32,13 +32,14 @@ class Params(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
                 ("scaling", C.c_int), ("precision", C.c_int), ("mode", C.c_int), ("inner_tol", C.c_double),
                 ("inner_maxit", C.c_int), ("check_every", C.c_int), ("divtol", C.c_double),
-                ("record_history", C.c_int)]
+                ("record_history", C.c_int), ("profile_every", C.c_int)]
 
 
 class Report(C.Structure):
     _fields_ = [("info", C.c_int), ("iters", C.c_int), ("outer_iters", C.c_int), ("spmv_count", C.c_int),
                 ("res_scaled", C.c_double), ("res_true", C.c_double), ("norm", C.c_double),
-                ("t_total_ms", C.c_double), ("t_krylov_ms", C.c_double)]
+                ("t_total_ms", C.c_double), ("t_krylov_ms", C.c_double), ("spmv_us", C.c_double),
+                ("spmv_samples", C.c_int), ("spmv_precision", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -72,11 +73,17 @@ SIGNATURES = {
     "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "ehip_spmv_config": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "ehip_copy_bench": (C.c_int, [_i64, C.c_int, _vp, _vp]),
     "ehip_spmv_tune": (C.c_int, [_vp, C.c_int]),
     "ehip_batch_create": (C.c_int, [C.c_int, _vp, _vp]),
     "ehip_batch_free": (C.c_int, [_vp]),
     "ehip_batch_solve": (C.c_int, [_vp, C.POINTER(Params), _vp, _vp]),
     "ehip_batch_get_solution": (C.c_int, [_vp, C.c_int, _vp]),
+    "ehip_server_create": (C.c_int, [C.POINTER(Params), C.c_int, _i64, _vp]),
+    "ehip_server_submit": (C.c_int, [_vp, _vp, _vp]),
+    "ehip_server_step": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int]),
+    "ehip_server_status": (C.c_int, [_vp, _vp, _vp]),
+    "ehip_server_free": (C.c_int, [_vp]),
 }
 
 _lib = None
@@ -86,7 +93,7 @@ class EhipError(RuntimeError):
     pass
 
 
-def load(require_all: bool = True):
+def load(require_all: bool = False):
     """Loads libehip.so and binds every declared symbol."""
     global _lib
     if _lib is not None:

@@ -1,0 +1,91 @@
+"""Batching and serving on the multi-system engine of libehip.
+
+`solve_batch` runs many independent systems in one graph-replayed launch
+sequence (each system keeps its own scalars and stopping test).  `Server`
+is the continuous-batching form: systems are admitted while a slot and the
+HBM budget allow, every `step()` advances all live systems by one block of
+iterations, and finished systems leave with their reports.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=0.0, inner_maxit=0,
+                check_every=0):
+    """Returns (solutions, reports, device ms)."""
+    L = _lib.init()
+    arr = (C.c_void_p * len(systems))(*[s._h for s in systems])
+    h = C.c_void_p()
+    _lib.check(L.ehip_batch_create(len(systems), arr, C.byref(h)), "ehip_batch_create")
+    try:
+        p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every)
+        reps = (_lib.Report * len(systems))()
+        ms = C.c_double()
+        _lib.check(L.ehip_batch_solve(h, C.byref(p), reps, C.byref(ms)), "ehip_batch_solve")
+        xs = []
+        for i, s in enumerate(systems):
+            x = np.zeros(s.n)
+            _lib.check(L.ehip_batch_get_solution(h, i, x.ctypes.data_as(C.c_void_p)), "ehip_batch_get_solution")
+            xs.append(x)
+        return xs, [r.as_dict() for r in reps], ms.value
+    finally:
+        L.ehip_batch_free(h)
+
+
+class Server:
+    """Continuous batching over device-resident systems."""
+
+    def __init__(self, kw, precision=_lib.MIXED_IR, max_slots=256, mem_budget=0, inner_tol=0.0, inner_maxit=0,
+                 check_every=0):
+        L = _lib.init()
+        self.p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, _lib.FAST, inner_tol, inner_maxit,
+                                                                     check_every)
+        self._h = C.c_void_p()
+        _lib.check(L.ehip_server_create(C.byref(self.p), max_slots, int(mem_budget), C.byref(self._h)),
+                   "ehip_server_create")
+        self.max_slots = max_slots
+        self._live = {}
+
+    def submit(self, system) -> bool:
+        """Admits the system; False when no slot or budget is free (keep it queued)."""
+        slot = C.c_int()
+        rc = _lib.load().ehip_server_submit(self._h, system._h, C.byref(slot))
+        if rc == -5:  # EHIP_ECAP
+            return False
+        _lib.check(rc, "ehip_server_submit")
+        self._live[system._h.value] = system
+        return True
+
+    def step(self):
+        """One block of iterations for every live system; returns [(system, report)] of finished ones."""
+        k = self.max_slots
+        nd = C.c_int()
+        sys_out = (C.c_void_p * k)()
+        rep_out = (_lib.Report * k)()
+        _lib.check(_lib.load().ehip_server_step(self._h, C.byref(nd), sys_out, rep_out, k), "ehip_server_step")
+        done = []
+        for i in range(nd.value):
+            s = self._live.pop(sys_out[i])
+            done.append((s, rep_out[i].as_dict()))
+        return done
+
+    def status(self):
+        live, nbytes = C.c_int(), C.c_int64()
+        _lib.check(_lib.load().ehip_server_status(self._h, C.byref(live), C.byref(nbytes)), "ehip_server_status")
+        return live.value, nbytes.value
+
+    def free(self):
+        if self._h:
+            _lib.load().ehip_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -1,0 +1,1219 @@
+// Multi-system Krylov engine of libehip: batching and continuous-batching
+// serving.
+//
+// Many independent systems run in the same kernel launches.  The work list
+// is a device-resident array of (slot, row chunk) items walked by a fixed
+// grid; every slot carries its own Krylov scalars and a phase, so systems
+// of very different size and iteration count share one launch sequence.
+// That sequence (one iteration of every active slot) is captured once in a
+// hipGraph and replayed; admitting or retiring a system only rewrites the
+// work list and the slot table between replays, so the graph never changes.
+//
+// Per-slot reductions are deterministic: each chunk publishes its partial
+// sums, an integer ticket per slot finds the chunk that finishes last, and
+// that workgroup adds the slot's chunk partials in chunk order.
+//
+// FP64 mode runs HUTI's CG / BiCGStab loop order per slot (true residual
+// every iteration, hutiter/src/huti_cg.src:145-227, huti_bicgstab.src:
+// 150-262).  Mixed mode runs FP32 inner Krylov (FP32 values, vectors and
+// accumulation; recurrence residual) inside per-slot FP64 iterative
+// refinement (true residual, correction and Jacobi diagonal in FP64).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "ehip_internal.h"
+#include "linalg_kernels.h"
+#include "spmv_core.h"
+
+extern "C" int ehip_internal_report(ehip_system *s, const double *b0, double *res_true, double *norm);
+
+namespace ehip {
+
+enum Phase : int {
+  PH_IDLE = 0,
+  PH_INIT = 1,    // FP64: r = b - A x
+  PH_START = 2,   // FP64: first search direction
+  PH_ITER = 3,    // FP64 Krylov iterations
+  PH_RES = 4,     // mixed: FP64 true residual
+  PH_PREP = 5,    // mixed: normalised FP32 residual, inner start
+  PH_INNER = 6,   // mixed: FP32 inner iterations
+  PH_CORR = 7,    // mixed: x += scale * d
+  PH_DONE = 8,
+};
+
+struct Chunk {
+  int slot, r0, r1, k;  // rows [r0, r1) of the slot's system; k-th chunk of that slot
+};
+
+constexpr int NV64 = 10, NV32 = 11;
+
+struct SlotPtrs {
+  const int *rows, *cols;
+  const double *vals;
+  const float *vals32;
+  double *b, *x;      // scaled right-hand side and solution (the system's own buffers)
+  double *w64;        // NV64 vectors of n doubles
+  float *w32;         // NV32 vectors of n floats
+  int n, ck0, nck, method;
+};
+
+struct BSys {
+  double rho, rho_new, alpha, omega, rhsnorm, res, scale, tol, snorm, pad64;
+  float rho32, rho_new32, alpha32, omega32, beta32, rhsnorm32, res32, itol;
+  int iter, info, done, phase, maxit, outer, inner_iter, inner_total, inner_maxit, snorm_exit;
+  unsigned ticket;
+  int pad;
+};
+
+struct BatchArgs {
+  const Chunk *chunks;
+  const int *nchunks;
+  const SlotPtrs *slots;
+  BSys *st;
+  double *p64;   // chunk partials, 4 per chunk
+  float *p32;
+  int *active;   // number of slots not yet done
+};
+
+// FP64 vector slots: CG r z p q t dg ; BiCGStab r rtld p t1v v s t2v t dg z
+template <typename T>
+__device__ __forceinline__ Vecs<T> slot_vecs(T *w, int n, int method, T *x, T *b) {
+  Vecs<T> v{};
+  const size_t N = (size_t)n;
+  v.x = x;
+  v.b = b;
+  if (method == EHIP_CG) {
+    v.r = w; v.z = w + N; v.p = w + 2 * N; v.q = w + 3 * N; v.t = w + 4 * N; v.dg = w + 5 * N;
+  } else {
+    v.r = w; v.rtld = w + N; v.p = w + 2 * N; v.t1v = w + 3 * N; v.v = w + 4 * N;
+    v.sv = w + 5 * N; v.t2v = w + 6 * N; v.t = w + 7 * N; v.dg = w + 8 * N; v.z = w + 9 * N;
+  }
+  return v;
+}
+
+__device__ __forceinline__ void slot_done(BatchArgs &a, BSys &st, int info) {
+  st.info = info;
+  st.done = 1;
+  st.phase = PH_DONE;
+  atomicSub(a.active, 1);
+}
+
+// One generic kernel: every work item whose slot the op accepts is handled by
+// one workgroup -- SpMV ops stage the chunk's products in LDS and give each
+// row its left-to-right sum, vector ops visit the rows -- and the op's finish
+// runs in the workgroup that completes the slot's last chunk.
+template <class Op, int BS, int NB>
+__global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
+  using T = typename Op::T;
+  constexpr int NV = Op::NV;
+  __shared__ T prod[Op::SPMV ? NB : 1];
+  __shared__ T sh[16 * 4];
+  __shared__ int amlast;
+  const int nch = *a.nchunks;
+  XcdWalk w(nch);
+  for (int c = w.first; c < w.last; c += w.step) {
+    const Chunk ck = a.chunks[c];
+    const SlotPtrs sp = a.slots[ck.slot];
+    BSys &st = a.st[ck.slot];
+    if (!op.active(st)) continue;  // uniform: the slot's state is fixed during this launch
+    typename Op::Ctx cx = op.begin(st, sp);
+    T acc[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int k = 0; k < (NV > 0 ? NV : 1); ++k) acc[k] = (T)0;
+    if constexpr (Op::SPMV) {
+      const int j0 = sp.rows[ck.r0], j1 = sp.rows[ck.r1];
+      const int len = j1 - j0;
+      const T *xin = op.xin(cx);
+      const auto *vals = op.vals(sp);
+      if (len <= NB) {
+        constexpr int ITEMS = NB / BS;
+        T vv[ITEMS], xv[ITEMS];
+        int cc[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          const int j = (int)threadIdx.x + k * BS;
+          if (j < len) { vv[k] = (T)ntload(vals + j0 + j); cc[k] = ntload(sp.cols + j0 + j); }
+        }
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          const int j = (int)threadIdx.x + k * BS;
+          if (j < len) xv[k] = xin[cc[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+          const int j = (int)threadIdx.x + k * BS;
+          if (j < len) prod[j] = vv[k] * xv[k];
+        }
+        __syncthreads();
+        for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) {
+          const int s0 = sp.rows[r] - j0, s1 = sp.rows[r + 1] - j0;
+          T s = (T)0;
+          for (int k = s0; k < s1; ++k) s = s + prod[k];
+          op.row(cx, r, s, acc);
+        }
+        __syncthreads();
+      } else {
+        for (int r = ck.r0; r < ck.r1; ++r) {
+          T part = (T)0;
+          for (int j = sp.rows[r] + threadIdx.x; j < sp.rows[r + 1]; j += BS)
+            part += (T)ntload(vals + j) * xin[ntload(sp.cols + j)];
+          T s = block_sum(part, sh);
+          if (threadIdx.x == 0) op.row(cx, r, s, acc);
+          __syncthreads();
+        }
+      }
+    } else {
+      for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) op.row(cx, r, (T)0, acc);
+    }
+    // publish this chunk's partials, take the slot's ticket
+    T bs[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) bs[k] = block_sum(acc[k], sh + 16 * k);
+    T *part = Op::template partials<T>(a);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        __hip_atomic_store(&part[(size_t)(sp.ck0 + ck.k) * 4 + k], bs[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      const unsigned t = atomicAdd(&st.ticket, 1u);
+      amlast = (t == (unsigned)sp.nck - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (amlast) {
+      __threadfence();
+      T tot[NV > 0 ? NV : 1];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        T s = (T)0;
+        for (int i = threadIdx.x; i < sp.nck; i += BS) s += load_coherent(&part[(size_t)(sp.ck0 + i) * 4 + k]);
+        tot[k] = block_sum(s, sh + 16 * k);
+      }
+      if (threadIdx.x == 0) {
+        st.ticket = 0;
+        op.finish(a, st, sp, tot);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename TT>
+struct OpBase {
+  using T = TT;
+  template <typename U>
+  __device__ static U *partials(BatchArgs &a) {
+    if constexpr (sizeof(U) == 8) return reinterpret_cast<U *>(a.p64);
+    else return reinterpret_cast<U *>(a.p32);
+  }
+};
+
+// ---------------------------------------------------------------- FP64 HUTI loop per slot
+
+struct Ctx64 {
+  Vecs<double> v;
+  double alpha, beta, omega;
+};
+
+__device__ __forceinline__ Ctx64 ctx64(const BSys &st, const SlotPtrs &sp) {
+  Ctx64 c;
+  c.v = slot_vecs<double>(sp.w64, sp.n, sp.method, sp.x, sp.b);
+  c.alpha = st.alpha;
+  c.omega = st.omega;
+  c.beta = 0.0;
+  return c;
+}
+
+// r = b - A x ; z = r/dg ; r.z, r.r, b.b
+struct OpInit : OpBase<double> {
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 3;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INIT; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ const double *xin(const Ctx &c) const { return c.v.x; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[3]) const {
+    const double bi = c.v.b[i], ri = bi - s;
+    c.v.r[i] = ri;
+    const double zi = ri / c.v.dg[i];
+    c.v.z[i] = zi;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+    acc[2] += bi * bi;
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &sp, double (&tot)[3]) const {
+    st.rhsnorm = sqrt(tot[2]);
+    st.rho = sp.method == EHIP_CG ? tot[0] : tot[1];
+    st.phase = PH_START;
+  }
+};
+
+// CG: p = z.  BiCGStab: rtld = r, p = r, v = 0, t1v = r/dg; alpha = 0, omega = 1
+struct OpStart : OpBase<double> {
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = Ctx64;
+  int method;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_START; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&)[1]) const {
+    if (method == EHIP_CG) {
+      c.v.p[i] = c.v.z[i];
+    } else {
+      const double ri = c.v.r[i];
+      c.v.rtld[i] = ri;
+      c.v.p[i] = ri;
+      c.v.v[i] = 0.0;
+      c.v.t1v[i] = ri / c.v.dg[i];
+    }
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&)[1]) const {
+    st.alpha = 0.0;
+    st.omega = 1.0;
+    st.iter = 1;
+    st.phase = PH_ITER;
+    if (st.rho == 0.0) slot_done(a, st, method == EHIP_CG ? 20 : 35);
+  }
+};
+
+// ---- CG, faithful
+struct OpCgQ : OpBase<double> {  // q = A p ; alpha = rho / p.q
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ const double *xin(const Ctx &c) const { return c.v.p; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[1]) const {
+    c.v.q[i] = s;
+    acc[0] += c.v.p[i] * s;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&tot)[1]) const {
+    st.alpha = st.rho / tot[0];
+  }
+};
+
+struct OpCgDa : OpBase<double> {  // x += alpha p ; r -= alpha q ; z = r/dg ; rho_new
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&acc)[1]) const {
+    c.v.x[i] = c.v.x[i] + c.alpha * c.v.p[i];
+    const double ri = c.v.r[i] - c.alpha * c.v.q[i];
+    c.v.r[i] = ri;
+    const double zi = ri / c.v.dg[i];
+    c.v.z[i] = zi;
+    acc[0] += ri * zi;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&tot)[1]) const { st.rho_new = tot[0]; }
+};
+
+struct OpCgRes : OpBase<double> {  // ||A x - b|| / ||b|| ; p = z + (rho_new/rho) p
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const {
+    Ctx c = ctx64(st, sp);
+    c.beta = st.rho_new / st.rho;
+    return c;
+  }
+  __device__ const double *xin(const Ctx &c) const { return c.v.x; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[1]) const {
+    const double ti = s - c.v.b[i];
+    acc[0] += ti * ti;
+    c.v.p[i] = c.v.z[i] + c.beta * c.v.p[i];
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[1]) const {
+    const double res = sqrt(tot[0]) / st.rhsnorm;
+    st.res = res;
+    if (res < st.tol) { slot_done(a, st, 1); return; }
+    st.rho = st.rho_new;
+    st.iter += 1;
+    if (st.iter > st.maxit) slot_done(a, st, 2);
+    else if (st.rho_new == 0.0) slot_done(a, st, 20);
+  }
+};
+
+// ---- BiCGStab, faithful
+template <typename TT>
+struct CtxT {
+  Vecs<TT> v;
+  TT alpha, beta, omega;
+};
+
+struct OpBiV : OpBase<double> {  // v = A t1v ; alpha = rho / rtld.v
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ const double *xin(const Ctx &c) const { return c.v.t1v; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[1]) const {
+    c.v.v[i] = s;
+    acc[0] += c.v.rtld[i] * s;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&tot)[1]) const { st.alpha = st.rho / tot[0]; }
+};
+
+struct OpBiS : OpBase<double> {  // s = r - alpha v ; t2v = s/dg ; ||s||
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&acc)[1]) const {
+    const double si = c.v.r[i] - c.alpha * c.v.v[i];
+    c.v.sv[i] = si;
+    c.v.t2v[i] = si / c.v.dg[i];
+    acc[0] += si * si;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&tot)[1]) const {
+    st.snorm = sqrt(tot[0]);
+    if (st.snorm < (double)1.17549435E-38f) st.snorm_exit = 1;  // HUTI_EPSILON
+  }
+};
+
+struct OpBiSnorm : OpBase<double> {  // s-norm exit: x += alpha t1v
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&)[1]) const { c.v.x[i] = c.v.x[i] + c.alpha * c.v.t1v[i]; }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&)[1]) const { slot_done(a, st, 36); }
+};
+
+struct OpBiT : OpBase<double> {  // t = A t2v ; omega = t.s / t.t
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 2;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ const double *xin(const Ctx &c) const { return c.v.t2v; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[2]) const {
+    c.v.t[i] = s;
+    acc[0] += s * c.v.sv[i];
+    acc[1] += s * s;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&tot)[2]) const { st.omega = tot[0] / tot[1]; }
+};
+
+struct OpBiXr : OpBase<double> {  // x += alpha t1v + omega t2v ; r = s - omega t ; rho_new = rtld.r
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&acc)[1]) const {
+    c.v.x[i] = c.v.x[i] + c.alpha * c.v.t1v[i] + c.omega * c.v.t2v[i];
+    const double ri = c.v.sv[i] - c.omega * c.v.t[i];
+    c.v.r[i] = ri;
+    acc[0] += c.v.rtld[i] * ri;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&tot)[1]) const { st.rho_new = tot[0]; }
+};
+
+struct OpBiRes : OpBase<double> {  // ||A x - b||/||b|| ; p = r + beta (p - omega v) ; t1v = p/dg
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 1;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const {
+    Ctx c = ctx64(st, sp);
+    c.beta = (st.rho_new * st.alpha) / (st.rho * st.omega);
+    return c;
+  }
+  __device__ const double *xin(const Ctx &c) const { return c.v.x; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[1]) const {
+    const double ti = s - c.v.b[i];
+    acc[0] += ti * ti;
+    const double pi = c.v.r[i] + c.beta * (c.v.p[i] - c.omega * c.v.v[i]);
+    c.v.p[i] = pi;
+    c.v.t1v[i] = pi / c.v.dg[i];
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[1]) const {
+    const double res = sqrt(tot[0]) / st.rhsnorm;
+    st.res = res;
+    if (res < st.tol) { slot_done(a, st, 1); return; }
+    if (st.omega == 0.0) { slot_done(a, st, 37); return; }
+    st.iter += 1;
+    if (st.iter > st.maxit) slot_done(a, st, 2);
+    else if (st.rho_new == 0.0) slot_done(a, st, 35);
+    else st.rho = st.rho_new;
+  }
+};
+
+// ---------------------------------------------------------------- mixed: FP64 refinement per slot
+
+struct CtxM {
+  Vecs<float> v;   // inner FP32 vectors; v.x = d (the correction), v.b = normalised residual
+  double *t64, *x64, *b64, *dg64;
+  double scale;
+  float alpha, beta, omega;
+};
+
+__device__ __forceinline__ CtxM ctxm(const BSys &st, const SlotPtrs &sp) {
+  CtxM c;
+  const int nvi = sp.method == EHIP_CG ? 6 : 10;
+  c.v = slot_vecs<float>(sp.w32, sp.n, sp.method, sp.w32 + (size_t)nvi * sp.n, nullptr);
+  c.v.b = c.v.r;
+  c.t64 = sp.w64;
+  c.dg64 = sp.w64 + (size_t)sp.n;
+  c.x64 = sp.x;
+  c.b64 = sp.b;
+  c.scale = st.scale;
+  c.alpha = st.alpha32;
+  c.beta = st.beta32;
+  c.omega = st.omega32;
+  return c;
+}
+
+// t = b - A x (FP64), the refinement's true residual
+struct OpRes : OpBase<double> {
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 2;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_RES; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ const double *xin(const Ctx &c) const { return c.x64; }
+  __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[2]) const {
+    const double bi = c.b64[i], ti = bi - s;
+    c.t64[i] = ti;
+    acc[0] += ti * ti;
+    acc[1] += bi * bi;
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[2]) const {
+    if (st.outer == 0) st.rhsnorm = sqrt(tot[1]);
+    const double nt = sqrt(tot[0]);
+    const double res = nt / st.rhsnorm;
+    st.res = res;
+    st.outer += 1;
+    if (res < st.tol) slot_done(a, st, 1);
+    else if (!(res == res)) slot_done(a, st, 3);
+    else if (st.inner_total >= st.maxit) slot_done(a, st, 2);
+    else {
+      st.scale = nt;
+      st.phase = PH_PREP;
+    }
+  }
+};
+
+// r32 = t/||t|| ; d = 0 ; CG: z = r/dg, p = q = 0 ; BiCGStab: rtld = p = r, v = 0, t1v = r/dg
+struct OpPrep : OpBase<float> {
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 2;
+  using Ctx = CtxM;
+  int method;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_PREP; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, float, float (&acc)[2]) const {
+    const float ri = (float)(c.t64[i] / c.scale);
+    c.v.r[i] = ri;
+    c.v.x[i] = 0.f;
+    if (method == EHIP_CG) {
+      const float zi = ri / c.v.dg[i];
+      c.v.z[i] = zi;
+      c.v.p[i] = 0.f;
+      c.v.q[i] = 0.f;
+      acc[0] += ri * zi;
+    } else {
+      c.v.rtld[i] = ri;
+      c.v.p[i] = ri;
+      c.v.v[i] = 0.f;
+      c.v.t1v[i] = ri / c.v.dg[i];
+      acc[0] += ri * ri;
+    }
+    acc[1] += ri * ri;
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, float (&tot)[2]) const {
+    st.rho32 = tot[0];
+    st.rhsnorm32 = sqrtf(tot[1]);
+    st.beta32 = 0.f;
+    st.alpha32 = 0.f;
+    st.omega32 = 1.f;
+    st.inner_iter = 0;
+    st.snorm_exit = 0;
+    if (tot[0] == 0.f) slot_done(a, st, method == EHIP_CG ? 20 : 35);
+    else st.phase = PH_INNER;
+  }
+};
+
+__device__ __forceinline__ void inner_step_done(BSys &st, float res, bool breakdown) {
+  st.res32 = res;
+  st.inner_iter += 1;
+  st.inner_total += 1;
+  if (res < st.itol || breakdown || !(res == res) || st.inner_iter >= st.inner_maxit ||
+      st.inner_total >= st.maxit)
+    st.phase = PH_CORR;
+}
+
+// inner CG step 1: q = A z + beta q ; p = z + beta p ; alpha = rho / p.q
+struct OpCgF1 : OpBase<float> {
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 1;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ const float *xin(const Ctx &c) const { return c.v.z; }
+  __device__ const float *vals(const SlotPtrs &sp) const { return sp.vals32; }
+  __device__ void row(Ctx &c, int i, float s, float (&acc)[1]) const {
+    const float qi = s + c.beta * c.v.q[i];
+    const float pi = c.v.z[i] + c.beta * c.v.p[i];
+    c.v.q[i] = qi;
+    c.v.p[i] = pi;
+    acc[0] += pi * qi;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[1]) const { st.alpha32 = st.rho32 / tot[0]; }
+};
+
+// inner CG step 2: d += alpha p ; r -= alpha q ; z = r/dg ; rho, ||r||
+struct OpCgF2 : OpBase<float> {
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 2;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, float, float (&acc)[2]) const {
+    c.v.x[i] = c.v.x[i] + c.alpha * c.v.p[i];
+    const float ri = c.v.r[i] - c.alpha * c.v.q[i];
+    c.v.r[i] = ri;
+    const float zi = ri / c.v.dg[i];
+    c.v.z[i] = zi;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[2]) const {
+    const float rho_old = st.rho32;
+    st.rho32 = tot[0];
+    st.beta32 = tot[0] / rho_old;
+    inner_step_done(st, sqrtf(tot[1]) / st.rhsnorm32, tot[0] == 0.f);
+  }
+};
+
+// inner BiCGStab (recurrence residual)
+struct OpBiV32 : OpBase<float> {
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 1;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ const float *xin(const Ctx &c) const { return c.v.t1v; }
+  __device__ const float *vals(const SlotPtrs &sp) const { return sp.vals32; }
+  __device__ void row(Ctx &c, int i, float s, float (&acc)[1]) const {
+    c.v.v[i] = s;
+    acc[0] += c.v.rtld[i] * s;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[1]) const { st.alpha32 = st.rho32 / tot[0]; }
+};
+
+struct OpBiS32 : OpBase<float> {
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 1;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, float, float (&acc)[1]) const {
+    const float si = c.v.r[i] - c.alpha * c.v.v[i];
+    c.v.sv[i] = si;
+    c.v.t2v[i] = si / c.v.dg[i];
+    acc[0] += si * si;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[1]) const {
+    if (sqrtf(tot[0]) < 1.17549435E-38f) st.snorm_exit = 1;
+  }
+};
+
+struct OpBiSnorm32 : OpBase<float> {  // d += alpha t1v, end of the inner solve
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER && st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, float, float (&)[1]) const { c.v.x[i] = c.v.x[i] + c.alpha * c.v.t1v[i]; }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&)[1]) const {
+    st.inner_iter += 1;
+    st.inner_total += 1;
+    st.phase = PH_CORR;
+  }
+};
+
+struct OpBiT32 : OpBase<float> {
+  static constexpr bool SPMV = true;
+  static constexpr int NV = 2;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ const float *xin(const Ctx &c) const { return c.v.t2v; }
+  __device__ const float *vals(const SlotPtrs &sp) const { return sp.vals32; }
+  __device__ void row(Ctx &c, int i, float s, float (&acc)[2]) const {
+    c.v.t[i] = s;
+    acc[0] += s * c.v.sv[i];
+    acc[1] += s * s;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[2]) const { st.omega32 = tot[0] / tot[1]; }
+};
+
+struct OpBiXr32 : OpBase<float> {
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 2;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, float, float (&acc)[2]) const {
+    c.v.x[i] = c.v.x[i] + c.alpha * c.v.t1v[i] + c.omega * c.v.t2v[i];
+    const float ri = c.v.sv[i] - c.omega * c.v.t[i];
+    c.v.r[i] = ri;
+    acc[0] += c.v.rtld[i] * ri;
+    acc[1] += ri * ri;
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[2]) const {
+    const float rho_new = tot[0];
+    const bool brk = st.omega32 == 0.f || rho_new == 0.f;
+    st.beta32 = (rho_new * st.alpha32) / (st.rho32 * st.omega32);
+    st.rho32 = rho_new;
+    inner_step_done(st, sqrtf(tot[1]) / st.rhsnorm32, brk);
+  }
+};
+
+struct OpBiP32 : OpBase<float> {  // p = r + beta (p - omega v) ; t1v = p/dg
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_INNER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, float, float (&)[1]) const {
+    const float pi = c.v.r[i] + c.beta * (c.v.p[i] - c.omega * c.v.v[i]);
+    c.v.p[i] = pi;
+    c.v.t1v[i] = pi / c.v.dg[i];
+  }
+  __device__ void finish(BatchArgs &, BSys &, const SlotPtrs &, float (&)[1]) const {}
+};
+
+// x += scale * d, then back to the FP64 residual
+struct OpCorr : OpBase<double> {
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = CtxM;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_CORR; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&)[1]) const {
+    c.x64[i] = c.x64[i] + c.scale * (double)c.v.x[i];
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&)[1]) const { st.phase = PH_RES; }
+};
+
+// ---------------------------------------------------------------- per-slot setup kernels
+
+__global__ void k_fill_x(int n, double *x, double v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
+}
+
+__global__ void k_inc(int *c) { *c += 1; }
+
+__global__ void k_to_f32(int64_t nnz, const double *v, float *v32) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x)
+    v32[j] = (float)v[j];
+}
+
+// dg64 / dg32 from the scaled diagonal (CRS_DiagPrecondition's divisor)
+__global__ void bk_dg(int n, const int *diag, const double *vals, double *dg64, float *dg32, int precond) {
+  const double AEPS = 10.0 * DBL_EPSILON;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double d = vals[diag[i]];
+    const double g = (precond && fabs(d) > AEPS) ? d : 1.0;
+    if (dg64) dg64[i] = g;
+    if (dg32) dg32[i] = (float)g;
+  }
+}
+
+}  // namespace ehip
+
+using namespace ehip;
+
+// ================================================================ host engine
+
+struct ehip_batch {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  ehip_params p{};
+  int capacity = 0;              // slots
+  int64_t chunk_cap = 0;         // work items
+  // device
+  Chunk *chunks = nullptr;
+  int *nchunks = nullptr;
+  SlotPtrs *slots = nullptr;
+  BSys *st = nullptr;
+  double *p64 = nullptr;
+  float *p32 = nullptr;
+  int *active = nullptr;
+  int *h_active = nullptr;       // pinned
+  BSys *h_st = nullptr;          // pinned mirror
+  // host view
+  struct Slot {
+    ehip_system *s = nullptr;
+    bool used = false;
+    std::vector<Chunk> ck;       // with slot/k filled, ck0 set at list build
+    double *w64 = nullptr;       // NV64*n (+ unscaled b copy)
+    float *w32 = nullptr;
+    double bscale = 1.0;
+    int64_t bytes = 0;
+  };
+  std::vector<Slot> slot;
+  bool graph_ok = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_iters = 0;
+  int mixed = 0, method = 0;
+  int64_t bytes = 0;
+  int64_t budget = 0;              // serving: HBM budget for solver workspace (0 = none)
+  std::vector<int> batch_members;  // for ehip_batch_* : slot of each system
+};
+
+namespace ehip {
+
+static constexpr int kChunkNB = 2048;
+
+static std::vector<Chunk> make_chunks(const ehip_system *s) {
+  std::vector<Chunk> out;
+  const int *rows = s->h_rows.data();
+  int r = 0, k = 0;
+  while (r < s->n) {
+    const int start = rows[r];
+    int e = r + 1;
+    if (rows[e] - start > kChunkNB) {
+      out.push_back({0, r, e, k++});
+      r = e;
+      continue;
+    }
+    int lo = e, hi = s->n;
+    while (lo < hi) {
+      int mid = lo + (hi - lo + 1) / 2;
+      if (rows[mid] - start <= kChunkNB) lo = mid; else hi = mid - 1;
+    }
+    out.push_back({0, r, lo, k++});
+    r = lo;
+  }
+  return out;
+}
+
+static BatchArgs args_of(ehip_batch *b) {
+  return BatchArgs{b->chunks, b->nchunks, b->slots, b->st, b->p64, b->p32, b->active};
+}
+
+template <class Op>
+static void launch_op(ehip_batch *b, const Op &op) {
+  const int grid = max_grid();
+  hipLaunchKernelGGL((bk_run<Op, 256, kChunkNB>), dim3(grid), dim3(256), 0, b->stream, args_of(b), op);
+}
+
+// one iteration of every active slot
+static void enqueue_iteration(ehip_batch *b) {
+  if (!b->mixed) {
+    launch_op(b, OpInit{});
+    launch_op(b, OpStart{{}, b->method});
+    if (b->method == EHIP_CG) {
+      launch_op(b, OpCgQ{});
+      launch_op(b, OpCgDa{});
+      launch_op(b, OpCgRes{});
+    } else {
+      launch_op(b, OpBiV{});
+      launch_op(b, OpBiS{});
+      launch_op(b, OpBiSnorm{});
+      launch_op(b, OpBiT{});
+      launch_op(b, OpBiXr{});
+      launch_op(b, OpBiRes{});
+    }
+  } else {
+    launch_op(b, OpCorr{});
+    launch_op(b, OpRes{});
+    launch_op(b, OpPrep{{}, b->method});
+    if (b->method == EHIP_CG) {
+      launch_op(b, OpCgF1{});
+      launch_op(b, OpCgF2{});
+    } else {
+      launch_op(b, OpBiV32{});
+      launch_op(b, OpBiS32{});
+      launch_op(b, OpBiSnorm32{});
+      launch_op(b, OpBiT32{});
+      launch_op(b, OpBiXr32{});
+      launch_op(b, OpBiP32{});
+    }
+  }
+}
+
+static int build_graph(ehip_batch *b, int iters) {
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  EHIP_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < iters; ++k) enqueue_iteration(b);
+  EHIP_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+  EHIP_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_iters = iters;
+  b->graph_ok = true;
+  return EHIP_OK;
+}
+
+// Rewrites the device work list from the used slots (between replays only).
+static int upload_worklist(ehip_batch *b) {
+  std::vector<Chunk> all;
+  std::vector<SlotPtrs> tab(b->capacity);
+  for (int i = 0; i < b->capacity; ++i) {
+    auto &sl = b->slot[i];
+    SlotPtrs sp{};
+    if (sl.used) {
+      ehip_system *s = sl.s;
+      sp.rows = s->rows; sp.cols = s->cols; sp.vals = s->vals; sp.vals32 = s->vals32;
+      sp.b = s->rhs; sp.x = s->x; sp.w64 = sl.w64; sp.w32 = sl.w32;
+      sp.n = s->n; sp.method = b->method;
+      sp.ck0 = (int)all.size();
+      sp.nck = (int)sl.ck.size();
+      for (Chunk c : sl.ck) {
+        c.slot = i;
+        all.push_back(c);
+      }
+    }
+    tab[i] = sp;
+  }
+  if ((int64_t)all.size() > b->chunk_cap) {
+    // grow the work list and the partials
+    if (b->chunks) (void)hipFree(b->chunks);
+    if (b->p64) (void)hipFree(b->p64);
+    if (b->p32) (void)hipFree(b->p32);
+    b->chunk_cap = (int64_t)all.size() * 2 + 1024;
+    EHIP_HIP_TRY(hipMalloc(&b->chunks, sizeof(Chunk) * b->chunk_cap));
+    EHIP_HIP_TRY(hipMalloc(&b->p64, sizeof(double) * 4 * b->chunk_cap));
+    EHIP_HIP_TRY(hipMalloc(&b->p32, sizeof(float) * 4 * b->chunk_cap));
+    b->graph_ok = false;  // kernel arguments changed
+  }
+  const int nch = (int)all.size();
+  if (nch) EHIP_HIP_TRY(hipMemcpyAsync(b->chunks, all.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, b->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(b->nchunks, &nch, sizeof(int), hipMemcpyHostToDevice, b->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(b->slots, tab.data(), sizeof(SlotPtrs) * b->capacity, hipMemcpyHostToDevice, b->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+  return EHIP_OK;
+}
+
+static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
+  ehip_batch *b = new ehip_batch();
+  EHIP_HIP_TRY(hipGetDevice(&b->dev));
+  EHIP_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  b->p = *p;
+  b->capacity = capacity;
+  b->slot.resize(capacity);
+  b->mixed = p->precision == EHIP_MIXED_IR;
+  b->method = p->method;
+  EHIP_HIP_TRY(hipMalloc(&b->nchunks, sizeof(int)));
+  EHIP_HIP_TRY(hipMalloc(&b->slots, sizeof(SlotPtrs) * capacity));
+  EHIP_HIP_TRY(hipMalloc(&b->st, sizeof(BSys) * capacity));
+  EHIP_HIP_TRY(hipMalloc(&b->active, sizeof(int)));
+  EHIP_HIP_TRY(hipHostMalloc((void **)&b->h_active, sizeof(int), hipHostMallocDefault));
+  EHIP_HIP_TRY(hipHostMalloc((void **)&b->h_st, sizeof(BSys) * capacity, hipHostMallocDefault));
+  std::vector<BSys> zero(capacity);
+  std::memset(zero.data(), 0, sizeof(BSys) * capacity);
+  for (auto &z : zero) { z.phase = PH_IDLE; z.done = 1; }
+  EHIP_HIP_TRY(hipMemcpy(b->st, zero.data(), sizeof(BSys) * capacity, hipMemcpyHostToDevice));
+  const int zi = 0;
+  EHIP_HIP_TRY(hipMemcpy(b->active, &zi, sizeof(int), hipMemcpyHostToDevice));
+  *out = b;
+  return upload_worklist(b);
+}
+
+static int64_t slot_bytes(const ehip_system *s, bool mixed) {
+  return (int64_t)s->n * (8 * (NV64 + 1) + (mixed ? 4 * (NV32 + 1) : 0)) + (mixed && !s->vals32 ? 4 * s->nnz : 0);
+}
+
+// Scales the system, prepares its vectors and state, and puts it into a free
+// slot.  Returns the slot or a negative status.
+static int engine_admit(ehip_batch *b, ehip_system *s) {
+  int slot = -1;
+  for (int i = 0; i < b->capacity; ++i)
+    if (!b->slot[i].used) { slot = i; break; }
+  if (slot < 0) {
+    set_error("batch: no free slot (capacity %d)", b->capacity);
+    return EHIP_ECAP;
+  }
+  if (!s->has_values) {
+    set_error("batch: system has no values");
+    return EHIP_ESTATE;
+  }
+  auto &sl = b->slot[slot];
+  const int n = s->n;
+  int rc;
+  if ((rc = ensure_state(s))) return rc;
+  // unscaled b kept for the report residual
+  EHIP_HIP_TRY(hipMalloc(&sl.w64, sizeof(double) * (size_t)n * (NV64 + 1)));
+  if (b->mixed) EHIP_HIP_TRY(hipMalloc(&sl.w32, sizeof(float) * (size_t)n * (NV32 + 1)));
+  sl.bytes = slot_bytes(s, b->mixed);
+  double *b0 = sl.w64 + (size_t)NV64 * n;
+  EHIP_HIP_TRY(hipMemcpyAsync(b0, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
+  // x = 0 initial guess -> 1e-8 after scaling (IterSolve.src:358)
+  EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
+  if (b->p.scaling) {
+    if ((rc = scale_system(s, s->rhs, true, s->x, &sl.bscale, b->mixed != 0))) return rc;
+  } else if (b->mixed) {
+    if (!s->vals32)
+      if ((rc = dmalloc(s, (void **)&s->vals32, sizeof(float) * s->nnz))) return rc;
+    hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, s->stream, s->nnz, s->vals, s->vals32);
+  }
+  hipLaunchKernelGGL(k_fill_x, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, 1.0e-8);
+  // Jacobi divisors
+  const int nvi = b->method == EHIP_CG ? 6 : 10;
+  double *dg64 = b->mixed ? sl.w64 + (size_t)n : sl.w64 + (size_t)(b->method == EHIP_CG ? 5 : 8) * n;
+  float *dg32 = b->mixed ? sl.w32 + (size_t)(b->method == EHIP_CG ? 5 : 8) * n : nullptr;
+  (void)nvi;
+  hipLaunchKernelGGL(bk_dg, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, dg64, dg32,
+                     b->p.precond);
+  EHIP_HIP_TRY(hipGetLastError());
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  // slot state
+  BSys z;
+  std::memset(&z, 0, sizeof(z));
+  z.tol = b->p.tol;
+  z.maxit = b->p.maxit;
+  z.itol = (float)(b->p.inner_tol > 0 ? b->p.inner_tol : 1e-4);
+  z.inner_maxit = b->p.inner_maxit > 0 ? b->p.inner_maxit : 1000;
+  z.phase = b->mixed ? PH_RES : PH_INIT;
+  z.omega = 1.0;
+  EHIP_HIP_TRY(hipMemcpyAsync(&b->st[slot], &z, sizeof(BSys), hipMemcpyHostToDevice, b->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+  sl.s = s;
+  sl.used = true;
+  sl.ck = make_chunks(s);
+  b->bytes += sl.bytes;
+  // the active counter counts this slot
+  hipLaunchKernelGGL(k_inc, dim3(1), dim3(1), 0, b->stream, b->active);
+  EHIP_HIP_TRY(hipGetLastError());
+  return slot;
+}
+
+// Back-scales the slot's system, fills its report, frees the slot.
+static int engine_retire(ehip_batch *b, int slot, ehip_report *r) {
+  auto &sl = b->slot[slot];
+  ehip_system *s = sl.s;
+  const BSys &h = b->h_st[slot];
+  ehip_report rep{};
+  rep.info = h.info;
+  rep.iters = b->mixed ? h.inner_total : h.iter;
+  rep.outer_iters = b->mixed ? h.outer : 0;
+  rep.res_scaled = h.res;
+  int rc;
+  if (b->p.scaling)
+    if ((rc = backscale_system(s, s->rhs, s->x, sl.bscale))) return rc;
+  // unscaled FP64 residual and ComputeNorm
+  double *b0 = sl.w64 + (size_t)NV64 * s->n;
+  double rt = 0, nrm = 0;
+  if ((rc = ehip_internal_report(s, b0, &rt, &nrm))) return rc;
+  rep.res_true = rt;
+  rep.norm = nrm;
+  s->has_x = true;
+  if (r) *r = rep;
+  (void)hipFree(sl.w64);
+  if (sl.w32) (void)hipFree(sl.w32);
+  b->bytes -= sl.bytes;
+  sl = ehip_batch::Slot();
+  return EHIP_OK;
+}
+
+// Replays the iteration graph once (graph_iters iterations of every slot).
+static int engine_step(ehip_batch *b) {
+  if (!b->graph_ok) {
+    int rc = build_graph(b, b->p.check_every > 0 ? b->p.check_every : 8);
+    if (rc) return rc;
+  }
+  EHIP_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(b->h_active, b->active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  EHIP_HIP_TRY(hipMemcpyAsync(b->h_st, b->st, sizeof(BSys) * b->capacity, hipMemcpyDeviceToHost, b->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+  return EHIP_OK;
+}
+
+static void engine_free(ehip_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto &sl : b->slot) {
+    if (sl.w64) (void)hipFree(sl.w64);
+    if (sl.w32) (void)hipFree(sl.w32);
+  }
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void *ptrs[] = {b->chunks, b->nchunks, b->slots, b->st, b->p64, b->p32, b->active};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->h_active) (void)hipHostFree(b->h_active);
+  if (b->h_st) (void)hipHostFree(b->h_st);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // namespace ehip
+
+// ================================================================ C ABI: batching
+
+extern "C" int ehip_batch_create(int nsys, ehip_system **systems, ehip_batch **out) {
+  if (nsys <= 0 || !systems || !out) return EHIP_EARG;
+  for (int i = 0; i < nsys; ++i)
+    if (!systems[i]) return EHIP_EARG;
+  // the engine is created per solve (its graph depends on the method and
+  // precision); the batch handle records the member systems
+  ehip_batch *b = new ehip_batch();
+  b->capacity = nsys;
+  b->batch_members.resize(nsys);
+  b->slot.resize(nsys);
+  for (int i = 0; i < nsys; ++i) b->slot[i].s = systems[i];
+  *out = b;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_free(ehip_batch *b) {
+  if (!b) return EHIP_OK;
+  delete b;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_solve(ehip_batch *bh, const ehip_params *p, ehip_report *reports, double *t_ms) {
+  if (!bh || !p) return EHIP_EARG;
+  const int nsys = bh->capacity;
+  std::vector<ehip_system *> sys(nsys);
+  for (int i = 0; i < nsys; ++i) sys[i] = bh->slot[i].s;
+  ehip_batch *e = nullptr;
+  int rc = engine_create(nsys, p, &e);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  EHIP_HIP_TRY(hipEventCreate(&e0));
+  EHIP_HIP_TRY(hipEventCreate(&e1));
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < nsys; ++i) {
+    int slot = engine_admit(e, sys[i]);
+    if (slot < 0) { engine_free(e); return slot; }
+    bh->batch_members[i] = slot;
+  }
+  if ((rc = upload_worklist(e))) { engine_free(e); return rc; }
+  EHIP_HIP_TRY(hipEventRecord(e0, e->stream));
+  // every replay runs graph_iters iterations of every live slot
+  const int gi = p->check_every > 0 ? p->check_every : 8;
+  const int64_t cap = (int64_t)p->maxit * (p->precision == EHIP_MIXED_IR ? 2 : 1) / gi + 64;
+  int64_t replays = 0;
+  for (;;) {
+    if ((rc = engine_step(e))) { engine_free(e); return rc; }
+    ++replays;
+    if (*e->h_active == 0 || replays > cap) break;
+  }
+  EHIP_HIP_TRY(hipEventRecord(e1, e->stream));
+  EHIP_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  for (int i = 0; i < nsys; ++i) {
+    const int slot = bh->batch_members[i];
+    ehip_report r{};
+    if (!e->h_st[slot].done) {
+      BSys &h = e->h_st[slot];
+      h.info = 2;  // replay cap reached before the slot finished
+    }
+    if ((rc = engine_retire(e, slot, &r))) { engine_free(e); return rc; }
+    r.t_krylov_ms = ms;
+    r.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (reports) reports[i] = r;
+  }
+  if (t_ms) *t_ms = ms;
+  engine_free(e);
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_get_solution(ehip_batch *b, int i, double *x) {
+  if (!b || i < 0 || i >= b->capacity || !x) return EHIP_EARG;
+  return ehip_get_solution(b->slot[i].s, x);
+}
+
+// ================================================================ C ABI: serving
+
+extern "C" int ehip_server_create(const ehip_params *p, int max_slots, int64_t mem_budget, ehip_batch **out) {
+  if (!p || max_slots <= 0 || !out) return EHIP_EARG;
+  ehip_batch *b = nullptr;
+  int rc = engine_create(max_slots, p, &b);
+  if (rc) return rc;
+  b->chunk_cap = 0;
+  b->bytes = 0;
+  b->graph_ok = false;
+  b->p.divtol = 0;
+  b->budget = mem_budget;
+  *out = b;
+  return EHIP_OK;
+}
+
+// Admits a system into a free slot if a slot and the HBM budget allow;
+// EHIP_ECAP otherwise (the caller keeps it queued).
+extern "C" int ehip_server_submit(ehip_batch *b, ehip_system *s, int *slot_out) {
+  if (!b || !s) return EHIP_EARG;
+  const int64_t budget = b->budget;
+  const int64_t need = slot_bytes(s, b->mixed != 0);
+  if (budget > 0 && b->bytes + need > budget) {
+    set_error("server: HBM budget %lld bytes would be exceeded", (long long)budget);
+    return EHIP_ECAP;
+  }
+  int slot = engine_admit(b, s);
+  if (slot < 0) return slot;
+  int rc = upload_worklist(b);
+  if (rc) return rc;
+  if (slot_out) *slot_out = slot;
+  return EHIP_OK;
+}
+
+// Runs one graph replay (graph_iters iterations of every live system), then
+// retires the finished ones: their systems get back-scaled solutions and
+// reports.  done_systems/done_reports receive up to max_done entries.
+extern "C" int ehip_server_step(ehip_batch *b, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
+                                int max_done) {
+  if (!b || !ndone) return EHIP_EARG;
+  *ndone = 0;
+  int live = 0;
+  for (auto &sl : b->slot) live += sl.used;
+  if (!live) return EHIP_OK;
+  int rc = engine_step(b);
+  if (rc) return rc;
+  bool changed = false;
+  for (int i = 0; i < b->capacity; ++i) {
+    auto &sl = b->slot[i];
+    if (!sl.used || !b->h_st[i].done) continue;
+    if (*ndone >= max_done) break;
+    ehip_system *s = sl.s;
+    ehip_report r{};
+    if ((rc = engine_retire(b, i, &r))) return rc;
+    if (done_systems) done_systems[*ndone] = s;
+    if (done_reports) done_reports[*ndone] = r;
+    (*ndone)++;
+    changed = true;
+  }
+  if (changed) return upload_worklist(b);
+  return EHIP_OK;
+}
+
+extern "C" int ehip_server_status(const ehip_batch *b, int *live, int64_t *bytes) {
+  if (!b) return EHIP_EARG;
+  int l = 0;
+  for (auto &sl : b->slot) l += sl.used;
+  if (live) *live = l;
+  if (bytes) *bytes = b->bytes;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_server_free(ehip_batch *b) {
+  engine_free(b);
+  return EHIP_OK;
+}

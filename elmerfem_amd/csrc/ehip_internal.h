@@ -92,6 +92,9 @@ struct ehip_system {
   double *emat = nullptr;         // nelem*ld*ld element matrices (row-major)
   double *evec = nullptr;         // nelem*ld element force vectors
   int64_t bytes = 0;
+  // sampled SpMV timing inside the Krylov loop
+  int prof_every = 0, prof_count = 0, prof_n = 0;
+  hipEvent_t prof_ev[2 * 512] = {};
 };
 
 namespace ehip {
@@ -115,5 +118,10 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
 int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale);
 
 int max_grid();
+
+// SpMV launch sampling (ehip_params.profile_every)
+void prof_begin(ehip_system *s);
+void prof_end(ehip_system *s);
+void prof_collect(ehip_system *s, double *mean_us, int *samples);
 
 }  // namespace ehip

@@ -236,14 +236,11 @@ namespace ehip {
 // ================================================================ Krylov kernels
 
 template <typename T>
-struct Vecs {
-  T *r, *z, *p, *q, *t, *dg, *b, *x;
-  T *rtld, *v, *sv, *t1v, *t2v;  // BiCGStab
-};
-
-template <typename T>
 __device__ __forceinline__ bool solve_done(const KStateT<T> *st) {
-  return __hip_atomic_load(&st->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+  // `done` only changes at the end of an earlier launch, so a plain load is
+  // coherent here; it is uniform, so it becomes one scalar load per wave
+  // instead of a same-address L2 request from every lane
+  return __builtin_amdgcn_readfirstlane(st->done) != 0;
 }
 
 #define EHIP_EPI_MEMBERS(T) \
@@ -259,11 +256,13 @@ struct EpiInit {
   static constexpr int NV = 3;
   EHIP_EPI_MEMBERS(T)
   __device__ void init() {}
-  __device__ void row(int i, T s, T (&acc)[3]) {
-    const T bi = v.b[i];
+  struct Pre { T b, dg; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.dg[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[3]) {
+    const T bi = pr.b;
     const T ri = bi - s;
     v.r[i] = ri;
-    const T zi = ri / v.dg[i];
+    const T zi = ri / pr.dg;
     v.z[i] = zi;
     acc[0] += ri * zi;
     acc[1] += ri * ri;
@@ -297,9 +296,11 @@ struct EpiCgFast1 {
   EHIP_EPI_MEMBERS(T)
   T beta;
   __device__ void init() { beta = st->beta; }
-  __device__ void row(int i, T s, T (&acc)[1]) {
-    const T qi = s + beta * v.q[i];
-    const T pi = v.z[i] + beta * v.p[i];
+  struct Pre { T q, z, p; };
+  __device__ Pre load(int i) const { return Pre{v.q[i], v.z[i], v.p[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+    const T qi = s + beta * pr.q;
+    const T pi = pr.z + beta * pr.p;
     v.q[i] = qi;
     v.p[i] = pi;
     acc[0] += pi * qi;
@@ -319,14 +320,26 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KState
   if (solve_done(st)) return;
   const T alpha = st->alpha;
   T acc[2] = {(T)0, (T)0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    v.x[i] = v.x[i] + alpha * v.p[i];
-    const T ri = v.r[i] - alpha * v.q[i];
-    v.r[i] = ri;
-    const T zi = ri / v.dg[i];
-    v.z[i] = zi;
-    acc[0] += ri * zi;
-    acc[1] += ri * ri;
+  for (int base = blockIdx.x * kVecBlock * kVI; base < n; base += gridDim.x * kVecBlock * kVI) {
+    T xk[kVI], pk[kVI], rk[kVI], qk[kVI], dk[kVI];
+#pragma unroll
+    for (int k = 0; k < kVI; ++k) {
+      const int i = base + k * kVecBlock + threadIdx.x;
+      if (i < n) { xk[k] = v.x[i]; pk[k] = v.p[i]; rk[k] = v.r[i]; qk[k] = v.q[i]; dk[k] = v.dg[i]; }
+    }
+#pragma unroll
+    for (int k = 0; k < kVI; ++k) {
+      const int i = base + k * kVecBlock + threadIdx.x;
+      if (i < n) {
+        v.x[i] = xk[k] + alpha * pk[k];
+        const T ri = rk[k] - alpha * qk[k];
+        v.r[i] = ri;
+        const T zi = ri / dk[k];
+        v.z[i] = zi;
+        acc[0] += ri * zi;
+        acc[1] += ri * ri;
+      }
+    }
   }
   T tot[2];
   if (last_block_reduce<T, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
@@ -361,10 +374,12 @@ struct EpiTrue {
   static constexpr int NV = 2;
   EHIP_EPI_MEMBERS(T)
   __device__ void init() {}
-  __device__ void row(int i, T s, T (&acc)[2]) {
-    const T ti = v.b[i] - s;
+  struct Pre { T b, dg; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.dg[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[2]) {
+    const T ti = pr.b - s;
     v.r[i] = ti;
-    const T zi = ti / v.dg[i];
+    const T zi = ti / pr.dg;
     v.z[i] = zi;
     acc[0] += ti * ti;
     acc[1] += ti * zi;
@@ -398,9 +413,11 @@ struct EpiCgQ {
   static constexpr int NV = 1;
   EHIP_EPI_MEMBERS(T)
   __device__ void init() {}
-  __device__ void row(int i, T s, T (&acc)[1]) {
+  struct Pre { T p; };
+  __device__ Pre load(int i) const { return Pre{v.p[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
     v.q[i] = s;
-    acc[0] += v.p[i] * s;
+    acc[0] += pr.p * s;
   }
   __device__ void finish(T (&tot)[1]) {
     if (threadIdx.x == 0) {
@@ -415,13 +432,25 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T
   if (solve_done(st)) return;
   const T alpha = st->alpha;
   T acc[1] = {(T)0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    v.x[i] = v.x[i] + alpha * v.p[i];
-    const T ri = v.r[i] - alpha * v.q[i];
-    v.r[i] = ri;
-    const T zi = ri / v.dg[i];
-    v.z[i] = zi;
-    acc[0] += ri * zi;
+  for (int base = blockIdx.x * kVecBlock * kVI; base < n; base += gridDim.x * kVecBlock * kVI) {
+    T xk[kVI], pk[kVI], rk[kVI], qk[kVI], dk[kVI];
+#pragma unroll
+    for (int k = 0; k < kVI; ++k) {
+      const int i = base + k * kVecBlock + threadIdx.x;
+      if (i < n) { xk[k] = v.x[i]; pk[k] = v.p[i]; rk[k] = v.r[i]; qk[k] = v.q[i]; dk[k] = v.dg[i]; }
+    }
+#pragma unroll
+    for (int k = 0; k < kVI; ++k) {
+      const int i = base + k * kVecBlock + threadIdx.x;
+      if (i < n) {
+        v.x[i] = xk[k] + alpha * pk[k];
+        const T ri = rk[k] - alpha * qk[k];
+        v.r[i] = ri;
+        const T zi = ri / dk[k];
+        v.z[i] = zi;
+        acc[0] += ri * zi;
+      }
+    }
   }
   T tot[1];
   if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0)
@@ -435,10 +464,12 @@ struct EpiCgResidual {
   double *hist;
   T beta;
   __device__ void init() { beta = st->rho_new / st->rho; }
-  __device__ void row(int i, T s, T (&acc)[1]) {
-    const T ti = s - v.b[i];
+  struct Pre { T b, z, p; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.z[i], v.p[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+    const T ti = s - pr.b;
     acc[0] += ti * ti;
-    v.p[i] = v.z[i] + beta * v.p[i];
+    v.p[i] = pr.z + beta * pr.p;
   }
   __device__ void finish(T (&tot)[1]) {
     if (threadIdx.x != 0) return;
@@ -475,9 +506,11 @@ struct EpiBiV {
   static constexpr int NV = 1;
   EHIP_EPI_MEMBERS(T)
   __device__ void init() {}
-  __device__ void row(int i, T s, T (&acc)[1]) {
+  struct Pre { T rtld; };
+  __device__ Pre load(int i) const { return Pre{v.rtld[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
     v.v[i] = s;
-    acc[0] += v.rtld[i] * s;
+    acc[0] += pr.rtld * s;
   }
   __device__ void finish(T (&tot)[1]) {
     if (threadIdx.x == 0) st->alpha = st->rho / tot[0];
@@ -512,9 +545,11 @@ struct EpiBiT {
   EHIP_EPI_MEMBERS(T)
   int active;
   __device__ void init() { active = !st->snorm_exit; }
-  __device__ void row(int i, T s, T (&acc)[2]) {
+  struct Pre { T sv; };
+  __device__ Pre load(int i) const { return Pre{v.sv[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[2]) {
     v.t[i] = s;
-    acc[0] += s * v.sv[i];
+    acc[0] += s * pr.sv;
     acc[1] += s * s;
   }
   __device__ void finish(T (&tot)[2]) {
@@ -601,12 +636,14 @@ struct EpiBiResidual {
     omega = st->omega;
     beta = (st->rho_new * st->alpha) / (st->rho * omega);
   }
-  __device__ void row(int i, T s, T (&acc)[1]) {
-    const T ti = s - v.b[i];
+  struct Pre { T b, r, p, v, dg; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.r[i], v.p[i], v.v[i], v.dg[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+    const T ti = s - pr.b;
     acc[0] += ti * ti;
-    const T pi = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
+    const T pi = pr.r + beta * (pr.p - omega * pr.v);
     v.p[i] = pi;
-    v.t1v[i] = pi / v.dg[i];
+    v.t1v[i] = pi / pr.dg;
   }
   __device__ void finish(T (&tot)[1]) {
     if (threadIdx.x != 0) return;
@@ -757,8 +794,10 @@ struct EpiOuter {
   unsigned *ticket;
   double *hist;
   __device__ void init() {}
-  __device__ void row(int i, double s, double (&acc)[1]) {
-    const double ti = b[i] - s;
+  struct Pre { double b; };
+  __device__ Pre load(int i) const { return Pre{b[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[1]) {
+    const double ti = pr.b - s;
     t[i] = ti;
     acc[0] += ti * ti;
   }
@@ -791,10 +830,12 @@ struct EpiResid {
   unsigned *ticket;
   double *out;
   __device__ void init() {}
-  __device__ void row(int i, double s, double (&acc)[2]) {
-    const double t = b[i] - s;
+  struct Pre { double b; };
+  __device__ Pre load(int i) const { return Pre{b[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[2]) {
+    const double t = pr.b - s;
     acc[0] += t * t;
-    acc[1] += b[i] * b[i];
+    acc[1] += pr.b * pr.b;
   }
   __device__ void finish(double (&tot)[2]) {
     if (threadIdx.x == 0) {
@@ -843,6 +884,7 @@ struct SolveCtx {
   double tol, divtol;
   double *hist;  // device, may be null
   int spmv;
+  bool prof;     // sample the SpMV launches of this loop
 };
 
 template <typename T>
@@ -887,7 +929,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
       for (;;) {
         for (int k = 0; k < c.check; ++k) {
           EpiCgQ<T> e1{v, st, part, pg, &st->ticket[0]};
+          if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.p, e1, st);
+          if (c.prof) prof_end(s);
           hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
           EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, (T)0};
           launch_epi<T>(s, plan, vals, v.x, e2, st);
@@ -901,7 +945,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
       for (;;) {
         for (int k = 0; k < c.check; ++k) {
           EpiCgFast1<T> e1{v, st, part, pg, &st->ticket[0], (T)0};
+          if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.z, e1, st);
+          if (c.prof) prof_end(s);
           hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part, c.hist);
           c.spmv++;
         }
@@ -921,7 +967,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
     for (;;) {
       for (int k = 0; k < c.check; ++k) {
         EpiBiV<T> e3{v, st, part, pg, &st->ticket[0]};
+        if (c.prof) prof_begin(s);
         launch_epi<T>(s, plan, vals, v.t1v, e3, st);
+        if (c.prof) prof_end(s);
         hipLaunchKernelGGL(k_bi_s<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
         hipLaunchKernelGGL(k_bi_snorm_exit<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
         hipLaunchKernelGGL(k_bi_snorm_mark<T>, dim3(1), dim3(1), 0, s->stream, st);
@@ -985,6 +1033,7 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   ci.hist = nullptr;
   ci.tol = inner_tol;
   ci.divtol = 0;
+  ci.prof = c.prof;
   for (;;) {
     EpiOuter eo{s->rhs, t64, st, s->partials, pg, &st->ticket[0], c.hist};
     launch_epi<double>(s, s->plan64, s->vals, s->x, eo, (const KState *)nullptr);
@@ -1109,6 +1158,10 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   c.divtol = p->divtol;
   c.check = p->check_every > 0 ? p->check_every : (n > 200000 ? 25 : 10);
   c.hist = dhist;
+  c.prof = p->profile_every > 0;
+  s->prof_every = p->profile_every;
+  s->prof_n = 0;
+  s->prof_count = 0;
   EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
   int inner_total = 0;
   if (mixed) {
@@ -1154,6 +1207,9 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   rep.t_krylov_ms = tk;
   rep.t_total_ms = tt;
   rep.spmv_count = c.spmv + 1;
+  prof_collect(s, &rep.spmv_us, &rep.spmv_samples);
+  rep.spmv_precision = mixed ? 1 : 0;
+  s->prof_every = 0;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(e2);
@@ -1179,4 +1235,21 @@ extern "C" int ehip_solve_csr(int n, const int *rows, const int *cols, const int
   if (!rc) rc = ehip_solve(s, p, x, r, nullptr);
   ehip_system_free(s);
   return rc;
+}
+
+// Unscaled FP64 residual ||b0 - A x|| / ||b0|| and ComputeNorm of the
+// system's current (back-scaled) solution.
+extern "C" int ehip_internal_report(ehip_system *s, const double *b0, double *res_true, double *norm) {
+  int rc;
+  if ((rc = ensure_state(s))) return rc;
+  const int n = s->n;
+  double *out = reinterpret_cast<double *>(&s->st->ts);
+  EpiResid er{b0, s->partials, max_grid(), &s->st->ticket[0], out};
+  launch_epi<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
+  hipLaunchKernelGGL(k_sumsq, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, s->partials,
+                     &s->st->ticket[1], &s->st->rr);
+  if ((rc = read_state(s, s->st, s->h_st))) return rc;
+  *res_true = sqrt(s->h_st->ts) / sqrt(s->h_st->tt);
+  *norm = sqrt(s->h_st->rr / (double)n);
+  return EHIP_OK;
 }

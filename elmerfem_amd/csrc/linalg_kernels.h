@@ -8,10 +8,21 @@ namespace ehip {
 
 constexpr int kVecBlock = 256;
 
+// The Krylov work vectors of one system (HUTI's work array columns).
+template <typename T>
+struct Vecs {
+  T *r, *z, *p, *q, *t, *dg, *b, *x;
+  T *rtld, *v, *sv, *t1v, *t2v;  // BiCGStab
+};
+
 // Grid for vector kernels: enough workgroups to fill 256 CUs, a multiple of 8.
+// Vector kernels give each lane kVI elements per tile (coalesced, all loads
+// of a tile issued before any arithmetic).  At most 4096 workgroups, so two
+// partial sums per workgroup fit the 4 x 2048 partials buffer.
+constexpr int kVI = 4;
 inline int vec_grid(int n) {
-  int g = (n + kVecBlock * 4 - 1) / (kVecBlock * 4);
-  if (g > 1024) g = 1024;
+  int g = (n + kVecBlock * kVI - 1) / (kVecBlock * kVI);
+  if (g > 4096) g = 4096;
   if (g < 8) g = 8;
   return (g + 7) / 8 * 8;
 }

@@ -93,8 +93,10 @@ struct EpiStore {
   A *partials = nullptr;
   int stride = 0;
   unsigned *ticket = nullptr;
+  struct Pre {};
+  __device__ Pre load(int) const { return Pre{}; }
   __device__ void init() {}
-  __device__ void row(int r, A s, A (&)[1]) { y[r] = s; }
+  __device__ void row(int r, A s, const Pre &, A (&)[1]) { y[r] = s; }
   __device__ void finish(A (&)[1]) {}
 };
 
@@ -151,6 +153,56 @@ int ensure_work32(ehip_system *s, int nvec) {
   if (rc) return rc;
   s->nwork32 = nvec;
   return EHIP_OK;
+}
+
+// ---------------------------------------------------------------- SpMV sampling
+
+void prof_begin(ehip_system *s) {
+  if (s->prof_every <= 0) return;
+  const bool take = (s->prof_count % s->prof_every) == 0 && s->prof_n < 512;
+  if (take) {
+    hipEvent_t &e = s->prof_ev[2 * s->prof_n];
+    if (!e) (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, s->stream);
+  }
+}
+
+void prof_end(ehip_system *s) {
+  if (s->prof_every <= 0) return;
+  const bool take = (s->prof_count % s->prof_every) == 0 && s->prof_n < 512;
+  if (take) {
+    hipEvent_t &e = s->prof_ev[2 * s->prof_n + 1];
+    if (!e) (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, s->stream);
+    s->prof_n++;
+  }
+  s->prof_count++;
+}
+
+// Mean over the samples, dropping launches that returned at once because the
+// solve had already finished inside the same block of queued iterations.
+void prof_collect(ehip_system *s, double *mean_us, int *samples) {
+  *mean_us = 0.0;
+  *samples = 0;
+  if (s->prof_n == 0) return;
+  (void)hipStreamSynchronize(s->stream);
+  std::vector<double> t;
+  for (int i = 0; i < s->prof_n; ++i) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, s->prof_ev[2 * i], s->prof_ev[2 * i + 1]) == hipSuccess) t.push_back(1000.0 * ms);
+  }
+  if (t.empty()) return;
+  std::vector<double> srt(t);
+  std::sort(srt.begin(), srt.end());
+  const double med = srt[srt.size() / 2];
+  double sum = 0;
+  int k = 0;
+  for (double v : t)
+    if (v >= 0.2 * med) { sum += v; ++k; }
+  *mean_us = k ? sum / k : 0.0;
+  *samples = k;
+  s->prof_n = 0;
+  s->prof_count = 0;
 }
 
 int ensure_state(ehip_system *s) {
@@ -277,6 +329,8 @@ int ehip_system_free(ehip_system *s) {
                   s->plan32.chunk_row};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (hipEvent_t e : s->prof_ev)
+    if (e) (void)hipEventDestroy(e);
   if (s->h_st) (void)hipHostFree(s->h_st);
   if (s->h_st32) (void)hipHostFree(s->h_st32);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -380,6 +434,47 @@ int ehip_spmv_bench(ehip_system *s, int precision, int reps, double *us_per_call
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return rc;
+}
+
+__global__ __launch_bounds__(256) void k_copy(int64_t n, const double4 *__restrict__ a, double4 *__restrict__ b) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double4 x0 = a[i], x1 = a[i + stride], x2 = a[i + 2 * stride], x3 = a[i + 3 * stride];
+    b[i] = x0;
+    b[i + stride] = x1;
+    b[i + 2 * stride] = x2;
+    b[i + 3 * stride] = x3;
+  }
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
+int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call) {
+  if (bytes < 1024 || reps <= 0 || !gbps) return EHIP_EARG;
+  const int64_t n4 = bytes / 32;
+  double4 *a = nullptr, *b = nullptr;
+  EHIP_HIP_TRY(hipMalloc(&a, n4 * 32));
+  EHIP_HIP_TRY(hipMalloc(&b, n4 * 32));
+  EHIP_HIP_TRY(hipMemset(a, 0, n4 * 32));
+  hipEvent_t e0, e1;
+  EHIP_HIP_TRY(hipEventCreate(&e0));
+  EHIP_HIP_TRY(hipEventCreate(&e1));
+  const int grid = 256 * 16;
+  hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, n4, a, b);
+  EHIP_HIP_TRY(hipEventRecord(e0, 0));
+  for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, n4, a, b);
+  EHIP_HIP_TRY(hipEventRecord(e1, 0));
+  EHIP_HIP_TRY(hipEventSynchronize(e1));
+  float ms = 0;
+  EHIP_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+  const double us = 1000.0 * ms / reps;
+  *gbps = 2.0 * (double)(n4 * 32) / (us * 1e3);
+  if (us_per_call) *us_per_call = us;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return EHIP_OK;
 }
 
 int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks) {

@@ -42,9 +42,10 @@ class LinearSystemKeywords:
     optimize_bandwidth: bool = True
 
     def to_params(self, precision: int = _lib.FP64, mode: int = _lib.FAITHFUL, inner_tol: float = 0.0,
-                  inner_maxit: int = 0, check_every: int = 0, record_history: bool = False) -> _lib.Params:
+                  inner_maxit: int = 0, check_every: int = 0, record_history: bool = False,
+                  profile_every: int = 0) -> _lib.Params:
         return _lib.Params(self.method, self.precond, self.tol, self.maxit, int(self.scaling), precision, mode,
-                           inner_tol, inner_maxit, check_every, self.divtol, int(record_history))
+                           inner_tol, inner_maxit, check_every, self.divtol, int(record_history), profile_every)
 
 
 def _method(name: str | None) -> tuple[int, str]:

@@ -150,9 +150,9 @@ class System:
 
     # ------------------------------------------------------------ solve
     def solve(self, kw: LinearSystemKeywords | _lib.Params, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL,
-              inner_tol=0.0, inner_maxit=0, check_every=0, history=False, return_x=True):
+              inner_tol=0.0, inner_maxit=0, check_every=0, history=False, return_x=True, profile_every=0):
         p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every,
-                                                              history)
+                                                              history, profile_every)
         if isinstance(kw, _lib.Params):
             p.record_history = int(history)
         x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
@@ -186,6 +186,14 @@ class System:
         b, nb, ch = C.c_int(), C.c_int(), C.c_int()
         _lib.check(_lib.load().ehip_spmv_config(self._h, C.byref(b), C.byref(nb), C.byref(ch)), "ehip_spmv_config")
         return {"block": b.value, "nnz_per_block": nb.value, "chunks": ch.value}
+
+
+def copy_bench(nbytes: int, reps: int = 20):
+    """Plain device copy of nbytes: (GB/s counting read + write, us per call)."""
+    L = _lib.init()
+    g, us = C.c_double(), C.c_double()
+    _lib.check(L.ehip_copy_bench(int(nbytes), reps, C.byref(g), C.byref(us)), "ehip_copy_bench")
+    return g.value, us.value
 
 
 def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL):

@@ -47,6 +47,17 @@ def poisson_hex8(n: int, sigma: float = 0.0, seed: int = DEFAULT_SEED, method="c
     }
 
 
+def poisson_hex27(n: int, sigma: float = 0.0, seed: int = DEFAULT_SEED) -> dict:
+    m = M.hex27_cube(n)
+    k = np.ones(m.nelem) if sigma == 0 else M.lognormal(seed, m.nelem, 0.0, sigma)
+    return {
+        "name": "poisson_hex27_%d" % n, "mesh": m, "physics": 0, "dofs": 1,
+        "coef": k, "conv": None, "velo": None, "load": np.ones(m.nelem),
+        "bc_nodes": m.boundary_nodes(), "kw": _kw("cg"),
+        "info": {"n": n, "sigma": sigma, "seed": seed},
+    }
+
+
 def elasticity_hex(n: int, quadratic: bool = False) -> dict:
     m = M.hex27_cube(n) if quadratic else M.hex8_cube(n)
     return {

@@ -71,6 +71,7 @@ typedef struct {
   int check_every;   /* host polls convergence every k iterations (0 = default) */
   double divtol;     /* "Linear System Divergence Tolerance" (0 = none) */
   int record_history;/* 1: keep the residual of every iteration */
+  int profile_every; /* > 0: time every k-th SpMV launch of the Krylov loop with HIP events */
 } ehip_params;
 
 typedef struct {
@@ -83,6 +84,9 @@ typedef struct {
   double norm;          /* ComputeNorm: sqrt(sum x^2 / n) */
   double t_total_ms;    /* wall time of the solve call on the device stream */
   double t_krylov_ms;   /* time inside the Krylov iteration */
+  double spmv_us;       /* mean duration of the sampled SpMV launches (profile_every > 0) */
+  int spmv_samples;     /* number of sampled launches */
+  int spmv_precision;   /* 0: the sampled launches were FP64, 1: FP32 */
 } ehip_report;
 
 /* ---- library ---- */
@@ -166,6 +170,10 @@ int ehip_spmv(ehip_system *s, int precision, const double *u, double *v);
 /* Times `reps` SpMV launches on device-resident vectors; returns the mean
  * kernel time in microseconds (HIP events on the system's stream). */
 int ehip_spmv_bench(ehip_system *s, int precision, int reps, double *us_per_call);
+/* Streams a device buffer of `bytes` through a plain copy kernel `reps`
+ * times; returns the achieved GB/s (read + write) -- the measured HBM
+ * ceiling, and the calibration kernel for hardware byte counters. */
+int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
 /* Chosen SpMV schedule: block size and nnz per block (autotuned). */
 int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks);
 int ehip_spmv_tune(ehip_system *s, int precision);
@@ -181,6 +189,22 @@ int ehip_batch_free(ehip_batch *b);
 /* reports[nsys]; solutions stay on the device (ehip_batch_get_solution). */
 int ehip_batch_solve(ehip_batch *b, const ehip_params *p, ehip_report *reports, double *t_ms);
 int ehip_batch_get_solution(ehip_batch *b, int i, double *x);
+
+/* ---- serving: continuous batching ----
+ * A server owns the multi-system engine.  Systems (assembled, Dirichlet
+ * rows applied) are admitted while a slot is free and the HBM budget
+ * (mem_budget bytes, 0 = none) holds their solver workspace; each step
+ * replays the iteration graph once for every live system and retires the
+ * finished ones, whose solutions and reports become available.  Requests
+ * that arrive while others run join at the next step. */
+typedef struct ehip_batch ehip_server;
+int ehip_server_create(const ehip_params *p, int max_slots, int64_t mem_budget, ehip_server **out);
+/* EHIP_ECAP: no free slot or over budget -- keep the request queued. */
+int ehip_server_submit(ehip_server *srv, ehip_system *s, int *slot);
+int ehip_server_step(ehip_server *srv, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
+                     int max_done);
+int ehip_server_status(const ehip_server *srv, int *live, int64_t *bytes);
+int ehip_server_free(ehip_server *srv);
 
 #ifdef __cplusplus
 }
