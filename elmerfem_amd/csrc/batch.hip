@@ -173,34 +173,44 @@ __global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
     } else {
       for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) op.row(cx, r, (T)0, acc);
     }
-    // publish this chunk's partials, take the slot's ticket
-    T bs[NV > 0 ? NV : 1];
+    // publish this chunk's partials (plain stores; the finalize launch reads them)
+    if constexpr (NV > 0) {
+      T bs[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) bs[k] = block_sum(acc[k], sh + 16 * k);
-    T *part = Op::template partials<T>(a);
-    if (threadIdx.x == 0) {
+      for (int k = 0; k < NV; ++k) bs[k] = block_sum(acc[k], sh + 16 * k);
+      T *part = Op::template partials<T>(a);
+      if (threadIdx.x == 0) {
 #pragma unroll
-      for (int k = 0; k < NV; ++k)
-        __hip_atomic_store(&part[(size_t)(sp.ck0 + ck.k) * 4 + k], bs[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      const unsigned t = atomicAdd(&st.ticket, 1u);
-      amlast = (t == (unsigned)sp.nck - 1) ? 1 : 0;
+        for (int k = 0; k < NV; ++k) part[(size_t)(sp.ck0 + ck.k) * 4 + k] = bs[k];
+      }
     }
     __syncthreads();
-    if (amlast) {
-      __threadfence();
-      T tot[NV > 0 ? NV : 1];
+  }
+  (void)amlast;
+}
+
+// Per-slot finalize: one workgroup per slot sums the slot's chunk partials
+// in chunk order and runs the op's scalar update (phase changes included).
+template <class Op>
+__global__ __launch_bounds__(256) void bk_fin(BatchArgs a, Op op, int nslots) {
+  using T = typename Op::T;
+  constexpr int NV = Op::NV;
+  __shared__ T sh[16 * 4];
+  for (int slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+    BSys &st = a.st[slot];
+    const SlotPtrs sp = a.slots[slot];
+    if (sp.nck == 0 || !op.active(st)) continue;
+    T tot[NV > 0 ? NV : 1];
+    if constexpr (NV > 0) {
+      const T *part = Op::template partials<T>(a);
 #pragma unroll
       for (int k = 0; k < NV; ++k) {
         T s = (T)0;
-        for (int i = threadIdx.x; i < sp.nck; i += BS) s += load_coherent(&part[(size_t)(sp.ck0 + i) * 4 + k]);
+        for (int i = threadIdx.x; i < sp.nck; i += blockDim.x) s += part[(size_t)(sp.ck0 + i) * 4 + k];
         tot[k] = block_sum(s, sh + 16 * k);
       }
-      if (threadIdx.x == 0) {
-        st.ticket = 0;
-        op.finish(a, st, sp, tot);
-      }
     }
+    if (threadIdx.x == 0) op.finish(a, st, sp, tot);
     __syncthreads();
   }
 }
@@ -821,6 +831,8 @@ template <class Op>
 static void launch_op(ehip_batch *b, const Op &op) {
   const int grid = max_grid();
   hipLaunchKernelGGL((bk_run<Op, 256, kChunkNB>), dim3(grid), dim3(256), 0, b->stream, args_of(b), op);
+  const int fgrid = std::min(b->capacity, 1024);
+  hipLaunchKernelGGL((bk_fin<Op>), dim3(fgrid), dim3(256), 0, b->stream, args_of(b), op, b->capacity);
 }
 
 // one iteration of every active slot

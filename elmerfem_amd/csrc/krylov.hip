@@ -7,11 +7,13 @@
 // with the TRESID_SCALED_BYB stop test, BackScaleLinearSystem (:6754-6866)
 // and ComputeNorm (:4887-5006).
 //
-// Every reduction finishes inside its producing kernel (last-workgroup
-// ticket, fixed summation order), every kernel reads the device-side `done`
-// flag first and returns when the solve has ended, so the host enqueues
-// blocks of iterations and polls the scalars only every `check_every`
-// iterations.
+// Every reduction is two launches on the solve's stream: the producing
+// kernel publishes one partial per workgroup with plain stores and a
+// one-workgroup finalize sums them in a fixed order (run-to-run
+// deterministic) and updates the Krylov scalars.  Every kernel reads the
+// device-side `done` flag first and returns when the solve has ended, so the
+// host enqueues blocks of iterations and polls the scalars only every
+// `check_every` iterations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,9 +76,16 @@ __global__ __launch_bounds__(kVecBlock) void k_scale_rhs(int n, double *__restri
     b[i] = v;
     acc[0] += v * v;
   }
+  (void)ticket;
+  (void)out;
+  block_publish<double, 1>(acc, partials, gridDim.x);
+}
+
+// one workgroup: *out = sum of g partials
+__global__ __launch_bounds__(256) void k_fin_sum(const double *partials, int g, double *out) {
   double tot[1];
-  if (last_block_reduce<double, 1>(acc, partials, gridDim.x, ticket, tot))
-    if (threadIdx.x == 0) *out = tot[0];
+  block_reduce_all<double, 1>(partials, g, g, tot);
+  if (threadIdx.x == 0) *out = tot[0];
 }
 
 // D *= bnorm; b /= bnorm; x /= D
@@ -116,6 +125,7 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
     const int g = vec_grid(n);
     hipLaunchKernelGGL(k_scale_rhs, dim3(g), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, s->partials,
                        &s->st->ticket[0], dsum);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s->stream, s->partials, g, dsum);
     if (rhs_scaling) {
       double ss = 0;
       EHIP_HIP_TRY(hipMemcpyAsync(&ss, dsum, sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -341,8 +351,18 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KState
       }
     }
   }
-  T tot[2];
-  if (last_block_reduce<T, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+  block_publish<T, 2>(acc, partials, gridDim.x);
+}
+
+template <typename T>
+struct FinCgFast2 {
+  static constexpr int NV = 2;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  double *hist;
+  __device__ void finish(T (&tot)[2]) {
+    if (threadIdx.x != 0) return;
     const T rho_old = st->rho;
     st->rho = tot[0];
     st->beta = tot[0] / rho_old;
@@ -366,6 +386,17 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KState
       st->done = 1;
     }
   }
+};
+
+// The one-workgroup finalize: sums the producing launch's partials in a
+// fixed order and runs its scalar update.  Skipped, like its producer, once
+// the solve is done.
+template <typename T, class Fin>
+__global__ __launch_bounds__(256) void k_fin(Fin f, int g, const KStateT<T> *check) {
+  if (check && solve_done(check)) return;
+  T tot[Fin::NV];
+  block_reduce_all<T, Fin::NV>(f.partials, f.stride, g, tot);
+  f.finish(tot);
 }
 
 // true residual t = b - A x ; restart r = t, z = t/dg ; partials t.t, t.z
@@ -452,10 +483,19 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T
       }
     }
   }
-  T tot[1];
-  if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0)
-    st->rho_new = tot[0];
+  block_publish<T, 1>(acc, partials, gridDim.x);
 }
+
+template <typename T>
+struct FinRhoNew {
+  static constexpr int NV = 1;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x == 0) st->rho_new = tot[0];
+  }
+};
 
 template <typename T>
 struct EpiCgResidual {
@@ -528,8 +568,17 @@ __global__ __launch_bounds__(kVecBlock) void k_bi_s(int n, Vecs<T> v, KStateT<T>
     v.t2v[i] = si / v.dg[i];
     acc[0] += si * si;
   }
-  T tot[1];
-  if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+  block_publish<T, 1>(acc, partials, gridDim.x);
+}
+
+template <typename T>
+struct FinSnorm {
+  static constexpr int NV = 1;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x != 0) return;
     const T sn = sqrt(tot[0]);
     st->snorm = sn;
     if (sn < (T)1.17549435E-38f) {  // HUTI_EPSILON (huti_fdefs.h)
@@ -537,7 +586,7 @@ __global__ __launch_bounds__(kVecBlock) void k_bi_s(int n, Vecs<T> v, KStateT<T>
       st->info = 36;
     }
   }
-}
+};
 
 template <typename T>
 struct EpiBiT {
@@ -587,8 +636,22 @@ __global__ __launch_bounds__(kVecBlock) void k_bi_xr(int n, Vecs<T> v, KStateT<T
     acc[0] += v.rtld[i] * ri;
     acc[1] += ri * ri;
   }
-  T tot[2];
-  if (last_block_reduce<T, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+  (void)fast;
+  (void)hist;
+  block_publish<T, 2>(acc, partials, gridDim.x);
+}
+
+template <typename T>
+struct FinBiXr {
+  static constexpr int NV = 2;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  int fast;
+  double *hist;
+  __device__ void finish(T (&tot)[2]) {
+    if (threadIdx.x != 0) return;
+    const T alpha = st->alpha, omega = st->omega;
     st->rho_new = tot[0];
     st->rr = tot[1];
     if (fast) {
@@ -624,7 +687,7 @@ __global__ __launch_bounds__(kVecBlock) void k_bi_xr(int n, Vecs<T> v, KStateT<T
       }
     }
   }
-}
+};
 
 template <typename T>
 struct EpiBiResidual {
@@ -699,8 +762,17 @@ __global__ __launch_bounds__(kVecBlock) void k_bi_start(int n, Vecs<T> v, KState
     v.t1v[i] = ri / v.dg[i];
     acc[0] += ri * ri;
   }
-  T tot[1];
-  if (last_block_reduce<T, 1>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+  block_publish<T, 1>(acc, partials, gridDim.x);
+}
+
+template <typename T>
+struct FinBiStart {
+  static constexpr int NV = 1;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  __device__ void finish(T (&tot)[1]) {
+    if (threadIdx.x != 0) return;
     st->rho = tot[0];
     st->omega = (T)1;
     st->alpha = (T)0;
@@ -709,7 +781,7 @@ __global__ __launch_bounds__(kVecBlock) void k_bi_start(int n, Vecs<T> v, KState
       st->done = 1;
     }
   }
-}
+};
 
 // dg = a_ii, or 1 when |a_ii| <= AEPS or without preconditioning (CRS_DiagPrecondition)
 template <typename T>
@@ -765,8 +837,17 @@ __global__ __launch_bounds__(kVecBlock) void k_to32(int n, const double *__restr
     acc[0] += ri * zi;
     acc[1] += ri * ri;
   }
-  float tot[2];
-  if (last_block_reduce<float, 2>(acc, partials, gridDim.x, &st->ticket[1], tot) && threadIdx.x == 0) {
+  (void)st;
+  block_publish<float, 2>(acc, partials, gridDim.x);
+}
+
+struct FinTo32 {
+  static constexpr int NV = 2;
+  KState32 *st;
+  float *partials;
+  int stride;
+  __device__ void finish(float (&tot)[2]) {
+    if (threadIdx.x != 0) return;
     st->rho = tot[0];
     st->rr = tot[1];
     st->rhsnorm = sqrtf(tot[1]);
@@ -776,7 +857,7 @@ __global__ __launch_bounds__(kVecBlock) void k_to32(int n, const double *__restr
     st->info = 0;
     st->rec_conv = 0;
   }
-}
+};
 
 // x64 += scale * d32
 __global__ void k_correct(int n, double *x, const float *__restrict__ d, double scale) {
@@ -849,8 +930,9 @@ __global__ __launch_bounds__(kVecBlock) void k_sumsq(int n, const double *x, dou
                                                      double *out) {
   double acc[1] = {0.0};
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc[0] += x[i] * x[i];
-  double tot[1];
-  if (last_block_reduce<double, 1>(acc, partials, gridDim.x, ticket, tot) && threadIdx.x == 0) *out = tot[0];
+  (void)ticket;
+  (void)out;
+  block_publish<double, 1>(acc, partials, gridDim.x);
 }
 
 // ================================================================ host drivers
@@ -870,6 +952,29 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
   EHIP_EPI_CASE(512, 4096)
   EHIP_EPI_CASE(256, 4096)
 #undef EHIP_EPI_CASE
+}
+
+// SpMV with epilogue, then its one-workgroup finalize
+template <typename T, class Epi>
+static void launch_epi_fin(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
+                           const KStateT<T> *st) {
+  launch_epi<T, Epi>(s, p, vals, xin, e, st);
+  if constexpr (Epi::NV > 0)
+    hipLaunchKernelGGL((k_fin<T, Epi>), dim3(1), dim3(256), 0, s->stream, e, p.grid, st);
+}
+
+// vector kernel's finalize
+template <typename T, class Fin>
+static void fin(ehip_system *s, const Fin &f, int g, const KStateT<T> *st) {
+  hipLaunchKernelGGL((k_fin<T, Fin>), dim3(1), dim3(256), 0, s->stream, f, g, st);
+}
+
+// *out = sum of x_i^2 (two launches, no in-kernel fence)
+static void sumsq(ehip_system *s, int n, const double *x, double *out) {
+  const int g = vec_grid(n);
+  hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(kVecBlock), 0, s->stream, n, x, s->partials, (unsigned *)nullptr,
+                     (double *)nullptr);
+  hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s->stream, s->partials, g, out);
 }
 
 template <typename T>
@@ -917,7 +1022,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
   if (init_residual) {
     hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0);
     EpiInit<T> e{v, st, part, pg, &st->ticket[0]};
-    launch_epi<T>(s, plan, vals, v.x, e, (const KStateT<T> *)nullptr);
+    launch_epi_fin<T>(s, plan, vals, v.x, e, (const KStateT<T> *)nullptr);
     c.spmv++;
   } else {
     // mixed inner solve: k_to32 prepared r, z, rho; x (= d) is zero
@@ -932,9 +1037,11 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.p, e1, st);
           if (c.prof) prof_end(s);
+          fin<T>(s, e1, plan.grid, st);
           hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+          fin<T>(s, FinRhoNew<T>{st, part, g}, g, st);
           EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, (T)0};
-          launch_epi<T>(s, plan, vals, v.x, e2, st);
+          launch_epi_fin<T>(s, plan, vals, v.x, e2, st);
           c.spmv += 2;
         }
         if ((rc = read_state(s, st, hst))) return rc;
@@ -948,7 +1055,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.z, e1, st);
           if (c.prof) prof_end(s);
+          fin<T>(s, e1, plan.grid, st);
           hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part, c.hist);
+          fin<T>(s, FinCgFast2<T>{st, part, g, c.hist}, g, st);
           c.spmv++;
         }
         if ((rc = read_state(s, st, hst))) return rc;
@@ -956,7 +1065,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
         if (!hst->rec_conv || !init_residual) break;  // inner solves stop on the recurrence
         // recurrence converged: confirm with the true residual, restart from it otherwise
         EpiTrue<T> et{v, st, part, pg, &st->ticket[0]};
-        launch_epi<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
+        launch_epi_fin<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
         c.spmv++;
         if ((rc = read_state(s, st, hst))) return rc;
         if (hst->done) break;
@@ -964,23 +1073,27 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
     }
   } else {
     hipLaunchKernelGGL(k_bi_start<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+    fin<T>(s, FinBiStart<T>{st, part, g}, g, st);
     for (;;) {
       for (int k = 0; k < c.check; ++k) {
         EpiBiV<T> e3{v, st, part, pg, &st->ticket[0]};
         if (c.prof) prof_begin(s);
         launch_epi<T>(s, plan, vals, v.t1v, e3, st);
         if (c.prof) prof_end(s);
+        fin<T>(s, e3, plan.grid, st);
         hipLaunchKernelGGL(k_bi_s<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+        fin<T>(s, FinSnorm<T>{st, part, g}, g, st);
         hipLaunchKernelGGL(k_bi_snorm_exit<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
         hipLaunchKernelGGL(k_bi_snorm_mark<T>, dim3(1), dim3(1), 0, s->stream, st);
         EpiBiT<T> e6{v, st, part, pg, &st->ticket[0], 1};
-        launch_epi<T>(s, plan, vals, v.t2v, e6, st);
+        launch_epi_fin<T>(s, plan, vals, v.t2v, e6, st);
         hipLaunchKernelGGL(k_bi_xr<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part,
                            c.mode == EHIP_FAST ? 1 : 0, c.hist);
+        fin<T>(s, FinBiXr<T>{st, part, g, c.mode == EHIP_FAST ? 1 : 0, c.hist}, g, st);
         c.spmv += 2;
         if (c.mode == EHIP_FAITHFUL) {
           EpiBiResidual<T> e8{v, st, part, pg, &st->ticket[2], c.hist, (T)0, (T)0};
-          launch_epi<T>(s, plan, vals, v.x, e8, st);
+          launch_epi_fin<T>(s, plan, vals, v.x, e8, st);
           c.spmv++;
         } else {
           hipLaunchKernelGGL(k_bi_p<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
@@ -990,11 +1103,12 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
       if (!hst->done) continue;
       if (c.mode == EHIP_FAITHFUL || !hst->rec_conv || !init_residual) break;
       EpiTrue<T> et{v, st, part, pg, &st->ticket[0]};
-      launch_epi<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
+      launch_epi_fin<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
       c.spmv++;
       if ((rc = read_state(s, st, hst))) return rc;
       if (hst->done) break;
       hipLaunchKernelGGL(k_bi_start<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
+      fin<T>(s, FinBiStart<T>{st, part, g}, g, st);
     }
   }
   EHIP_HIP_TRY(hipGetLastError());
@@ -1020,8 +1134,7 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   KState *st = s->st;
   // ||b|| of the scaled system and the outer state
   hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0);
-  hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->rhs, s->partials, &st->ticket[1],
-                     &st->rr);
+  sumsq(s, n, s->rhs, &st->rr);
   if ((rc = read_state(s, st, s->h_st))) return rc;
   const double bn = sqrt(s->h_st->rr);
   EHIP_HIP_TRY(hipMemcpyAsync(&st->rhsnorm, &bn, sizeof(double), hipMemcpyHostToDevice, s->stream));
@@ -1036,7 +1149,7 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   ci.prof = c.prof;
   for (;;) {
     EpiOuter eo{s->rhs, t64, st, s->partials, pg, &st->ticket[0], c.hist};
-    launch_epi<double>(s, s->plan64, s->vals, s->x, eo, (const KState *)nullptr);
+    launch_epi_fin<double>(s, s->plan64, s->vals, s->x, eo, (const KState *)nullptr);
     c.spmv++;
     if ((rc = read_state(s, st, s->h_st))) return rc;
     if (s->h_st->done) break;
@@ -1044,6 +1157,8 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
     const double scale = s->h_st->scale;
     hipLaunchKernelGGL(k_to32, dim3(g), dim3(kVecBlock), 0, s->stream, n, t64, 1.0 / scale, v.r, v.z, v.x, v.dg,
                        s->st32, s->partials32);
+    hipLaunchKernelGGL((k_fin<float, FinTo32>), dim3(1), dim3(256), 0, s->stream, FinTo32{s->st32, s->partials32, g},
+                       g, (const KState32 *)nullptr);
     ci.maxit = std::min(inner_maxit, c.maxit - *inner_total);
     ci.spmv = 0;
     if ((rc = krylov<float>(s, ci, s->plan32, s->vals32, v, s->st32, s->h_st32, s->partials32, false))) return rc;
@@ -1103,8 +1218,7 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
     EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
   }
   // bnorm and the zero right-hand side exit (SolverUtils.src:7324-7341)
-  hipLaunchKernelGGL(k_sumsq, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rhs, s->partials,
-                     &s->st->ticket[1], &s->st->rr);
+  sumsq(s, n, s->rhs, &s->st->rr);
   if ((rc = read_state(s, s->st, s->h_st))) return rc;
   const double b2 = s->h_st->rr;
   if (!(sqrt(b2) > DBL_MIN)) {
@@ -1191,9 +1305,8 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   {
     double *out = reinterpret_cast<double *>(&s->st->ts);
     EpiResid er{b0, s->partials, max_grid(), &s->st->ticket[0], out};
-    launch_epi<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
-    hipLaunchKernelGGL(k_sumsq, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, s->partials,
-                       &s->st->ticket[1], &s->st->rr);
+    launch_epi_fin<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
+    sumsq(s, n, s->x, &s->st->rr);
     if ((rc = read_state(s, s->st, s->h_st))) return rc;
     rep.res_true = sqrt(s->h_st->ts) / sqrt(s->h_st->tt);
     rep.norm = sqrt(s->h_st->rr / (double)n);
@@ -1245,9 +1358,8 @@ extern "C" int ehip_internal_report(ehip_system *s, const double *b0, double *re
   const int n = s->n;
   double *out = reinterpret_cast<double *>(&s->st->ts);
   EpiResid er{b0, s->partials, max_grid(), &s->st->ticket[0], out};
-  launch_epi<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
-  hipLaunchKernelGGL(k_sumsq, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, s->partials,
-                     &s->st->ticket[1], &s->st->rr);
+  launch_epi_fin<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
+  sumsq(s, n, s->x, &s->st->rr);
   if ((rc = read_state(s, s->st, s->h_st))) return rc;
   *res_true = sqrt(s->h_st->ts) / sqrt(s->h_st->tt);
   *norm = sqrt(s->h_st->rr / (double)n);

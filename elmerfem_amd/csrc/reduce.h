@@ -38,6 +38,37 @@ __device__ __forceinline__ T load_coherent(const T *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Writes the workgroup's NV partial sums to partials[k*stride + blockIdx.x]
+// with plain stores.  A later launch on the same stream (the one-workgroup
+// finalize kernel) reads them: kernel boundaries order the memory, so no
+// fence is needed here.  (An agent-scope fence on gfx950 writes back and
+// invalidates the whole L2 of the XCD -- per workgroup that costs more than
+// the reduction saves.)
+template <typename T, int NV>
+__device__ __forceinline__ void block_publish(const T (&v)[NV], T *partials, int stride) {
+  __shared__ T sh[16 * NV];
+  T bs[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) bs[k] = block_sum(v[k], sh + 16 * k);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[k * stride + blockIdx.x] = bs[k];
+  }
+}
+
+// Sums g published partials per value in a fixed order; one workgroup, every
+// thread receives the totals.
+template <typename T, int NV>
+__device__ __forceinline__ void block_reduce_all(const T *partials, int stride, int g, T (&tot)[NV]) {
+  __shared__ T sh[16 * NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    T a = (T)0;
+    for (int i = threadIdx.x; i < g; i += blockDim.x) a += partials[k * stride + i];
+    tot[k] = block_sum(a, sh + 16 * k);
+  }
+}
+
 // Publishes NV partial sums of this workgroup (partials[k*stride + blockIdx])
 // and returns true in the workgroup that arrived last; that workgroup then
 // finds the totals in tot[0..NV).  `ticket` must start at 0; it is reset by

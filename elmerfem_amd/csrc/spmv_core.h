@@ -28,7 +28,7 @@ __device__ __forceinline__ T ntload(const T *p) {
 //   struct Pre;  Pre load(int r);            row operands, loaded early
 //   __device__ void init();                  read scalars once per workgroup
 //   __device__ void row(int r, A s, const Pre &, A (&acc)[NV]);
-//   __device__ void finish(A (&tot)[NV]);    run by the last workgroup
+//   __device__ void finish(A (&tot)[NV]);    run by the finalize kernel
 //   A *partials; int stride; unsigned *ticket;
 template <typename V, typename A, int BS, int NB, class Epi>
 __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
@@ -103,10 +103,7 @@ __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
       __syncthreads();
     }
   }
-  if constexpr (NV > 0) {
-    A tot[NV];
-    if (last_block_reduce<A, NV>(acc, epi.partials, epi.stride, epi.ticket, tot)) epi.finish(tot);
-  }
+  if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
 }
 
 }  // namespace ehip
