@@ -79,7 +79,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-forms", action="store_true", help="time only the headline form")
     args = ap.parse_args()

@@ -54,6 +54,7 @@ SIGNATURES = {
     "ehip_last_error": (C.c_char_p, []),
     "ehip_version": (C.c_int, []),
     "ehip_matvec": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_matvec_": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ehip_matvec_free": (None, [_vp]),
     "ehip_system_create": (C.c_int, [C.c_int, _i64, _vp, _vp, _vp, _vp]),
     "ehip_system_from_mesh": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int, C.c_int,

@@ -565,6 +565,12 @@ void ehip_matvec(void **spmv, int *n, int *rows, int *cols, double *vals, double
   ehip_spmv(c->s, 0, u, v);
 }
 
+// The name ElmerSolver's loader asks dlsym for when it was built with the
+// lower-case + underscore Fortran link type (fortranMangle, fem/src/Load.c:183-208).
+void ehip_matvec_(void **spmv, int *n, int *rows, int *cols, double *vals, double *u, double *v, int *reinit) {
+  ehip_matvec(spmv, n, rows, cols, vals, u, v, reinit);
+}
+
 void ehip_matvec_free(void **spmv) {
   if (!spmv || !*spmv) return;
   MatvecCache *c = static_cast<MatvecCache *>(*spmv);

@@ -102,6 +102,12 @@ int ehip_version(void);
  * when their checksum changes. */
 void ehip_matvec(void **spmv, int *n, int *rows, int *cols, double *vals, double *u, double *v,
                  int *reinit);
+/* The same entry under the name ElmerSolver's loader looks up when it was
+ * built with the lower-case + trailing-underscore Fortran link type
+ * (fortranMangle, fem/src/Load.c:183-208): "Matrix Vector Proc =
+ * "libehip" "ehip_matvec"" in a .sif resolves to this symbol there. */
+void ehip_matvec_(void **spmv, int *n, int *rows, int *cols, double *vals, double *u, double *v,
+                  int *reinit);
 /* Releases what ehip_matvec cached under *spmv. */
 void ehip_matvec_free(void **spmv);
 
@@ -179,10 +185,11 @@ int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks
 int ehip_spmv_tune(ehip_system *s, int precision);
 
 /* ---- batching: many independent systems in one multi-system launch ----
- * Systems keep their own device memory; the batch concatenates their
- * scaled matrices into one block-diagonal CRS and runs per-system scalars
- * (alpha, beta, omega, rho, convergence) in every launch of the iteration,
- * replayed from a captured hipGraph. */
+ * Systems keep their own device memory.  The batch builds one device work
+ * list of (system, row range) chunks over all of them; every launch of an
+ * iteration walks that list, each system keeping its own scalars (alpha,
+ * beta, omega, rho, convergence) and phase, and the iteration is replayed
+ * from a captured hipGraph. */
 typedef struct ehip_batch ehip_batch;
 int ehip_batch_create(int nsys, ehip_system **systems, ehip_batch **out);
 int ehip_batch_free(ehip_batch *b);

@@ -1,0 +1,100 @@
+"""The C ABI without a GPU: libehip.so loads and exports every function
+include/ehip.h declares, the ctypes structs match the C layout, and the
+Fortran interface module declares every entry point it binds with the C
+name the header gives.  No compute call is made here."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+from elmerfem_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ehip.h")
+FORTRAN = os.path.join(ROOT, "elmerfem_amd", "fortran", "ehip.F90")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ehip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "ehip_matvec" in names and "ehip_solve" in names and "ehip_server_step" in names
+    assert len(names) >= 30
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    assert set(declared_functions()) == set(_lib.SIGNATURES)
+    _lib.load(require_all=True)
+
+
+def test_struct_layout_matches_c():
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "ehip.h"
+#define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+  printf("ehip_params %zu\nehip_report %zu\n", sizeof(ehip_params), sizeof(ehip_report));
+  P(ehip_params, tol) P(ehip_params, inner_tol) P(ehip_params, divtol) P(ehip_params, profile_every)
+  P(ehip_report, res_scaled) P(ehip_report, spmv_us) P(ehip_report, spmv_precision)
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        exe = os.path.join(d, "l")
+        open(c, "w").write(src)
+        subprocess.check_call(["gcc", "-I" + os.path.join(ROOT, "include"), "-o", exe, c])
+        out = subprocess.check_output([exe], text=True).split("\n")
+    got = dict(line.rsplit(" ", 1) for line in out if line)
+    assert int(got["ehip_params"]) == C.sizeof(_lib.Params)
+    assert int(got["ehip_report"]) == C.sizeof(_lib.Report)
+    for key, val in got.items():
+        if "." in key:
+            struct, field = key.split(".")
+            cls = _lib.Params if struct == "ehip_params" else _lib.Report
+            assert getattr(cls, field).offset == int(val), key
+
+
+def test_fortran_interface_names():
+    text = open(FORTRAN).read().lower()
+    bound = set(re.findall(r'bind\(c,\s*name="(ehip_[a-z0-9_]+)"\)', text))
+    assert bound, "no bind(C) interfaces found"
+    assert bound <= set(declared_functions())
+    for must in ("ehip_matvec", "ehip_solve_csr", "ehip_solve", "ehip_system_create", "ehip_assemble"):
+        assert must in bound
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/lib/llvm/bin/flang"), reason="flang absent")
+def test_fortran_struct_layout():
+    """The bind(C) derived types have the C sizes (flang lays them out)."""
+    prog = """
+program l
+  use ehip
+  type(ehip_params) :: p
+  type(ehip_report) :: r
+  print '(i0,1x,i0)', c_sizeof(p), c_sizeof(r)
+end program l
+"""
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "l.F90")
+        exe = os.path.join(d, "l")
+        open(f, "w").write(prog)
+        libdir = os.path.dirname(_lib.LIB_PATH)
+        subprocess.check_call(["/opt/rocm/lib/llvm/bin/flang", "-J", d, "-o", exe, FORTRAN, f, "-L" + libdir,
+                               "-lehip", "-Wl,-rpath," + libdir], cwd=d)
+        a, b = map(int, subprocess.check_output([exe], text=True).split())
+    assert a == C.sizeof(_lib.Params) and b == C.sizeof(_lib.Report)

@@ -275,3 +275,16 @@ def test_solve_deterministic(ehip):
         s.set_values(vals, rhs)
         xb, rb = s.solve(w["kw"], precision=precision, mode=mode)
         assert np.array_equal(xa, xb) and ra["iters"] == rb["iters"]
+
+
+def test_fortran_host_program(ehip):
+    """Fortran host code through the ISO_C_BINDING module: the Matrix Vector
+    Proc entry bit-exact against the CRS_MatrixVectorProd loop run in
+    Fortran, and a whole solve to 1e-10 with the residual checked there."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(_lib.LIB_PATH), "fortran", "ehip_demo")
+    assert os.path.exists(exe), "Fortran example not built (python -m elmerfem_amd.build)"
+    r = subprocess.run([exe, "48"], capture_output=True, text=True, timeout=60)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "matvec bit-exact: T" in r.stdout and "ehip_demo: ok" in r.stdout
