@@ -32,15 +32,23 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 
-def spmv_bytes(nnz: int, n: int, fp32: bool = False) -> int:
-    """Algorithmic bytes of one CSR SpMV (int32 indices, x read once, y written once)."""
-    return (8 * nnz + 4 * (n + 1) + 8 * n) if fp32 else (12 * nnz + 4 * (n + 1) + 16 * n)
+def index_bytes(nnz: int, bits: int) -> int:
+    """Column index bytes one SpMV streams: int32 Cols, or 16-bit offsets plus
+    one int32 base per segment of 2048 nonzeros."""
+    return 4 * nnz if bits == 32 else 2 * nnz + 4 * ((nnz + 2047) // 2048 + 2)
 
 
-def fused_cg32_bytes(nnz: int, n: int) -> int:
+def spmv_bytes(nnz: int, n: int, fp32: bool = False, bits: int = 32) -> int:
+    """Bytes of one CSR SpMV: values, column indices, row pointers, x read
+    once, y written once."""
+    vb = 4 if fp32 else 8
+    return vb * nnz + index_bytes(nnz, bits) + 4 * (n + 1) + 2 * vb * n
+
+
+def fused_cg32_bytes(nnz: int, n: int, bits: int = 32) -> int:
     """Fused FP32 CG step 1 (q = A z + beta q, p = z + beta p, p.q):
-    vals32 + cols + rows + z + q (r/w) + p (r/w)."""
-    return 8 * nnz + 4 * (n + 1) + 4 * n + 16 * n
+    vals32 + column indices + rows + z + q (r/w) + p (r/w)."""
+    return 4 * nnz + index_bytes(nnz, bits) + 4 * (n + 1) + 4 * n + 16 * n
 
 
 def log(*a):
@@ -71,8 +79,172 @@ def cpu_baseline(sys_, kw, iters: int):
             "seconds": dt}
 
 
+def form_batch(args):
+    """Batching form: independent steady 2-D convection-diffusion problems on
+    structured Quad4 grids, side N ~ U{32..512} elements, velocity of random
+    direction with Peclet ~ U[1, 100], BiCGStab + Jacobi to 1e-10, all in one
+    graph-replayed multi-system launch sequence -- FP64 (HUTI order), then
+    mixed precision -- beside the same systems solved one at a time in FP64."""
+    from elmerfem_amd import _lib, workloads as W
+    from elmerfem_amd.batch import solve_batch
+    from elmerfem_amd.mesh import uniforms
+    _lib.init(0)
+    k = args.systems
+    u, _ = uniforms(args.seed, np.arange(1, k + 1, dtype=np.uint64), stream=11)
+    sizes = (32 + np.minimum((u * 481).astype(int), 480)).tolist()
+    t0 = time.perf_counter()
+    systems = []
+    for i, nn in enumerate(sizes):
+        w = W.convdiff_quad4(nn, seed=args.seed, index=i + 1)
+        systems.append(W.build_system(w, optimize_bandwidth=True)[0])
+        if (i + 1) % 64 == 0:
+            log("built %d/%d systems (%.1fs)" % (i + 1, k, time.perf_counter() - t0))
+    dofs = np.array([s.n for s in systems], dtype=np.float64)
+    nnz = int(sum(s.nnz for s in systems))
+    kw = W._kw("bicgstab", "diagonal", 1e-10, 20000)
+    out = {}
+    for name, prec, mode in (("fp64_batched", _lib.FP64, _lib.FAITHFUL), ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
+        solve_batch(systems[:2], kw, precision=prec, mode=mode)  # warm-up
+        t = time.perf_counter()
+        xs, reps, ms = solve_batch(systems, kw, precision=prec, mode=mode)
+        wall = time.perf_counter() - t
+        it = np.array([r["iters"] for r in reps], dtype=np.float64)
+        conv = sum(1 for r in reps if r["info"] == 1)
+        out[name] = {"wall_s": wall, "device_ms": ms, "dof_iters_per_s": float((dofs * it).sum() / wall),
+                     "iters_mean": float(it.mean()), "iters_max": int(it.max()), "converged": conv,
+                     "res_true_max": float(max(r["res_true"] for r in reps))}
+        log("%-14s %.2f s, %d/%d converged, %.3e DOF-iter/s" % (name, wall, conv, k, out[name]["dof_iters_per_s"]))
+    t = time.perf_counter()
+    it_seq = []
+    for s in systems:
+        _, r = s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False)
+        it_seq.append(r["iters"])
+    wall = time.perf_counter() - t
+    out["fp64_one_at_a_time"] = {"wall_s": wall, "dof_iters_per_s": float((dofs * np.array(it_seq)).sum() / wall)}
+    log("fp64 one at a time %.2f s" % wall)
+    head = out["mixed_batched"]
+    rec = {"metric": "DOF-iters/s, batched BiCGStab to 1e-10 (mixed precision)", "value": head["dof_iters_per_s"],
+           "unit": "DOF-iter/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1000.0 * head["wall_s"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement",
+           "data": "synthetic",
+           "config": {"workload": "convdiff_quad4_batch", "systems": k, "dofs_total": int(dofs.sum()),
+                      "nnz_total": nnz, "side_min": int(min(sizes)), "side_max": int(max(sizes)),
+                      "seed": args.seed, "method": "bicgstab", "precond": "diagonal", "tol": 1e-10},
+           "forms": out,
+           "speedup_vs_one_at_a_time": {n: out["fp64_one_at_a_time"]["wall_s"] / out[n]["wall_s"]
+                                        for n in ("fp64_batched", "mixed_batched")}}
+    print(json.dumps(rec), flush=True)
+    for s in systems:
+        s.free()
+    return 0
+
+
+def form_serve(args):
+    """Serving form: Poisson arrivals of steady 3-D heat solves on jittered
+    Tet4 cubes (n^3 hexes x 6, n log-uniform in [10, 100]), lognormal(0, 0.5)
+    conductivity, mixed-precision refined CG to 1e-10 on the continuous-
+    batching server with HBM-footprint admission; latency is arrival to
+    report on the host.  Requests draw their (assembled, device-resident)
+    system from a pool built before the loop; one-at-a-time plain FP64 CG
+    over the same pool is the reference rate."""
+    from elmerfem_amd import _lib, workloads as W
+    from elmerfem_amd.batch import Server
+    from elmerfem_amd.mesh import uniforms
+    _lib.init(0)
+    P = args.pool
+    u, _ = uniforms(args.seed, np.arange(1, P + 1, dtype=np.uint64), stream=13)
+    sizes = np.clip(np.floor(10.0 ** (1.0 + u)).astype(int), 10, 100).tolist()
+    t0 = time.perf_counter()
+    pool = []
+    for i, nn in enumerate(sizes):
+        pool.append(W.build_system(W.heat_tet4(nn, seed=args.seed, index=i + 1), optimize_bandwidth=True)[0])
+        if (i + 1) % 16 == 0:
+            log("built %d/%d pool systems (%.1fs)" % (i + 1, P, time.perf_counter() - t0))
+    kw = W._kw("cg", "diagonal", 1e-10, 20000)
+    # one at a time, plain FP64 CG
+    for s in pool[:2]:
+        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False)
+    t = time.perf_counter()
+    seq_lat = []
+    for s in pool:
+        t1 = time.perf_counter()
+        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False)
+        seq_lat.append(time.perf_counter() - t1)
+    seq_rate = P / (time.perf_counter() - t)
+    log("one-at-a-time FP64: %.1f solves/s" % seq_rate)
+    rate = args.rate if args.rate > 0 else args.load * seq_rate
+    R = args.requests
+    rng = np.random.default_rng(args.seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)))
+    busy = {}
+    queue = []
+    lat = np.zeros(R)
+    infos = np.zeros(R, dtype=int)
+    res_true = np.zeros(R)
+    nxt = 0
+    done = 0
+    max_live = 0
+    t0 = time.perf_counter()
+    while done < R:
+        now = time.perf_counter() - t0
+        while nxt < R and arrivals[nxt] <= now:
+            queue.append(nxt)
+            nxt += 1
+        while queue:
+            req = queue[0]
+            s = pool[req % P]
+            if s._h.value in busy or not srv.submit(s):
+                break
+            busy[s._h.value] = req
+            queue.pop(0)
+        live, _ = srv.status()
+        max_live = max(max_live, live)
+        if live == 0:
+            if nxt < R:
+                time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.01)))
+            continue
+        for s, rep in srv.step():
+            req = busy.pop(s._h.value)
+            lat[req] = time.perf_counter() - t0 - arrivals[req]
+            infos[req] = rep["info"]
+            res_true[req] = rep["res_true"]
+            done += 1
+    elapsed = time.perf_counter() - t0
+    srv.free()
+    served_rate = R / elapsed
+    rec = {"metric": "solves/s and p50/p99 latency, continuous-batching mixed-precision CG to 1e-10",
+           "value": served_rate, "unit": "solves/s", "n_gpus": 1, "steps": R, "warmup": 0,
+           "ms_per_step": 1000.0 * elapsed / R, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 inner / f64 refinement", "data": "synthetic",
+           "config": {"workload": "heat_tet4_serving", "requests": R, "arrival_rate_per_s": rate,
+                      "pool_systems": P, "n_min": int(min(sizes)), "n_max": int(max(sizes)),
+                      "dofs_mean": float(np.mean([s.n for s in pool])), "slots": args.slots,
+                      "mem_budget_gib": args.budget_gib, "seed": args.seed, "tol": 1e-10},
+           "latency_ms": {"p50": 1000.0 * float(np.percentile(lat, 50)), "p99": 1000.0 * float(np.percentile(lat, 99)),
+                          "max": 1000.0 * float(lat.max())},
+           "one_at_a_time_fp64": {"solves_per_s": seq_rate,
+                                  "latency_ms_p50": 1000.0 * float(np.percentile(seq_lat, 50)),
+                                  "latency_ms_p99": 1000.0 * float(np.percentile(seq_lat, 99))},
+           "speedup_solves_per_s": served_rate / seq_rate, "converged": int((infos == 1).sum()),
+           "res_true_max": float(res_true.max()), "max_live": max_live}
+    print(json.dumps(rec), flush=True)
+    for s in pool:
+        s.free()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--form", choices=["headline", "batch", "serve"], default="headline",
+                    help="headline: the 128^3 solve; batch / serve: the batching and serving forms")
+    ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
+    ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request systems")
+    ap.add_argument("--requests", type=int, default=1000, help="serve form: requests")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve form: arrivals per second (0: load x FP64 rate)")
+    ap.add_argument("--load", type=float, default=8.0, help="serve form: arrival rate / one-at-a-time FP64 rate")
+    ap.add_argument("--slots", type=int, default=64, help="serve form: server slots")
+    ap.add_argument("--budget-gib", type=float, default=64.0, help="serve form: HBM admission budget")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -88,6 +260,10 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.form == "batch":
+        return form_batch(args)
+    if args.form == "serve":
+        return form_serve(args)
     from elmerfem_amd import _lib, workloads as W
     from elmerfem_amd.system import copy_bench
 
@@ -158,17 +334,22 @@ def main():
 
     # roofline of the dominant kernel: the fused FP32 SpMV of the inner CG,
     # its launches sampled with HIP events on the system's stream in the timed solves
+    bits = sys_.spmv_config(1)["index_bits"]
     us = float(np.mean([r["spmv_us"] for r in steps if r["spmv_samples"] > 0]))
-    bytes32 = fused_cg32_bytes(nnz, n)
+    bytes32 = fused_cg32_bytes(nnz, n, bits)
     achieved = bytes32 / (us * 1e-6) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_spmv_epi<float,256,2048,EpiCgFast1<float>> (fused FP32 SpMV + CG update)",
+                "kernel": "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>> (fused FP32 SpMV + CG update)" % (
+                    sys_.spmv_config(1)["block"], sys_.spmv_config(1)["nnz_per_block"],
+                    "ColsU16" if bits == 16 else "ColsI32"),
                 "bytes_per_launch": bytes32, "mean_us": us,
                 "samples": int(sum(r["spmv_samples"] for r in steps))}
-    spmv = {"fp64_us": spmv64_us, "fp64_gbs": spmv_bytes(nnz, n) / (spmv64_us * 1e-6) / 1e9,
-            "fp32_us": spmv32_us, "fp32_gbs": spmv_bytes(nnz, n, True) / (spmv32_us * 1e-6) / 1e9,
-            "copy_gbs_measured": copy_gbs, "config64": sys_.spmv_config()}
+    spmv = {"fp64_us": spmv64_us, "fp64_gbs": spmv_bytes(nnz, n, False, bits) / (spmv64_us * 1e-6) / 1e9,
+            "fp32_us": spmv32_us, "fp32_gbs": spmv_bytes(nnz, n, True, bits) / (spmv32_us * 1e-6) / 1e9,
+            "fp64_gbs_int32_equiv": spmv_bytes(nnz, n) / (spmv64_us * 1e-6) / 1e9,
+            "index_bits": bits, "copy_gbs_measured": copy_gbs,
+            "config64": sys_.spmv_config(0), "config32": sys_.spmv_config(1)}
     spmv["fp64_frac_peak"] = spmv["fp64_gbs"] / HBM_PEAK_GBS
     assembly = {"ms": t_asm_ms, "elements_per_s": m.nelem / (t_asm_ms / 1000.0),
                 "gbs_algorithmic": asm_bytes / (t_asm_ms * 1e-3) / 1e9, "host_setup_s": t_setup}

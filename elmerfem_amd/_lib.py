@@ -49,6 +49,9 @@ class Report(C.Structure):
 _vp = C.c_void_p
 _ip = C.POINTER(C.c_int)
 _i64 = C.c_int64
+# the SpMV schedules compiled into the library (EHIP_SPMV_CONFIGS)
+SPMV_SCHEDULES = [(256, 2048), (256, 1024), (512, 2048)]
+
 SIGNATURES = {
     "ehip_init": (C.c_int, [C.c_int]),
     "ehip_last_error": (C.c_char_p, []),
@@ -73,7 +76,8 @@ SIGNATURES = {
     "ehip_solve_csr": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params), C.POINTER(Report)]),
     "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
-    "ehip_spmv_config": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "ehip_spmv_set_schedule": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ehip_copy_bench": (C.c_int, [_i64, C.c_int, _vp, _vp]),
     "ehip_spmv_tune": (C.c_int, [_vp, C.c_int]),
     "ehip_batch_create": (C.c_int, [C.c_int, _vp, _vp]),

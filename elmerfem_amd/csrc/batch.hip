@@ -66,7 +66,7 @@ struct SlotPtrs {
 };
 
 struct BSys {
-  double rho, rho_new, alpha, omega, rhsnorm, res, scale, tol, snorm, pad64;
+  double rho, rho_new, alpha, omega, rhsnorm, res, scale, tol, snorm, scale_prev;
   float rho32, rho_new32, alpha32, omega32, beta32, rhsnorm32, res32, itol;
   int iter, info, done, phase, maxit, outer, inner_iter, inner_total, inner_maxit, snorm_exit;
   unsigned ticket;
@@ -475,6 +475,7 @@ struct CtxM {
   Vecs<float> v;   // inner FP32 vectors; v.x = d (the correction), v.b = normalised residual
   double *t64, *x64, *b64, *dg64;
   double scale;
+  double ratio;  // previous residual scale / current one (0 before the first replacement)
   float alpha, beta, omega;
 };
 
@@ -488,6 +489,7 @@ __device__ __forceinline__ CtxM ctxm(const BSys &st, const SlotPtrs &sp) {
   c.x64 = sp.x;
   c.b64 = sp.b;
   c.scale = st.scale;
+  c.ratio = st.scale_prev > 0.0 ? st.scale_prev / st.scale : 0.0;
   c.alpha = st.alpha32;
   c.beta = st.beta32;
   c.omega = st.omega32;
@@ -519,13 +521,18 @@ struct OpRes : OpBase<double> {
     else if (!(res == res)) slot_done(a, st, 3);
     else if (st.inner_total >= st.maxit) slot_done(a, st, 2);
     else {
+      st.scale_prev = st.outer > 1 ? st.scale : 0.0;
       st.scale = nt;
       st.phase = PH_PREP;
     }
   }
 };
 
-// r32 = t/||t|| ; d = 0 ; CG: z = r/dg, p = q = 0 ; BiCGStab: rtld = p = r, v = 0, t1v = r/dg
+// r32 = t/||t|| ; d = 0 ; BiCGStab: rtld = p = r, v = 0, t1v = r/dg (restart).
+// CG (reliable update): z = r/dg, and the search direction survives the
+// residual replacement -- p and q = A p are carried over, rescaled from the
+// previous residual's scale to this one, and beta continues the sequence
+// (beta = r.z / (r.z before the last step, rescaled)).  First time: p = q = 0.
 struct OpPrep : OpBase<float> {
   static constexpr bool SPMV = false;
   static constexpr int NV = 2;
@@ -540,8 +547,9 @@ struct OpPrep : OpBase<float> {
     if (method == EHIP_CG) {
       const float zi = ri / c.v.dg[i];
       c.v.z[i] = zi;
-      c.v.p[i] = 0.f;
-      c.v.q[i] = 0.f;
+      const float ratio = (float)c.ratio;
+      c.v.p[i] = ratio > 0.f ? c.v.p[i] * ratio : 0.f;
+      c.v.q[i] = ratio > 0.f ? c.v.q[i] * ratio : 0.f;
       acc[0] += ri * zi;
     } else {
       c.v.rtld[i] = ri;
@@ -553,9 +561,12 @@ struct OpPrep : OpBase<float> {
     acc[1] += ri * ri;
   }
   __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, float (&tot)[2]) const {
+    const double ratio = st.scale_prev > 0.0 ? st.scale_prev / st.scale : 0.0;
+    st.beta32 = 0.f;
+    if (method == EHIP_CG && ratio > 0.0 && st.rho_new32 != 0.f)
+      st.beta32 = (float)((double)tot[0] / ((double)st.rho_new32 * ratio * ratio));
     st.rho32 = tot[0];
     st.rhsnorm32 = sqrtf(tot[1]);
-    st.beta32 = 0.f;
     st.alpha32 = 0.f;
     st.omega32 = 1.f;
     st.inner_iter = 0;
@@ -611,6 +622,7 @@ struct OpCgF2 : OpBase<float> {
   }
   __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, float (&tot)[2]) const {
     const float rho_old = st.rho32;
+    st.rho_new32 = rho_old;  // kept for the next residual replacement
     st.rho32 = tot[0];
     st.beta32 = tot[0] / rho_old;
     inner_step_done(st, sqrtf(tot[1]) / st.rhsnorm32, tot[0] == 0.f);
@@ -999,7 +1011,8 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   std::memset(&z, 0, sizeof(z));
   z.tol = b->p.tol;
   z.maxit = b->p.maxit;
-  z.itol = (float)(b->p.inner_tol > 0 ? b->p.inner_tol : 1e-4);
+  // CG: residual drop between replacements; BiCGStab: restarted inner tolerance
+  z.itol = (float)(b->p.inner_tol > 0 ? b->p.inner_tol : (b->p.method == EHIP_CG ? 1e-2 : 1e-4));
   z.inner_maxit = b->p.inner_maxit > 0 ? b->p.inner_maxit : 1000;
   z.phase = b->mixed ? PH_RES : PH_INIT;
   z.omega = 1.0;

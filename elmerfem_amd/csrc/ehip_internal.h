@@ -33,12 +33,19 @@ struct SpmvPlan {
   int *chunk_row = nullptr; // device, nchunks+1 (0-based first row of each chunk)
 };
 
-// Device-resident Krylov scalars.  Reductions finish inside the producing
-// kernel: the last workgroup to arrive (integer ticket) sums the per-block
-// partials in a fixed order, so results are run-to-run deterministic.
+// The SpMV schedules (workgroup size, nonzeros per chunk) compiled and
+// offered to the autotuner.
+#define EHIP_SPMV_CONFIGS(X) X(256, 2048) X(256, 1024) X(512, 2048)
+
+// Device-resident Krylov scalars.  Kernels publish per-workgroup partial
+// sums; a one-workgroup finalize launch adds them in a fixed order and
+// updates these scalars, so results are run-to-run deterministic.
+// ru: reliable-update threshold of the mixed-precision CG (0 = off): the
+// FP32 iteration stops for an FP64 residual replacement once its recurrence
+// residual falls below it.
 template <typename T>
 struct KStateT {
-  T rho, rho_new, alpha, beta, omega, pq, rr, rhsnorm, res, ts, tt, snorm, res_true, scale;
+  T rho, rho_new, alpha, beta, omega, pq, rr, rhsnorm, res, ts, tt, snorm, res_true, scale, ru;
   int iter, info, done, maxit, snorm_exit, rec_conv, pad0, pad1;
   T tol, divtol;
   unsigned int ticket[4];
@@ -62,6 +69,12 @@ struct ehip_system {
   int bw_before = 0, bw_after = 0, bw_accepted = 0;
   // device CRS, 0-based
   int *rows = nullptr, *cols = nullptr, *diag = nullptr;
+  // compressed columns for SpMV (see ColsU16): 16-bit offsets from a base
+  // per segment of 2048 nonzeros, when every segment's span fits
+  unsigned short *cols16 = nullptr;
+  int *seg_base = nullptr;
+  int nseg = 0;
+  bool c16 = false;
   double *vals = nullptr, *rhs = nullptr, *x = nullptr;
   float *vals32 = nullptr;
   double *dscale = nullptr;  // ScaleLinearSystem's D (n)

@@ -289,12 +289,14 @@ struct EpiInit {
   }
 };
 
-template <typename T, int BS, int NB, class Epi>
-__global__ __launch_bounds__(BS) void k_spmv_epi(const int *__restrict__ rows, const int *__restrict__ cols,
-                                                 const T *__restrict__ vals, const T *__restrict__ xin,
-                                                 const int *__restrict__ chunk_row, int nchunks, Epi epi,
-                                                 const KStateT<T> *st) {
+template <typename T, int BS, int NB, class CS, class Epi>
+__global__ __launch_bounds__(BS) void k_spmv_epi(const int *__restrict__ rows, const int *__restrict__ c32,
+                                                 const unsigned short *__restrict__ c16,
+                                                 const int *__restrict__ segb, const T *__restrict__ vals,
+                                                 const T *__restrict__ xin, const int *__restrict__ chunk_row,
+                                                 int nchunks, Epi epi, const KStateT<T> *st) {
   if (st && solve_done(st)) return;
+  const CS cols(c32, c16, segb);
   spmv_stream_body<T, T, BS, NB>(rows, cols, vals, xin, chunk_row, nchunks, epi);
 }
 
@@ -364,6 +366,7 @@ struct FinCgFast2 {
   __device__ void finish(T (&tot)[2]) {
     if (threadIdx.x != 0) return;
     const T rho_old = st->rho;
+    st->rho_new = rho_old;  // kept for a residual replacement, which recomputes beta
     st->rho = tot[0];
     st->beta = tot[0] / rho_old;
     st->rr = tot[1];
@@ -383,6 +386,9 @@ struct FinCgFast2 {
       st->done = 1;
     } else if (it >= st->maxit) {
       st->info = 2;
+      st->done = 1;
+    } else if (res < st->ru) {
+      st->rec_conv = 2;  // reliable update due
       st->done = 1;
     }
   }
@@ -817,6 +823,8 @@ __global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int ite
   st->snorm_exit = 0;
   st->rec_conv = 0;
   st->beta = (T)0;
+  st->rho_new = (T)0;
+  st->ru = (T)0;
   st->res_true = (T)-1;
   st->pad0 = 0;
   st->pad1 = 0;
@@ -902,6 +910,70 @@ struct EpiOuter {
   }
 };
 
+// Residual replacement of the reliable-update mixed CG, on the FP64 matrix:
+// t = b - A x64 ; r32 = t ; z32 = r32/dg32 ; x32 = 0 ; partials t.t, r32.z32.
+// The caller has folded x32 into x64 (k_correct) first.  The finalize sets
+// the FP64 outer residual and restarts the FP32 recurrence from r32 with the
+// search direction kept: rho = r32.z32 and beta = rho / (r.z of the step
+// before), so the next FP32 step continues the same CG sequence.
+struct EpiRU {
+  static constexpr int NV = 2;
+  const double *b;
+  Vecs<float> v;
+  KState *st;
+  KState32 *st32;
+  double *partials;
+  int stride;
+  unsigned *ticket;
+  double *hist;
+  double delta;  // next replacement when the FP32 residual has dropped by this factor
+  __device__ void init() {}
+  struct Pre { double b; float dg; };
+  __device__ Pre load(int i) const { return Pre{b[i], v.dg[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[2]) {
+    const double t = pr.b - s;
+    const float r = (float)t;
+    const float z = r / pr.dg;
+    v.r[i] = r;
+    v.z[i] = z;
+    v.x[i] = 0.f;
+    acc[0] += t * t;
+    acc[1] += (double)(r * z);
+  }
+  __device__ void finish(double (&tot)[2]) {
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(tot[0]) / st->rhsnorm;
+    st->res_true = res;
+    st->res = res;
+    const int k = st->pad0;  // replacement count
+    if (hist && k < st->maxit) hist[k] = res;
+    st->pad0 = k + 1;
+    const float rz = (float)tot[1];
+    const float rho_prev = st32->rho_new;
+    st32->rho = rz;
+    st32->beta = rho_prev != 0.f ? rz / rho_prev : 0.f;
+    st32->res = (float)res;
+    if (res < st->tol) {
+      st->info = 1;
+      st->done = 1;
+    } else if (!(res == res)) {
+      st->info = 3;
+      st->done = 1;
+    } else {
+      st32->ru = (float)(res * delta);
+      st32->rec_conv = 0;
+      st32->done = 0;
+    }
+  }
+};
+
+// rhsnorm = sqrt(rr) in both states (after sumsq into st->rr)
+__global__ void k_set_rhsnorm(KState *st, KState32 *st32) {
+  const double bn = sqrt(st->rr);
+  st->rhsnorm = bn;
+  st32->rhsnorm = (float)bn;
+}
+
 // sum of squares of b and of b - A x (unscaled report residual)
 struct EpiResid {
   static constexpr int NV = 2;
@@ -937,21 +1009,26 @@ __global__ __launch_bounds__(kVecBlock) void k_sumsq(int n, const double *x, dou
 
 // ================================================================ host drivers
 
+template <typename T, class CS, class Epi>
+static void launch_epi_cs(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
+                          const KStateT<T> *st) {
+#define EHIP_EPI_CASE(BS, NB)                                                                              \
+  if (p.block == BS && p.nb == NB) {                                                                       \
+    hipLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows,    \
+                       s->cols, s->cols16, s->seg_base, vals, xin, p.chunk_row, p.nchunks, e, st);         \
+    return;                                                                                                \
+  }
+  EHIP_SPMV_CONFIGS(EHIP_EPI_CASE)
+#undef EHIP_EPI_CASE
+}
+
 template <typename T, class Epi>
 static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
                        const KStateT<T> *st) {
-#define EHIP_EPI_CASE(BS, NB)                                                                              \
-  if (p.block == BS && p.nb == NB) {                                                                       \
-    hipLaunchKernelGGL((k_spmv_epi<T, BS, NB, Epi>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows, s->cols, \
-                       vals, xin, p.chunk_row, p.nchunks, e, st);                                          \
-    return;                                                                                                \
-  }
-  EHIP_EPI_CASE(256, 2048)
-  EHIP_EPI_CASE(256, 1024)
-  EHIP_EPI_CASE(128, 1024)
-  EHIP_EPI_CASE(512, 4096)
-  EHIP_EPI_CASE(256, 4096)
-#undef EHIP_EPI_CASE
+  if (s->c16)
+    launch_epi_cs<T, ColsU16>(s, p, vals, xin, e, st);
+  else
+    launch_epi_cs<T, ColsI32>(s, p, vals, xin, e, st);
 }
 
 // SpMV with epilogue, then its one-workgroup finalize
@@ -1184,6 +1261,99 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   return EHIP_OK;
 }
 
+// Default residual drop between two replacements of the reliable-update CG.
+constexpr double kRuDelta = 1e-2;
+
+// Mixed-precision CG with reliable updates: one FP32 CG sequence (FP32
+// values, vectors and accumulation, fused one-SpMV steps) whose residual is
+// replaced by the FP64 true residual b - A x64 each time the FP32 recurrence
+// residual has dropped by `delta`; the FP32 solution increment is folded
+// into x64 at each replacement.  Unlike restarted refinement the search
+// direction survives a replacement, so the iteration count stays that of
+// FP64 CG.  Convergence is decided only on the FP64 true residual.
+static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_total, int *updates) {
+  const int n = s->n;
+  int rc;
+  const int nv = nvecs(EHIP_CG);
+  if ((rc = ensure_work32(s, nv + 1))) return rc;
+  Vecs<float> v = make_vecs<float>(s->work32, n, EHIP_CG, s->work32 + (size_t)nv * n, nullptr);
+  const int g = vec_grid(n);
+  const int pg = max_grid();
+  KState *st = s->st;
+  KState32 *st32 = s->st32;
+  hipLaunchKernelGGL(k_make_dg<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, v.dg,
+                     c.precond);
+  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0);
+  hipLaunchKernelGGL(k_init_state<float>, dim3(1), dim3(1), 0, s->stream, st32, (float)c.tol, c.maxit,
+                     (float)c.divtol, 0);
+  sumsq(s, n, s->rhs, &st->rr);
+  hipLaunchKernelGGL(k_set_rhsnorm, dim3(1), dim3(1), 0, s->stream, st, st32);
+  hipLaunchKernelGGL(k_zero2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v.p, v.q);
+  EpiRU eru{s->rhs, v, st, st32, s->partials, pg, &st->ticket[0], c.hist, delta};
+  launch_epi_fin<double>(s, s->plan64, s->vals, s->x, eru, (const KState *)nullptr);
+  c.spmv++;
+  int nupd = 0;
+  if ((rc = read_state(s, st, s->h_st))) return rc;
+  // iterations per check: predicted from the observed convergence rate so
+  // that few no-op launches follow the step that asks for a replacement
+  int it_prev = 0;
+  double lres_prev = std::log(std::max(s->h_st->res, 1e-300));
+  double rate = 0.0;  // log residual change per iteration (< 0)
+  int batch = std::min(c.check, 8);
+  while (!s->h_st->done) {
+    for (int k = 0; k < batch; ++k) {
+      EpiCgFast1<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], 0.f};
+      if (c.prof) prof_begin(s);
+      launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
+      if (c.prof) prof_end(s);
+      fin<float>(s, e1, s->plan32.grid, st32);
+      hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32, s->partials32,
+                         (double *)nullptr);
+      fin<float>(s, FinCgFast2<float>{st32, s->partials32, g, nullptr}, g, st32);
+      c.spmv++;
+    }
+    if ((rc = read_state(s, st32, s->h_st32))) return rc;
+    const KState32 h = *s->h_st32;
+    if (h.iter > it_prev && h.res > 0.f) {
+      const double lr = std::log((double)h.res);
+      const double rnew = (lr - lres_prev) / (h.iter - it_prev);
+      rate = rate < 0.0 ? 0.5 * (rate + rnew) : rnew;
+      it_prev = h.iter;
+      lres_prev = lr;
+    }
+    if (!h.done) {
+      // predicted iterations until the replacement threshold
+      int want = c.check;
+      if (rate < 0.0 && h.ru > 0.f && h.res > h.ru)
+        want = (int)std::ceil(std::log((double)h.ru / (double)h.res) / rate) + 1;
+      batch = std::max(2, std::min(c.check, want));
+      continue;
+    }
+    // FP32 stopped: replacement due, recurrence converged, or failure
+    hipLaunchKernelGGL(k_correct, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->x, v.x, 1.0);
+    launch_epi_fin<double>(s, s->plan64, s->vals, s->x, eru, (const KState *)nullptr);
+    c.spmv++;
+    ++nupd;
+    if ((rc = read_state(s, st, s->h_st))) return rc;
+    if (s->h_st->done) break;
+    if (h.info != 0) {
+      // max iterations, divergence or rho = 0 in the FP32 sequence
+      s->h_st->info = h.info;
+      s->h_st->done = 1;
+      break;
+    }
+    it_prev = h.iter;
+    lres_prev = std::log(std::max(s->h_st->res, 1e-300));
+    batch = std::max(2, std::min(c.check, rate < 0.0 ? (int)std::ceil(std::log(delta) / rate) + 1 : c.check));
+  }
+  if ((rc = read_state(s, st32, s->h_st32))) return rc;
+  *inner_total = s->h_st32->iter;
+  *updates = nupd;
+  s->h_st->iter = *inner_total;
+  s->h_st->pad1 = nupd;
+  return EHIP_OK;
+}
+
 }  // namespace ehip
 
 // ================================================================ C ABI
@@ -1278,7 +1448,13 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   s->prof_count = 0;
   EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
   int inner_total = 0;
-  if (mixed) {
+  if (mixed && p->method == EHIP_CG) {
+    // reliable-update CG: inner_tol is the residual drop between replacements
+    const double delta = p->inner_tol > 0 ? p->inner_tol : kRuDelta;
+    int nupd = 0;
+    if ((rc = mixed_cg_ru(s, c, delta, &inner_total, &nupd))) return rc;
+    rep.outer_iters = nupd;
+  } else if (mixed) {
     const double itol = p->inner_tol > 0 ? p->inner_tol : 1e-4;
     const int imax = p->inner_maxit > 0 ? p->inner_maxit : 1000;
     // FP32 vectors are the workspace; the FP64 workspace holds the residual

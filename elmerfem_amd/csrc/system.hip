@@ -85,6 +85,7 @@ int plan_spmv(ehip_system *s, SpmvPlan &plan, int block, int nb) {
 }
 
 // ---------------------------------------------------------------- plain SpMV
+// (the schedule list EHIP_SPMV_CONFIGS is in ehip_internal.h)
 
 template <typename A>
 struct EpiStore {
@@ -100,30 +101,39 @@ struct EpiStore {
   __device__ void finish(A (&)[1]) {}
 };
 
-template <typename V, typename A, int BS, int NB>
-__global__ __launch_bounds__(BS) void k_spmv(const int *__restrict__ rows, const int *__restrict__ cols,
-                                             const V *__restrict__ vals, const A *__restrict__ x,
-                                             const int *__restrict__ chunk_row, int nchunks, A *y) {
+template <typename V, typename A, int BS, int NB, class CS>
+__global__ __launch_bounds__(BS) void k_spmv(const int *__restrict__ rows, const int *__restrict__ c32,
+                                             const unsigned short *__restrict__ c16,
+                                             const int *__restrict__ segb, const V *__restrict__ vals,
+                                             const A *__restrict__ x, const int *__restrict__ chunk_row,
+                                             int nchunks, A *y) {
   EpiStore<A> epi{y};
+  const CS cols(c32, c16, segb);
   spmv_stream_body<V, A, BS, NB>(rows, cols, vals, x, chunk_row, nchunks, epi);
+}
+
+template <typename V, typename A, class CS>
+static int launch_spmv_cs(ehip_system *s, const SpmvPlan &p, const V *vals, const A *x, A *y) {
+#define EHIP_SPMV_CASE(BS, NB)                                                                     \
+  if (p.block == BS && p.nb == NB) {                                                               \
+    hipLaunchKernelGGL((k_spmv<V, A, BS, NB, CS>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows,  \
+                       s->cols, s->cols16, s->seg_base, vals, x, p.chunk_row, p.nchunks, y);       \
+    return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;                                  \
+  }
+  EHIP_SPMV_CONFIGS(EHIP_SPMV_CASE)
+#undef EHIP_SPMV_CASE
+  set_error("no SpMV kernel for block %d nb %d", p.block, p.nb);
+  return EHIP_EUNSUP;
 }
 
 template <typename V, typename A>
 static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A *x, A *y) {
-#define EHIP_SPMV_CASE(BS, NB)                                                              \
-  if (p.block == BS && p.nb == NB) {                                                        \
-    hipLaunchKernelGGL((k_spmv<V, A, BS, NB>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows, \
-                       s->cols, vals, x, p.chunk_row, p.nchunks, y);                        \
-    return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;                           \
+  if (s->nnz == 0) {
+    EHIP_HIP_TRY(hipMemsetAsync(y, 0, sizeof(A) * s->n, s->stream));
+    return EHIP_OK;
   }
-  EHIP_SPMV_CASE(256, 2048)
-  EHIP_SPMV_CASE(256, 1024)
-  EHIP_SPMV_CASE(128, 1024)
-  EHIP_SPMV_CASE(512, 4096)
-  EHIP_SPMV_CASE(256, 4096)
-#undef EHIP_SPMV_CASE
-  set_error("no SpMV kernel for block %d nb %d", p.block, p.nb);
-  return EHIP_EUNSUP;
+  if (s->c16) return launch_spmv_cs<V, A, ColsU16>(s, p, vals, x, y);
+  return launch_spmv_cs<V, A, ColsI32>(s, p, vals, x, y);
 }
 
 int spmv_f64(ehip_system *s, const double *x, double *y) {
@@ -259,6 +269,36 @@ static int system_alloc(int n, int64_t nnz, ehip_system **out) {
   return EHIP_OK;
 }
 
+// Compressed SpMV columns (ColsU16) when every segment of 2^kSegShift
+// nonzeros spans fewer than 65536 columns; otherwise SpMV reads Cols.
+// c0 holds the 0-based columns.
+static int build_cols16(ehip_system *s, const std::vector<int> &c0) {
+  const int64_t nnz = s->nnz;
+  const int64_t seg = (int64_t)1 << kSegShift;
+  const int nseg = (int)((nnz + seg - 1) / seg);
+  std::vector<int> base((size_t)nseg + 2, 0);
+  for (int k = 0; k < nseg; ++k) {
+    int lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t j = k * seg; j < std::min(nnz, (k + 1) * seg); ++j) {
+      lo = std::min(lo, c0[j]);
+      hi = std::max(hi, c0[j]);
+    }
+    if ((int64_t)hi - lo > 65535) return EHIP_OK;  // keep the int32 path
+    base[k] = lo;
+  }
+  base[nseg] = base[nseg + 1] = nseg > 0 ? base[nseg - 1] : 0;
+  std::vector<unsigned short> c16((size_t)nnz);
+  for (int64_t j = 0; j < nnz; ++j) c16[j] = (unsigned short)(c0[j] - base[j >> kSegShift]);
+  int rc;
+  if ((rc = dmalloc(s, (void **)&s->cols16, sizeof(unsigned short) * std::max<int64_t>(nnz, 1)))) return rc;
+  if ((rc = dmalloc(s, (void **)&s->seg_base, sizeof(int) * base.size()))) return rc;
+  EHIP_HIP_TRY(hipMemcpy(s->cols16, c16.data(), sizeof(unsigned short) * nnz, hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->seg_base, base.data(), sizeof(int) * base.size(), hipMemcpyHostToDevice));
+  s->nseg = nseg;
+  s->c16 = true;
+  return EHIP_OK;
+}
+
 // Uploads the host pattern (1-based) as 0-based device arrays and plans SpMV.
 int ehip_internal_upload_pattern(ehip_system *s) {
   const int n = s->n;
@@ -276,13 +316,15 @@ int ehip_internal_upload_pattern(ehip_system *s) {
   EHIP_HIP_TRY(hipMemcpy(s->rows, tmp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
   for (int64_t j = 0; j < nnz; ++j) tmp[j] = s->h_cols[j] - 1;
   EHIP_HIP_TRY(hipMemcpy(s->cols, tmp.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-  for (int i = 0; i < n; ++i) tmp[i] = s->h_diag[i] - 1;
-  EHIP_HIP_TRY(hipMemcpy(s->diag, tmp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  std::vector<int> d0(n);
+  for (int i = 0; i < n; ++i) d0[i] = s->h_diag[i] - 1;
+  EHIP_HIP_TRY(hipMemcpy(s->diag, d0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemsetAsync(s->vals, 0, sizeof(double) * nnz, s->stream));
   EHIP_HIP_TRY(hipMemsetAsync(s->rhs, 0, sizeof(double) * n, s->stream));
   EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
   if ((rc = plan_spmv(s, s->plan64, 256, 2048))) return rc;
   if ((rc = plan_spmv(s, s->plan32, 256, 2048))) return rc;
+  if ((rc = build_cols16(s, tmp))) return rc;
   return ensure_state(s);
 }
 
@@ -322,11 +364,12 @@ int ehip_system_create(int n, int64_t nnz, const int *rows, const int *cols, con
 int ehip_system_free(ehip_system *s) {
   if (!s) return EHIP_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void *ptrs[] = {s->rows,      s->cols,       s->diag,       s->vals,        s->rhs,
-                  s->x,         s->vals32,     s->dscale,     s->work,        s->work32,
-                  s->partials,  s->st,         s->elements,   s->st32,        s->partials32,  s->hist,   s->slot_start,  s->slot_contrib,
-                  s->rhs_start, s->rhs_contrib, s->emat,      s->evec,        s->plan64.chunk_row,
-                  s->plan32.chunk_row};
+  void *ptrs[] = {s->rows,       s->cols,        s->diag,      s->vals,       s->rhs,
+                  s->x,          s->vals32,      s->dscale,    s->work,       s->work32,
+                  s->partials,   s->st,          s->elements,  s->st32,       s->partials32,
+                  s->hist,       s->slot_start,  s->slot_contrib, s->rhs_start, s->rhs_contrib,
+                  s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
+                  s->cols16,     s->seg_base};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
@@ -477,12 +520,28 @@ int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call) 
   return EHIP_OK;
 }
 
-int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks) {
+int ehip_spmv_config(ehip_system *s, int precision, int *block, int *nnz_per_block, int *chunks,
+                     int *index_bits) {
   if (!s) return EHIP_EARG;
-  if (block) *block = s->plan64.block;
-  if (nnz_per_block) *nnz_per_block = s->plan64.nb;
-  if (chunks) *chunks = s->plan64.nchunks;
+  const SpmvPlan &p = precision == 0 ? s->plan64 : s->plan32;
+  if (block) *block = p.block;
+  if (nnz_per_block) *nnz_per_block = p.nb;
+  if (chunks) *chunks = p.nchunks;
+  if (index_bits) *index_bits = s->c16 ? 16 : 32;
   return EHIP_OK;
+}
+
+int ehip_spmv_set_schedule(ehip_system *s, int precision, int block, int nnz_per_block) {
+  if (!s) return EHIP_EARG;
+  bool known = false;
+#define EHIP_CFG_KNOWN(BS, NB) known = known || (block == BS && nnz_per_block == NB);
+  EHIP_SPMV_CONFIGS(EHIP_CFG_KNOWN)
+#undef EHIP_CFG_KNOWN
+  if (!known) {
+    set_error("ehip_spmv_set_schedule: no SpMV kernel for block %d, %d nonzeros per chunk", block, nnz_per_block);
+    return EHIP_EUNSUP;
+  }
+  return plan_spmv(s, precision == 0 ? s->plan64 : s->plan32, block, nnz_per_block);
 }
 
 // Autotuner: times each (block, nnz-per-chunk) schedule on this matrix and
@@ -490,11 +549,14 @@ int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks
 // every schedule sums each row in the same order.
 int ehip_spmv_tune(ehip_system *s, int precision) {
   if (!s) return EHIP_EARG;
-  static const int cfg[][2] = {{256, 2048}, {256, 1024}, {128, 1024}, {512, 4096}, {256, 4096}};
+#define EHIP_CFG_ENTRY(BS, NB) {BS, NB},
+  static const int cfg[][2] = {EHIP_SPMV_CONFIGS(EHIP_CFG_ENTRY)};
+#undef EHIP_CFG_ENTRY
+  const int ncfg = (int)(sizeof(cfg) / sizeof(cfg[0]));
   SpmvPlan &plan = precision == 0 ? s->plan64 : s->plan32;
   double best = 1e30;
   int bi = 0, rc;
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < ncfg; ++i) {
     if ((rc = plan_spmv(s, plan, cfg[i][0], cfg[i][1]))) return rc;
     double us = 0;
     int reps = s->nnz > 10000000 ? 20 : 200;

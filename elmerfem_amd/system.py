@@ -180,12 +180,17 @@ class System:
 
     def spmv_tune(self, precision=0):
         _lib.check(_lib.load().ehip_spmv_tune(self._h, precision), "ehip_spmv_tune")
-        return self.spmv_config()
+        return self.spmv_config(precision)
 
-    def spmv_config(self):
-        b, nb, ch = C.c_int(), C.c_int(), C.c_int()
-        _lib.check(_lib.load().ehip_spmv_config(self._h, C.byref(b), C.byref(nb), C.byref(ch)), "ehip_spmv_config")
-        return {"block": b.value, "nnz_per_block": nb.value, "chunks": ch.value}
+    def spmv_set_schedule(self, block, nnz_per_block, precision=0):
+        _lib.check(_lib.load().ehip_spmv_set_schedule(self._h, precision, block, nnz_per_block),
+                   "ehip_spmv_set_schedule")
+
+    def spmv_config(self, precision=0):
+        b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.load().ehip_spmv_config(self._h, precision, C.byref(b), C.byref(nb), C.byref(ch),
+                                                C.byref(ib)), "ehip_spmv_config")
+        return {"block": b.value, "nnz_per_block": nb.value, "chunks": ch.value, "index_bits": ib.value}
 
 
 def copy_bench(nbytes: int, reps: int = 20):

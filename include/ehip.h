@@ -66,8 +66,10 @@ typedef struct {
   int scaling;       /* "Linear System Scaling", default 1 */
   int precision;     /* EHIP_FP64 | EHIP_MIXED_IR */
   int mode;          /* EHIP_FAITHFUL | EHIP_FAST */
-  double inner_tol;  /* mixed form: relative reduction per inner solve (0 = default 1e-4) */
-  int inner_maxit;   /* mixed form: inner iteration cap per refinement step (0 = default) */
+  double inner_tol;  /* mixed form: residual reduction per refinement step.  CG: drop of the
+                        FP32 residual between two FP64 residual replacements (0 = 1e-2);
+                        BiCGStab: relative tolerance of each restarted inner solve (0 = 1e-4) */
+  int inner_maxit;   /* mixed BiCGStab: inner iteration cap per refinement step (0 = 1000) */
   int check_every;   /* host polls convergence every k iterations (0 = default) */
   double divtol;     /* "Linear System Divergence Tolerance" (0 = none) */
   int record_history;/* 1: keep the residual of every iteration */
@@ -180,9 +182,17 @@ int ehip_spmv_bench(ehip_system *s, int precision, int reps, double *us_per_call
  * times; returns the achieved GB/s (read + write) -- the measured HBM
  * ceiling, and the calibration kernel for hardware byte counters. */
 int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
-/* Chosen SpMV schedule: block size and nnz per block (autotuned). */
-int ehip_spmv_config(ehip_system *s, int *block, int *nnz_per_block, int *chunks);
+/* SpMV schedule of one precision (0 = FP64 values, 1 = FP32): workgroup
+ * size, nonzeros per chunk, number of chunks, and the column index width the
+ * kernels read (16: per-segment offsets, used when every segment of 2048
+ * nonzeros spans < 65536 columns; 32: Cols itself). */
+int ehip_spmv_config(ehip_system *s, int precision, int *block, int *nnz_per_block, int *chunks,
+                     int *index_bits);
+/* Autotuner: times every compiled schedule on this matrix, keeps the fastest. */
 int ehip_spmv_tune(ehip_system *s, int precision);
+/* Sets a schedule explicitly (EHIP_EUNSUP if it is not compiled in).  Every
+ * schedule sums each row in the same order, so results do not change. */
+int ehip_spmv_set_schedule(ehip_system *s, int precision, int block, int nnz_per_block);
 
 /* ---- batching: many independent systems in one multi-system launch ----
  * Systems keep their own device memory.  The batch builds one device work

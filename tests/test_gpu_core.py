@@ -54,6 +54,66 @@ def test_spmv_bit_exact(ehip, case):
     s.free()
 
 
+def _wide_matrix(n, far):
+    """Tridiagonal pattern plus entries (0, far) and (far, 0): with far >= 65536
+    the first segment spans too many columns for 16-bit offsets."""
+    rows, cols = [1], []
+    for i in range(n):
+        c = {i - 1, i, i + 1} & set(range(n))
+        if i == 0:
+            c.add(far)
+        if i == far:
+            c.add(0)
+        cols.extend(sorted(j + 1 for j in c))
+        rows.append(len(cols) + 1)
+    rows = np.array(rows, np.int32)
+    cols = np.array(cols, np.int32)
+    diag = np.array([rows[i] + np.nonzero(cols[rows[i] - 1:rows[i + 1] - 1] == i + 1)[0][0] for i in range(n)],
+                    np.int32)
+    return rows, cols, diag
+
+
+@pytest.mark.parametrize("far,bits", [(70000, 32), (60000, 16)])
+def test_spmv_every_schedule_and_index_width(ehip, far, bits):
+    """Every compiled schedule, with 16-bit segment offsets and with int32
+    columns, gives the reference loop's bits; the FP32 copy stays within
+    FP32 rounding of the FP64 product."""
+    n = 70001
+    rows, cols, diag = _wide_matrix(n, far)
+    rng = np.random.default_rng(5)
+    vals = rng.standard_normal(len(cols))
+    vals[diag - 1] = 4.0 + rng.random(n)
+    s = System.from_csr(rows, cols, diag)
+    s.set_values(vals, np.ones(n))
+    assert s.spmv_config(0)["index_bits"] == bits
+    u = rng.standard_normal(n)
+    ref = np.zeros(n)
+    O.lib().or_spmv(n, O._p(rows), O._p(cols), O._p(vals), O._p(u), O._p(ref))
+    for block, nb in _lib.SPMV_SCHEDULES:
+        s.spmv_set_schedule(block, nb)
+        cfg = s.spmv_config(0)
+        assert (cfg["block"], cfg["nnz_per_block"]) == (block, nb)
+        assert np.array_equal(s.spmv(u), ref), (block, nb)
+    with pytest.raises(_lib.EhipError):
+        s.spmv_set_schedule(64, 100)
+    # FP32 values appear with a mixed-precision solve (on the scaled matrix)
+    s.solve(_lib.Params(0, 1, 1e-8, 50, 0, _lib.MIXED_IR, _lib.FAST, 0, 0, 0, 0, 0, 0))
+    ref32 = dense_free_spmv(rows, cols, vals.astype(np.float32).astype(np.float64), u.astype(np.float32))
+    for block, nb in _lib.SPMV_SCHEDULES:
+        s.spmv_set_schedule(block, nb, precision=1)
+        got = s.spmv(u, precision=1)
+        assert np.abs(got - ref32).max() <= 1e-5 * np.abs(ref32).max(), (block, nb)
+    s.free()
+
+
+def dense_free_spmv(rows, cols, vals, u):
+    n = len(rows) - 1
+    out = np.zeros(n)
+    r = np.repeat(np.arange(n), np.diff(rows))
+    np.add.at(out, r, vals * u[cols - 1])
+    return out
+
+
 def test_spmv_long_rows(ehip):
     """Rows longer than a workgroup's chunk take the workgroup-reduction path."""
     n = 300
