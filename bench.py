@@ -234,10 +234,64 @@ def form_serve(args):
     return 0
 
 
+def form_elastic(args):
+    """Precision form on 3-D linear elasticity: structured Hex8 cube (or Hex27
+    with --quadratic), E = 1, nu = 0.3, x = 0 clamped, traction (0, 0, -1e-3)
+    on x = 1; CG + Jacobi to 1e-10 in FP64 and in mixed precision (FP32 inner
+    CG with reliable FP64 residual replacement).  The mixed solution's error
+    is stated against the FP64 solution of the same system."""
+    from elmerfem_amd import _lib, workloads as W
+    _lib.init(0)
+    t0 = time.perf_counter()
+    w = W.elasticity_hex(args.n, quadratic=args.quadratic)
+    sys_, t_asm = W.build_system(w, optimize_bandwidth=True)
+    log("elasticity %s %d^3: n=%d nnz=%d, built in %.1fs" % ("hex27" if args.quadratic else "hex8", args.n, sys_.n,
+                                                              sys_.nnz, time.perf_counter() - t0))
+    t_asm = float(np.median([sys_.assemble(w["physics"], w["mesh"].coords, elem_coef=w["coef"],
+                                           poisson=w["poisson"]) for _ in range(3)]))
+    m = w["mesh"]
+    sel = m.bnd_ids == w["traction_bnd"]
+    sys_.assemble_traction(m.bnd_code, m.coords, m.bnd_elements[sel], w["traction"])
+    sys_.dirichlet_nodes(w["bc_nodes"])
+    vals, rhs = sys_.values()
+    kw = w["kw"]
+    kw.tol, kw.maxit = 1e-10, 50000
+    out = {}
+    xs = {}
+    for name, prec, mode in (("fp64_fast", _lib.FP64, _lib.FAST), ("fp64_faithful", _lib.FP64, _lib.FAITHFUL),
+                             ("mixed", _lib.MIXED_IR, _lib.FAST)):
+        sys_.set_values(vals, rhs)
+        t = time.perf_counter()
+        x, r = sys_.solve(kw, precision=prec, mode=mode)
+        wall = time.perf_counter() - t
+        xs[name] = x
+        out[name] = {"iters": r["iters"], "ms": 1000.0 * wall, "info": r["info"], "res_true": r["res_true"],
+                     "dof_iters_per_s": sys_.n * r["iters"] / wall, "replacements": r["outer_iters"]}
+        log("%-14s info=%d iters=%d %.1f ms res_true=%.2e" % (name, r["info"], r["iters"], 1000 * wall,
+                                                             r["res_true"]))
+    ref = xs["fp64_fast"]
+    out["mixed"]["rel_err_vs_fp64"] = float(np.abs(xs["mixed"] - ref).max() / np.abs(ref).max())
+    head = out["mixed"]
+    rec = {"metric": "DOF-iters/s, mixed-precision refined CG to 1e-10 (elasticity)", "value": head["dof_iters_per_s"],
+           "unit": "DOF-iter/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": head["ms"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement",
+           "data": "synthetic",
+           "config": {"workload": "elasticity_%s_%d" % ("hex27" if args.quadratic else "hex8", args.n),
+                      "dofs": sys_.n, "nnz": sys_.nnz, "E": 1.0, "nu": 0.3, "tol": 1e-10},
+           "forms": out, "assembly_ms": t_asm,
+           "speedup_vs_fp64_faithful": out["fp64_faithful"]["ms"] / head["ms"],
+           "speedup_vs_fp64_fast": out["fp64_fast"]["ms"] / head["ms"]}
+    print(json.dumps(rec), flush=True)
+    sys_.free()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", choices=["headline", "batch", "serve"], default="headline",
-                    help="headline: the 128^3 solve; batch / serve: the batching and serving forms")
+    ap.add_argument("--form", choices=["headline", "batch", "serve", "elastic"], default="headline",
+                    help="headline: the 128^3 solve; batch / serve: the batching and serving forms; "
+                         "elastic: the precision form on elasticity (--n 96, or --quadratic --n 48)")
+    ap.add_argument("--quadratic", action="store_true", help="elastic form: Hex27 elements")
     ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
     ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request systems")
     ap.add_argument("--requests", type=int, default=1000, help="serve form: requests")
@@ -264,6 +318,10 @@ def main():
         return form_batch(args)
     if args.form == "serve":
         return form_serve(args)
+    if args.form == "elastic":
+        if args.n == 128:
+            args.n = 48 if args.quadratic else 96
+        return form_elastic(args)
     from elmerfem_amd import _lib, workloads as W
     from elmerfem_amd.system import copy_bench
 

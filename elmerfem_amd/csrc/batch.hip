@@ -27,6 +27,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "ehip_internal.h"
@@ -134,32 +135,40 @@ __global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
       const T *xin = op.xin(cx);
       const auto *vals = op.vals(sp);
       if (len <= NB) {
+        // branch-free stream: indices past the chunk are clamped to its last
+        // nonzero (every load unconditional and in bounds); the extra
+        // products land in LDS slots no row reads
         constexpr int ITEMS = NB / BS;
-        T vv[ITEMS], xv[ITEMS];
+        using VT = typename std::remove_cv<typename std::remove_pointer<decltype(vals)>::type>::type;
+        VT vv[ITEMS];
         int cc[ITEMS];
+        T xv[ITEMS];
+        const int rr = ck.r0 + (int)threadIdx.x;
+        const int rl = min(rr, ck.r1 - 1);
+        const int ra = sp.rows[rl], rb = sp.rows[rl + 1];
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
-          const int j = (int)threadIdx.x + k * BS;
-          if (j < len) { vv[k] = (T)ntload(vals + j0 + j); cc[k] = ntload(sp.cols + j0 + j); }
+          const int j = lane_nz<BS>(j0, len, k);
+          vv[k] = ntload(vals + j);
+          cc[k] = ntload(sp.cols + j);
         }
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-          const int j = (int)threadIdx.x + k * BS;
-          if (j < len) xv[k] = xin[cc[k]];
-        }
+        for (int k = 0; k < ITEMS; ++k) xv[k] = xin[cc[k]];
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-          const int j = (int)threadIdx.x + k * BS;
-          if (j < len) prod[j] = vv[k] * xv[k];
+        for (int k = 0; k < ITEMS; ++k) prod[threadIdx.x + k * BS] = (T)vv[k] * xv[k];
+        lds_barrier();
+        if (rr < ck.r1) {
+          T s = (T)0;
+          for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
+          op.row(cx, rr, s, acc);
         }
-        __syncthreads();
-        for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) {
+        for (int r = rr + BS; r < ck.r1; r += BS) {
           const int s0 = sp.rows[r] - j0, s1 = sp.rows[r + 1] - j0;
           T s = (T)0;
           for (int k = s0; k < s1; ++k) s = s + prod[k];
           op.row(cx, r, s, acc);
         }
-        __syncthreads();
+        lds_barrier();
       } else {
         for (int r = ck.r0; r < ck.r1; ++r) {
           T part = (T)0;

@@ -1,7 +1,7 @@
-// Deterministic wave64 / workgroup reductions and the "last workgroup
-// finishes" pattern used by every fused Krylov kernel.  No floating-point
-// atomics: workgroups publish partial sums, take an integer ticket, and the
-// workgroup holding the final ticket sums all partials in index order.
+// Deterministic wave64 / workgroup reductions for the fused Krylov kernels.
+// No floating-point atomics and no in-kernel fences: each workgroup publishes
+// its partial sums, and a later one-workgroup launch adds them in index
+// order, so every result is run-to-run identical.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -33,11 +33,6 @@ __device__ __forceinline__ T block_sum(T v, T *sh) {
   return t;
 }
 
-template <typename T>
-__device__ __forceinline__ T load_coherent(const T *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Writes the workgroup's NV partial sums to partials[k*stride + blockIdx.x]
 // with plain stores.  A later launch on the same stream (the one-workgroup
 // finalize kernel) reads them: kernel boundaries order the memory, so no
@@ -57,51 +52,40 @@ __device__ __forceinline__ void block_publish(const T (&v)[NV], T *partials, int
 }
 
 // Sums g published partials per value in a fixed order; one workgroup, every
-// thread receives the totals.
+// thread receives the totals.  The partials were written by workgroups on
+// every XCD, so each load misses this XCD's L2: a thread issues all of its
+// loads (up to kFinPer per value) before adding any, so the reduction costs
+// one memory latency instead of one per strided step.
+constexpr int kFinPer = 16;
 template <typename T, int NV>
 __device__ __forceinline__ void block_reduce_all(const T *partials, int stride, int g, T (&tot)[NV]) {
   __shared__ T sh[16 * NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    T a = (T)0;
-    for (int i = threadIdx.x; i < g; i += blockDim.x) a += partials[k * stride + i];
-    tot[k] = block_sum(a, sh + 16 * k);
-  }
-}
-
-// Publishes NV partial sums of this workgroup (partials[k*stride + blockIdx])
-// and returns true in the workgroup that arrived last; that workgroup then
-// finds the totals in tot[0..NV).  `ticket` must start at 0; it is reset by
-// the last workgroup, so the next launch can reuse it.
-template <typename T, int NV>
-__device__ __forceinline__ bool last_block_reduce(const T (&v)[NV], T *partials, int stride,
-                                                  unsigned int *ticket, T (&tot)[NV]) {
-  __shared__ T sh[16 * NV];
-  __shared__ int amlast;
-  T bs[NV];
-#pragma unroll
-  for (int k = 0; k < NV; ++k) bs[k] = block_sum(v[k], sh + 16 * k);
-  if (threadIdx.x == 0) {
+  const int nt = blockDim.x;
+  if (g <= kFinPer * nt) {
+    T v[NV][kFinPer];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&partials[k * stride + blockIdx.x], bs[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned int t = atomicAdd(ticket, 1u);
-    amlast = (t == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!amlast) return false;
-  __threadfence();
-  const int g = gridDim.x;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    T a = (T)0;
-    for (int i = threadIdx.x; i < g; i += blockDim.x) a += load_coherent(&partials[k * stride + i]);
-    tot[k] = block_sum(a, sh + 16 * k);
+      for (int u = 0; u < kFinPer; ++u) {
+        const int i = threadIdx.x + u * nt;  // load clamped (g >= 1), keep only i < g
+        const T x = partials[k * stride + min(i, g - 1)];
+        v[k][u] = i < g ? x : (T)0;
+      }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      T a = (T)0;
+#pragma unroll
+      for (int u = 0; u < kFinPer; ++u) a += v[k][u];
+      tot[k] = block_sum(a, sh + 16 * k);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      T a = (T)0;
+      for (int i = threadIdx.x; i < g; i += nt) a += partials[k * stride + i];
+      tot[k] = block_sum(a, sh + 16 * k);
+    }
   }
-  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
 
 // XCD-aware chunk walk: gridDim.x is a multiple of 8 and the hardware
