@@ -37,6 +37,12 @@ struct SpmvPlan {
 // offered to the autotuner.
 #define EHIP_SPMV_CONFIGS(X) X(256, 2048) X(256, 1024) X(512, 2048)
 
+// Per-system partial-sum buffer (entries): SpMV epilogues publish at
+// [0, 4 * max_grid), vector kernels whose producer's partials are still
+// being read in the same launch publish at [kVecPartOff, kPartials).
+constexpr int kPartials = 16 * 2048;
+constexpr int kVecPartOff = 8 * 2048;
+
 // Device-resident Krylov scalars.  Kernels publish per-workgroup partial
 // sums; a one-workgroup finalize launch adds them in a fixed order and
 // updates these scalars, so results are run-to-run deterministic.

@@ -325,12 +325,21 @@ struct EpiCgFast1 {
   }
 };
 
-// step 2: x += alpha p ; r -= alpha q ; z = r/dg ; partials r.z, r.r ; stop tests
+// step 2: alpha = rho / p.q, summed from step 1's partials by every
+// workgroup in the same fixed order (so all agree bit for bit; workgroup 0
+// records it) -- this replaces a one-workgroup finalize launch;
+// x += alpha p ; r -= alpha q ; z = r/dg ; partials r.z, r.r
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, T *partials,
-                                                        double *hist) {
+__global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
+                                                        int pq_g, T *partials, double *hist) {
   if (solve_done(st)) return;
-  const T alpha = st->alpha;
+  T pq[1];
+  block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
+  const T alpha = st->rho / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq[0];
+    st->alpha = alpha;
+  }
   T acc[2] = {(T)0, (T)0};
   for (int base = blockIdx.x * kVecBlock * kVI; base < n; base += gridDim.x * kVecBlock * kVI) {
     T xk[kVI], pk[kVI], rk[kVI], qk[kVI], dk[kVI];
@@ -464,10 +473,19 @@ struct EpiCgQ {
   }
 };
 
+// alpha = rho / p.q from Kc's partials, summed by every workgroup in the
+// same order (workgroup 0 records it), in place of a finalize launch
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T> *st, T *partials) {
+__global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
+                                                     int pq_g, T *partials) {
   if (solve_done(st)) return;
-  const T alpha = st->alpha;
+  T pq[1];
+  block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
+  const T alpha = st->rho / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq[0];
+    st->alpha = alpha;
+  }
   T acc[1] = {(T)0};
   for (int base = blockIdx.x * kVecBlock * kVI; base < n; base += gridDim.x * kVecBlock * kVI) {
     T xk[kVI], pk[kVI], rk[kVI], qk[kVI], dk[kVI];
@@ -493,23 +511,21 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T
 }
 
 template <typename T>
-struct FinRhoNew {
-  static constexpr int NV = 1;
-  KStateT<T> *st;
-  T *partials;
-  int stride;
-  __device__ void finish(T (&tot)[1]) {
-    if (threadIdx.x == 0) st->rho_new = tot[0];
-  }
-};
-
-template <typename T>
 struct EpiCgResidual {
   static constexpr int NV = 1;
   EHIP_EPI_MEMBERS(T)
   double *hist;
+  const T *rn_part;  // Kda's partials of rho_new = r.z
+  int rn_g;
   T beta;
-  __device__ void init() { beta = st->rho_new / st->rho; }
+  // rho_new summed from Kda's partials by every workgroup in the same order
+  // (workgroup 0 records it for the finalize), in place of a finalize launch
+  __device__ void init() {
+    T rn[1];
+    block_reduce_all<T, 1>(rn_part, 0, rn_g, rn);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->rho_new = rn[0];
+    beta = rn[0] / st->rho;
+  }
   struct Pre { T b, z, p; };
   __device__ Pre load(int i) const { return Pre{v.b[i], v.z[i], v.p[i]}; }
   __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
@@ -1114,10 +1130,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.p, e1, st);
           if (c.prof) prof_end(s);
-          fin<T>(s, e1, plan.grid, st);
-          hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
-          fin<T>(s, FinRhoNew<T>{st, part, g}, g, st);
-          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, (T)0};
+          hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
+                             plan.grid, part + kVecPartOff);
+          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, part + kVecPartOff, g, (T)0};
           launch_epi_fin<T>(s, plan, vals, v.x, e2, st);
           c.spmv += 2;
         }
@@ -1132,9 +1147,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.z, e1, st);
           if (c.prof) prof_end(s);
-          fin<T>(s, e1, plan.grid, st);
-          hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part, c.hist);
-          fin<T>(s, FinCgFast2<T>{st, part, g, c.hist}, g, st);
+          hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
+                             plan.grid, part + kVecPartOff, c.hist);
+          fin<T>(s, FinCgFast2<T>{st, part + kVecPartOff, g, c.hist}, g, st);
           c.spmv++;
         }
         if ((rc = read_state(s, st, hst))) return rc;
@@ -1306,10 +1321,10 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
       if (c.prof) prof_begin(s);
       launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
       if (c.prof) prof_end(s);
-      fin<float>(s, e1, s->plan32.grid, st32);
-      hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32, s->partials32,
+      hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32,
+                         (const float *)s->partials32, s->plan32.grid, s->partials32 + kVecPartOff,
                          (double *)nullptr);
-      fin<float>(s, FinCgFast2<float>{st32, s->partials32, g, nullptr}, g, st32);
+      fin<float>(s, FinCgFast2<float>{st32, s->partials32 + kVecPartOff, g, nullptr}, g, st32);
       c.spmv++;
     }
     if ((rc = read_state(s, st32, s->h_st32))) return rc;

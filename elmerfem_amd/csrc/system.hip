@@ -220,8 +220,8 @@ int ensure_state(ehip_system *s) {
   int rc = dmalloc(s, (void **)&s->st, sizeof(KState));
   if (rc) return rc;
   if ((rc = dmalloc(s, (void **)&s->st32, sizeof(KState32)))) return rc;
-  if ((rc = dmalloc(s, (void **)&s->partials, sizeof(double) * 4 * max_grid()))) return rc;
-  if ((rc = dmalloc(s, (void **)&s->partials32, sizeof(float) * 4 * max_grid()))) return rc;
+  if ((rc = dmalloc(s, (void **)&s->partials, sizeof(double) * kPartials))) return rc;
+  if ((rc = dmalloc(s, (void **)&s->partials32, sizeof(float) * kPartials))) return rc;
   EHIP_HIP_TRY(hipMemsetAsync(s->st, 0, sizeof(KState), s->stream));
   EHIP_HIP_TRY(hipMemsetAsync(s->st32, 0, sizeof(KState32), s->stream));
   EHIP_HIP_TRY(hipHostMalloc((void **)&s->h_st, sizeof(KState), hipHostMallocDefault));
