@@ -266,8 +266,11 @@ __global__ __launch_bounds__(BS) void k_element(ElemArgs a) {
   __shared__ double Mat[EPB][NGP][6];  // D, C, a0, a1, a2, load  (ModelPDE) | Young, Poisson (elasticity)
   __shared__ double dB[EPB][NGP][NPE][3];
   __shared__ double Bs[NGP][NPE];
-  const int e0 = blockIdx.x * EPB;
   for (int k = threadIdx.x; k < NGP * NPE; k += BS) Bs[k / NPE][k % NPE] = a.basis[k];
+  // a fixed grid walks the element batches: far fewer workgroups to
+  // dispatch than one per batch
+  for (int e0 = blockIdx.x * EPB; e0 < a.nelem; e0 += gridDim.x * EPB) {
+  __syncthreads();  // the previous batch's readers are done with the LDS tiles
   for (int k = threadIdx.x; k < EPB * NPE; k += BS) {
     const int le = k / NPE, nd = k % NPE, e = e0 + le;
     if (e < a.nelem) {
@@ -398,6 +401,7 @@ __global__ __launch_bounds__(BS) void k_element(ElemArgs a) {
       if (e < a.nelem) a.evec[(size_t)e * LD + i] = 0.0;
     }
   }
+  }  // element batches
 }
 
 // Values[s] = sum over the slot's contributions in element order, from 0
@@ -435,18 +439,18 @@ static void free_tables(DevTables &d) {
 
 template <int NPE, int NGP, int DIM, int CDIM, int PHYS, int EPB, int BS>
 static void launch_element(ehip_system *s, const ElemArgs &a) {
-  const int grid = (a.nelem + EPB - 1) / EPB;
+  const int grid = std::min((a.nelem + EPB - 1) / EPB, 8192);
   hipLaunchKernelGGL((k_element<NPE, NGP, DIM, CDIM, PHYS, EPB, BS>), dim3(grid), dim3(BS), 0, s->stream, a);
 }
 
 static int dispatch_element(ehip_system *s, int physics, int code, int cdim, const ElemArgs &a) {
   if (physics == EHIP_PHYS_MODELPDE) {
-    if (code == 808 && cdim == 3) { launch_element<8, 8, 3, 3, 0, 4, 256>(s, a); return EHIP_OK; }
+    if (code == 808 && cdim == 3) { launch_element<8, 8, 3, 3, 0, 8, 256>(s, a); return EHIP_OK; }
     if (code == 504 && cdim == 3) { launch_element<4, 4, 3, 3, 0, 16, 256>(s, a); return EHIP_OK; }
     if (code == 404 && cdim == 2) { launch_element<4, 4, 2, 2, 0, 16, 256>(s, a); return EHIP_OK; }
     if (code == 827 && cdim == 3) { launch_element<27, 27, 3, 3, 0, 1, 256>(s, a); return EHIP_OK; }
   } else {
-    if (code == 808) { launch_element<8, 8, 3, 3, 1, 4, 256>(s, a); return EHIP_OK; }
+    if (code == 808) { launch_element<8, 8, 3, 3, 1, 8, 256>(s, a); return EHIP_OK; }
     if (code == 827) { launch_element<27, 27, 3, 3, 1, 1, 256>(s, a); return EHIP_OK; }
   }
   set_error("ehip_assemble: element %d / physics %d / cdim %d not supported", code, physics, cdim);
