@@ -51,6 +51,19 @@ def fused_cg32_bytes(nnz: int, n: int, bits: int = 32) -> int:
     return 4 * nnz + index_bytes(nnz, bits) + 4 * (n + 1) + 4 * n + 16 * n
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB copy kernel's
+    reading), or None when the summary is for a different kernel."""
+    path = os.path.join(HERE, "profiles", "hbm_traffic_poisson_hex8_128.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    norm = lambda k: k.replace(" ", "").replace("ehip::", "")
+    return float(rec["traffic_bytes"]) if norm(rec.get("kernel", "")) == norm(kernel) else None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -393,14 +406,17 @@ def main():
     # roofline of the dominant kernel: the fused FP32 SpMV of the inner CG,
     # its launches sampled with HIP events on the system's stream in the timed solves
     bits = sys_.spmv_config(1)["index_bits"]
+    c32 = sys_.spmv_config(1)
+    kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
+                                                              "ColsU16" if bits == 16 else "ColsI32")
     us = float(np.mean([r["spmv_us"] for r in steps if r["spmv_samples"] > 0]))
     bytes32 = fused_cg32_bytes(nnz, n, bits)
     achieved = bytes32 / (us * 1e-6) / 1e9
+    traffic = measured_traffic(kname)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                "kernel": "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>> (fused FP32 SpMV + CG update)" % (
-                    sys_.spmv_config(1)["block"], sys_.spmv_config(1)["nnz_per_block"],
-                    "ColsU16" if bits == 16 else "ColsI32"),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "kernel": kname + " (fused FP32 SpMV + CG update)",
+                "traffic_source": "profiles/hbm_traffic_poisson_hex8_128.json" if traffic else None,
                 "bytes_per_launch": bytes32, "mean_us": us,
                 "samples": int(sum(r["spmv_samples"] for r in steps))}
     spmv = {"fp64_us": spmv64_us, "fp64_gbs": spmv_bytes(nnz, n, False, bits) / (spmv64_us * 1e-6) / 1e9,
