@@ -32,23 +32,27 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 
-def index_bytes(nnz: int, bits: int) -> int:
-    """Column index bytes one SpMV streams: int32 Cols, or 16-bit offsets plus
-    one int32 base per segment of 2048 nonzeros."""
+def index_bytes(nnz: int, bits: int, fmt: str = "csr", units: int = 0) -> int:
+    """Column index bytes one SpMV streams (padding not counted): int32
+    indices, or 16-bit offsets plus one int32 base per segment of 2048
+    nonzeros (CSR stream) or per slice (SELL-64, which also reads one offset
+    per slice)."""
+    if fmt == "sell":
+        return (4 if bits == 32 else 2) * nnz + (4 if bits == 32 else 8) * units
     return 4 * nnz if bits == 32 else 2 * nnz + 4 * ((nnz + 2047) // 2048 + 2)
 
 
-def spmv_bytes(nnz: int, n: int, fp32: bool = False, bits: int = 32) -> int:
-    """Bytes of one CSR SpMV: values, column indices, row pointers, x read
-    once, y written once."""
+def spmv_bytes(nnz: int, n: int, fp32: bool = False, bits: int = 32, fmt: str = "csr", units: int = 0) -> int:
+    """Algorithmic bytes of one SpMV: values, column indices, row pointers,
+    x read once, y written once."""
     vb = 4 if fp32 else 8
-    return vb * nnz + index_bytes(nnz, bits) + 4 * (n + 1) + 2 * vb * n
+    return vb * nnz + index_bytes(nnz, bits, fmt, units) + 4 * (n + 1) + 2 * vb * n
 
 
-def fused_cg32_bytes(nnz: int, n: int, bits: int = 32) -> int:
+def fused_cg32_bytes(nnz: int, n: int, bits: int = 32, fmt: str = "csr", units: int = 0) -> int:
     """Fused FP32 CG step 1 (q = A z + beta q, p = z + beta p, p.q):
     vals32 + column indices + rows + z + q (r/w) + p (r/w)."""
-    return 4 * nnz + index_bytes(nnz, bits) + 4 * (n + 1) + 4 * n + 16 * n
+    return 4 * nnz + index_bytes(nnz, bits, fmt, units) + 4 * (n + 1) + 4 * n + 16 * n
 
 
 def measured_traffic(kernel: str):
@@ -405,12 +409,15 @@ def main():
 
     # roofline of the dominant kernel: the fused FP32 SpMV of the inner CG,
     # its launches sampled with HIP events on the system's stream in the timed solves
-    bits = sys_.spmv_config(1)["index_bits"]
     c32 = sys_.spmv_config(1)
-    kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
-                                                              "ColsU16" if bits == 16 else "ColsI32")
+    bits = c32["index_bits"]
+    if c32["format"] == "sell":
+        kname = "k_spmv_epi_sell<float,256,8,%s,EpiCgFast1<float>>" % ("true" if bits == 16 else "false")
+    else:
+        kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
+                                                                  "ColsU16" if bits == 16 else "ColsI32")
     us = float(np.mean([r["spmv_us"] for r in steps if r["spmv_samples"] > 0]))
-    bytes32 = fused_cg32_bytes(nnz, n, bits)
+    bytes32 = fused_cg32_bytes(nnz, n, bits, c32["format"], c32["chunks"])
     achieved = bytes32 / (us * 1e-6) / 1e9
     traffic = measured_traffic(kname)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -419,8 +426,11 @@ def main():
                 "traffic_source": "profiles/hbm_traffic_poisson_hex8_128.json" if traffic else None,
                 "bytes_per_launch": bytes32, "mean_us": us,
                 "samples": int(sum(r["spmv_samples"] for r in steps))}
-    spmv = {"fp64_us": spmv64_us, "fp64_gbs": spmv_bytes(nnz, n, False, bits) / (spmv64_us * 1e-6) / 1e9,
-            "fp32_us": spmv32_us, "fp32_gbs": spmv_bytes(nnz, n, True, bits) / (spmv32_us * 1e-6) / 1e9,
+    c64 = sys_.spmv_config(0)
+    b64 = spmv_bytes(nnz, n, False, c64["index_bits"], c64["format"], c64["chunks"])
+    b32 = spmv_bytes(nnz, n, True, bits, c32["format"], c32["chunks"])
+    spmv = {"fp64_us": spmv64_us, "fp64_gbs": b64 / (spmv64_us * 1e-6) / 1e9, "fp64_bytes": b64,
+            "fp32_us": spmv32_us, "fp32_gbs": b32 / (spmv32_us * 1e-6) / 1e9, "fp32_bytes": b32,
             "fp64_gbs_int32_equiv": spmv_bytes(nnz, n) / (spmv64_us * 1e-6) / 1e9,
             "index_bits": bits, "copy_gbs_measured": copy_gbs,
             "config64": sys_.spmv_config(0), "config32": sys_.spmv_config(1)}

@@ -18,6 +18,7 @@ PRECOND_NONE, PRECOND_DIAGONAL = 0, 1
 FP64, MIXED_IR = 0, 1
 FAITHFUL, FAST = 0, 1
 PHYS_MODELPDE, PHYS_ELASTIC = 0, 1
+SPMV_CSR, SPMV_SELL = 0, 1
 
 HUTI_CONVERGENCE = 1
 HUTI_MAXITER = 2
@@ -76,7 +77,8 @@ SIGNATURES = {
     "ehip_solve_csr": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params), C.POINTER(Report)]),
     "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
-    "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_spmv_set_format": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ehip_spmv_set_schedule": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ehip_copy_bench": (C.c_int, [_i64, C.c_int, _vp, _vp]),
     "ehip_spmv_tune": (C.c_int, [_vp, C.c_int]),

@@ -579,6 +579,7 @@ extern "C" int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdi
     const int g = 2048;
     hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, s->stream, s->nnz, s->slot_start, s->slot_contrib, s->emat,
                        s->vals);
+    touch_vals(s, true, false);
     hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, s->stream, (int64_t)s->n, s->rhs_start, s->rhs_contrib,
                        s->evec, s->rhs);
   }

@@ -1004,6 +1004,7 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
     if (!s->vals32)
       if ((rc = dmalloc(s, (void **)&s->vals32, sizeof(float) * s->nnz))) return rc;
     hipLaunchKernelGGL(k_to_f32, dim3(2048), dim3(256), 0, s->stream, s->nnz, s->vals, s->vals32);
+    touch_vals(s, false, true);
   }
   hipLaunchKernelGGL(k_fill_x, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, 1.0e-8);
   // Jacobi divisors

@@ -81,6 +81,22 @@ struct ehip_system {
   int *seg_base = nullptr;
   int nseg = 0;
   bool c16 = false;
+  // SELL-64 copy for SpMV (see spmv_sell_body): slice s holds rows
+  // [64 s, 64 s + 64) column-major at [sell_off[s], sell_off[s+1]), padded to
+  // its longest row.  Built at pattern upload when the padding stays small;
+  // its values are refreshed from Values / the FP32 copy when those change.
+  int nslice = 0;
+  int64_t sell_nnz = 0;
+  int *sell_off = nullptr;            // nslice + 1
+  int *sell_pos = nullptr;            // nnz: SELL position of each CRS nonzero
+  int *sell_c32 = nullptr;            // sell_nnz column indices, or
+  unsigned short *sell_c16 = nullptr; // 16-bit offsets from sell_base[slice]
+  int *sell_base = nullptr;           // nslice
+  double *sell64 = nullptr;
+  float *sell32 = nullptr;
+  bool sell_ok = false;
+  int fmt64 = 0, fmt32 = 0;           // SpMV format per precision: 0 CSR stream, 1 SELL-64
+  uint64_t vals_ver = 1, vals32_ver = 1, sell64_ver = 0, sell32_ver = 0;
   double *vals = nullptr, *rhs = nullptr, *x = nullptr;
   float *vals32 = nullptr;
   double *dscale = nullptr;  // ScaleLinearSystem's D (n)
@@ -114,6 +130,9 @@ struct ehip_system {
   // sampled SpMV timing inside the Krylov loop
   int prof_every = 0, prof_count = 0, prof_n = 0;
   hipEvent_t prof_ev[2 * 512] = {};
+  // events for the next SpMV launch when it is a sample (hipExtLaunchKernelGGL
+  // timestamps the kernel itself), else null
+  hipEvent_t prof_start = nullptr, prof_stop = nullptr;
 };
 
 namespace ehip {
@@ -125,6 +144,25 @@ void dfree(ehip_system *s, void *p, size_t bytes);
 int ensure_work(ehip_system *s, int nvec);
 int ensure_work32(ehip_system *s, int nvec);
 int ensure_state(ehip_system *s);
+
+// Values (or the FP32 copy) changed: SELL copies must be refreshed before use.
+inline void touch_vals(ehip_system *s, bool v64, bool v32) {
+  if (v64) ++s->vals_ver;
+  if (v32) ++s->vals32_ver;
+}
+// SELL copies of the current values (refreshed on the system's stream when stale)
+const double *sell_values64(ehip_system *s);
+const float *sell_values32(ehip_system *s);
+
+// SELL-64 launch shape: nonzeros per lane in flight, and the grid (slice
+// groups of four per workgroup step, at most 2048 workgroups, a multiple of 8)
+constexpr int kSellU = 8;
+inline int sell_grid(const ehip_system *s) {
+  int g = (s->nslice + 3) / 4;
+  if (g > 2048) g = 2048;
+  if (g < 8) g = 8;
+  return (g + 7) / 8 * 8;
+}
 
 // SpMV
 int plan_spmv(ehip_system *s, SpmvPlan &plan, int block, int nb);

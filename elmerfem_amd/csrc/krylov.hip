@@ -15,6 +15,7 @@
 // host enqueues blocks of iterations and polls the scalars only every
 // `check_every` iterations.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cfloat>
@@ -119,6 +120,7 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
   ScaleValues f{s->vals, s->cols, s->dscale, make_f32 ? s->vals32 : nullptr};
   hipLaunchKernelGGL(k_nonzeros<ScaleValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
                      s->plan64.chunk_row, s->plan64.nchunks, f);
+  touch_vals(s, true, make_f32);
   double bnorm = 1.0;
   if (b) {
     double *dsum = reinterpret_cast<double *>(&s->st->ts);
@@ -147,6 +149,7 @@ int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale) {
   BackscaleValues f{s->vals, s->cols, s->dscale};
   hipLaunchKernelGGL(k_nonzeros<BackscaleValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
                      s->plan64.chunk_row, s->plan64.nchunks, f);
+  touch_vals(s, true, false);
   EHIP_HIP_TRY(hipGetLastError());
   return EHIP_OK;
 }
@@ -234,6 +237,7 @@ extern "C" int ehip_dirichlet(ehip_system *s, int nbc, const int *bc_rows, const
   DirichletValues f{s->vals, s->cols, D, dflag, symmetric};
   hipLaunchKernelGGL(k_nonzeros<DirichletValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
                      s->plan64.chunk_row, s->plan64.nchunks, f);
+  touch_vals(s, true, false);
   EHIP_HIP_TRY(hipGetLastError());
   EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
   dfree(s, dflag, n);
@@ -1028,19 +1032,58 @@ __global__ __launch_bounds__(kVecBlock) void k_sumsq(int n, const double *x, dou
 template <typename T, class CS, class Epi>
 static void launch_epi_cs(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
                           const KStateT<T> *st) {
-#define EHIP_EPI_CASE(BS, NB)                                                                              \
-  if (p.block == BS && p.nb == NB) {                                                                       \
-    hipLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows,    \
-                       s->cols, s->cols16, s->seg_base, vals, xin, p.chunk_row, p.nchunks, e, st);         \
-    return;                                                                                                \
+#define EHIP_EPI_CASE(BS, NB)                                                                               \
+  if (p.block == BS && p.nb == NB) {                                                                        \
+    hipExtLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream,           \
+                          s->prof_start, s->prof_stop, 0, s->rows, s->cols, s->cols16, s->seg_base, vals,    \
+                          xin, p.chunk_row, p.nchunks, e, st);                                               \
+    return;                                                                                                 \
   }
   EHIP_SPMV_CONFIGS(EHIP_EPI_CASE)
 #undef EHIP_EPI_CASE
 }
 
+template <typename T, int BS, int U, bool C16, class Epi>
+__global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ rows, int n,
+                                                      const int *__restrict__ c32,
+                                                      const unsigned short *__restrict__ c16,
+                                                      const int *__restrict__ sbase, const int *__restrict__ off,
+                                                      const T *__restrict__ vals, const T *__restrict__ xin,
+                                                      int nslice, Epi epi, const KStateT<T> *st) {
+  if (st && solve_done(st)) return;
+  const SellCols<C16> sc{c32, c16, sbase};
+  spmv_sell_body<T, T, BS, U>(rows, n, sc, off, vals, xin, nslice, epi);
+}
+
+// Workgroups (and so published partials) of an epilogue SpMV launch
+template <typename T>
+static int epi_grid(const ehip_system *s, const SpmvPlan &p) {
+  constexpr bool F32 = sizeof(T) == 4;
+  return ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) ? sell_grid(s) : p.grid;
+}
+
+// SpMV with epilogue on the system's active format (SELL-64 or CSR stream);
+// `vals` names the CSR values (FP64 Values or the FP32 copy) whose SELL
+// copy is used in the SELL format.
 template <typename T, class Epi>
 static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
                        const KStateT<T> *st) {
+  constexpr bool F32 = sizeof(T) == 4;
+  if ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) {
+    const T *sv;
+    if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
+    const int g = sell_grid(s);
+    // (a sampled launch carries its timing events: s->prof_start/stop)
+    if (s->sell_c16)
+      hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream,
+                            s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
+                            s->sell_off, sv, xin, s->nslice, e, st);
+    else
+      hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), 0, s->stream,
+                            s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
+                            s->sell_off, sv, xin, s->nslice, e, st);
+    return;
+  }
   if (s->c16)
     launch_epi_cs<T, ColsU16>(s, p, vals, xin, e, st);
   else
@@ -1053,7 +1096,7 @@ static void launch_epi_fin(ehip_system *s, const SpmvPlan &p, const T *vals, con
                            const KStateT<T> *st) {
   launch_epi<T, Epi>(s, p, vals, xin, e, st);
   if constexpr (Epi::NV > 0)
-    hipLaunchKernelGGL((k_fin<T, Epi>), dim3(1), dim3(256), 0, s->stream, e, p.grid, st);
+    hipLaunchKernelGGL((k_fin<T, Epi>), dim3(1), dim3(256), 0, s->stream, e, epi_grid<T>(s, p), st);
 }
 
 // vector kernel's finalize
@@ -1131,7 +1174,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           launch_epi<T>(s, plan, vals, v.p, e1, st);
           if (c.prof) prof_end(s);
           hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
-                             plan.grid, part + kVecPartOff);
+                             epi_grid<T>(s, plan), part + kVecPartOff);
           EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, part + kVecPartOff, g, (T)0};
           launch_epi_fin<T>(s, plan, vals, v.x, e2, st);
           c.spmv += 2;
@@ -1148,7 +1191,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           launch_epi<T>(s, plan, vals, v.z, e1, st);
           if (c.prof) prof_end(s);
           hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
-                             plan.grid, part + kVecPartOff, c.hist);
+                             epi_grid<T>(s, plan), part + kVecPartOff, c.hist);
           fin<T>(s, FinCgFast2<T>{st, part + kVecPartOff, g, c.hist}, g, st);
           c.spmv++;
         }
@@ -1172,7 +1215,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
         if (c.prof) prof_begin(s);
         launch_epi<T>(s, plan, vals, v.t1v, e3, st);
         if (c.prof) prof_end(s);
-        fin<T>(s, e3, plan.grid, st);
+        fin<T>(s, e3, epi_grid<T>(s, plan), st);
         hipLaunchKernelGGL(k_bi_s<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, part);
         fin<T>(s, FinSnorm<T>{st, part, g}, g, st);
         hipLaunchKernelGGL(k_bi_snorm_exit<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
@@ -1322,7 +1365,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
       launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
       if (c.prof) prof_end(s);
       hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32,
-                         (const float *)s->partials32, s->plan32.grid, s->partials32 + kVecPartOff,
+                         (const float *)s->partials32, epi_grid<float>(s, s->plan32), s->partials32 + kVecPartOff,
                          (double *)nullptr);
       fin<float>(s, FinCgFast2<float>{st32, s->partials32 + kVecPartOff, g, nullptr}, g, st32);
       c.spmv++;
@@ -1433,6 +1476,7 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
     ScaleValues f{s->vals, s->cols, s->dscale, s->vals32};
     hipLaunchKernelGGL(k_nonzeros<ScaleValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
                        s->plan64.chunk_row, s->plan64.nchunks, f);
+    touch_vals(s, true, true);
   }
   if (xzero)
     hipLaunchKernelGGL(k_fill<double>, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, 1.0e-8);

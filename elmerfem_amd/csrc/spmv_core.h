@@ -99,10 +99,25 @@ __device__ __forceinline__ void stream_chunk(const V *__restrict__ vals, const C
 //   __device__ void row(int r, A s, const Pre &, A (&acc)[NV]);
 //   __device__ void finish(A (&tot)[NV]);    run by the finalize kernel
 //   A *partials; int stride; unsigned *ticket;
-//
-// Software pipeline per workgroup: the Values/column stream of chunk c+1 is
-// issued before the row sums of chunk c, so its memory latency overlaps
-// the LDS phase instead of following it.
+
+// One chunk in flight: its bounds, the streamed values / column indices, and
+// this lane's first row's bounds and epilogue operands.  All of it is loaded
+// one chunk ahead, so nothing the row sums use is loaded (and waited for)
+// after the next chunk's stream has been issued.
+template <typename V, typename Raw, int ITEMS, class Pre>
+struct ChunkBuf {
+  V vv[ITEMS];
+  Raw cr[ITEMS];
+  int r0, r1, j0, len;
+  int ra, rb;
+  Pre pre;
+};
+
+// Software pipeline: the next chunk's value/index stream is issued right
+// after this chunk's x gathers, into the other of two register buffers, so
+// its memory latency overlaps the gathers, the products and the LDS row sums
+// of this chunk.  (Issued any earlier, the in-order vmcnt wait for the
+// gathers would also wait for it.)
 template <typename V, typename A, int BS, int NB, class CS, class Epi>
 __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
                                                  const CS &cols,
@@ -113,6 +128,7 @@ __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
   static_assert(NB % BS == 0, "chunk capacity must be a multiple of the workgroup size");
   static_assert(NB <= (1 << kSegShift), "a chunk may touch at most two column segments");
   constexpr int ITEMS = NB / BS;
+  using Buf = ChunkBuf<V, typename CS::Raw, ITEMS, typename Epi::Pre>;
   __shared__ A prod[NB];
   constexpr int NV = Epi::NV;
   constexpr int NA = NV > 0 ? NV : 1;
@@ -121,70 +137,155 @@ __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
   const int tid = (int)threadIdx.x;
-  XcdWalk w(nchunks);
-  if (w.first < w.last) {
-    V vv[ITEMS];
-    typename CS::Raw cr[ITEMS];
-    int c = w.first;
-    int r0 = chunk_row[c], r1 = chunk_row[c + 1];
-    int j0 = rows[r0];
-    int len = rows[r1] - j0;
-    stream_chunk<V, BS, ITEMS>(vals, cols, j0, len <= NB ? len : 1, vv, cr);
-    while (c < w.last) {
-      // the next chunk's bounds (the current chunk again after the last one)
-      const int cn = c + w.step;
-      const bool more = cn < w.last;
-      const int cq = more ? cn : c;
-      const int nr0 = chunk_row[cq], nr1 = chunk_row[cq + 1];
-      const int nj0 = rows[nr0];
-      const int nlen = rows[nr1] - nj0;
-      // this lane's first row, clamped into the chunk so that its operand
-      // and bound loads are unconditional (used only when rr < r1)
-      const int rr = r0 + tid;
-      const int rl = min(rr, r1 - 1);
-      const typename Epi::Pre pre = epi.load(rl);
-      const int ra = rows[rl], rb = rows[rl + 1];
-      if (len <= NB) {
-        const typename CS::Seg g = cols.seg(j0);
-        A xv[ITEMS];
+  const XcdWalk w(nchunks);
+
+  // loads chunk c's bounds into b and issues its stream (one cache line when
+  // c is past this workgroup's range -- b is then the current chunk `self`
+  // again and never processed -- or when c is a long row)
+  auto fill = [&](Buf &b, int c, int self) {
+    const bool valid = c < w.last;
+    const int cq = valid ? c : self;
+    b.r0 = chunk_row[cq];
+    b.r1 = chunk_row[cq + 1];
+    b.j0 = rows[b.r0];
+    b.len = rows[b.r1] - b.j0;
+    stream_chunk<V, BS, ITEMS>(vals, cols, b.j0, (valid && b.len <= NB) ? b.len : 1, b.vv, b.cr);
+    // this lane's first row, clamped into the chunk so that the loads are
+    // unconditional (used only when r0 + tid < r1)
+    const int rl = min(b.r0 + tid, b.r1 - 1);
+    b.ra = rows[rl];
+    b.rb = rows[rl + 1];
+    b.pre = epi.load(rl);
+  };
+
+  // processes the chunk held in b (chunk c) and streams chunk `next` into nb
+  auto process = [&](Buf &b, Buf &nb, int c, int next) {
+    const int r0 = b.r0, r1 = b.r1, j0 = b.j0, len = b.len;
+    const int rr = r0 + tid;
+    const int ra = b.ra, rb = b.rb;
+    const typename Epi::Pre pre = b.pre;
+    if (len <= NB) {
+      const typename CS::Seg g = cols.seg(j0);
+      A xv[ITEMS];
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) xv[k] = x[cols.col(cr[k], lane_nz<BS>(j0, len, k), g)];
-        // entries past the chunk are written too and never read
+      for (int k = 0; k < ITEMS; ++k) xv[k] = x[cols.col(b.cr[k], lane_nz<BS>(j0, len, k), g)];
+      fill(nb, next, c);
+      // entries past the chunk are written too and never read
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) prod[tid + k * BS] = (A)vv[k] * xv[k];
-      } else {
-        // a single row longer than the chunk capacity: workgroup-parallel sum
-        __shared__ A sh[16];
-        A part = (A)0;
-        for (int j = j0 + tid; j < j0 + len; j += BS) part += (A)ntload(vals + j) * x[cols.at_any(j)];
-        const A s = block_sum(part, sh);
-        if (tid == 0) epi.row(r0, s, pre, acc);
-      }
-      // the next chunk's stream, in flight during this chunk's row sums (one
-      // cache line when there is no next chunk or it is a long row)
-      stream_chunk<V, BS, ITEMS>(vals, cols, nj0, (more && nlen <= NB) ? nlen : 1, vv, cr);
+      for (int k = 0; k < ITEMS; ++k) prod[tid + k * BS] = (A)b.vv[k] * xv[k];
       lds_barrier();
-      if (len <= NB) {
-        if (rr < r1) {
-          A s = (A)0;
-          for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
-          epi.row(rr, s, pre, acc);
-        }
-        for (int r = rr + BS; r < r1; r += BS) {  // lanes holding more than one row
-          const int a = rows[r] - j0, b = rows[r + 1] - j0;
-          A s = (A)0;
-          for (int k = a; k < b; ++k) s = s + prod[k];
-          const typename Epi::Pre p2 = epi.load(r);
-          epi.row(r, s, p2, acc);
-        }
+      // (the row sums stay inside this branch: a join with the long-row path
+      // would make the compiler wait for the in-flight stream here)
+      if (rr < r1) {
+        A s = (A)0;
+        for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
+        epi.row(rr, s, pre, acc);
+      }
+      for (int r = rr + BS; r < r1; r += BS) {  // lanes holding more than one row
+        const int a = rows[r] - j0, bb = rows[r + 1] - j0;
+        A s = (A)0;
+        for (int k = a; k < bb; ++k) s = s + prod[k];
+        const typename Epi::Pre p2 = epi.load(r);
+        epi.row(r, s, p2, acc);
       }
       lds_barrier();
-      c = cn;
-      r0 = nr0;
-      r1 = nr1;
-      j0 = nj0;
-      len = nlen;
+    } else {
+      // a single row longer than the chunk capacity: workgroup-parallel sum
+      fill(nb, next, c);
+      __shared__ A sh[16];
+      A part = (A)0;
+      for (int j = j0 + tid; j < j0 + len; j += BS) part += (A)ntload(vals + j) * x[cols.at_any(j)];
+      const A s = block_sum(part, sh);
+      if (tid == 0) epi.row(r0, s, pre, acc);
+      __syncthreads();
     }
+  };
+
+  if (w.first < w.last) {
+    Buf b0, b1;
+    int c = w.first;
+    fill(b0, c, c);
+    for (;;) {
+      process(b0, b1, c, c + w.step);
+      c += w.step;
+      if (c >= w.last) break;
+      process(b1, b0, c, c + w.step);
+      c += w.step;
+      if (c >= w.last) break;
+    }
+  }
+  if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
+}
+
+// ---------------------------------------------------------------- SELL-64
+// Sliced ELLPACK with slices of 64 rows (one wavefront): inside slice s the
+// k-th nonzero of row 64 s + l sits at position off[s] + 64 k + l; rows
+// shorter than the slice's longest are padded (value 0, a valid column).
+// One lane owns one row and adds its products for k = 0, 1, ... in order --
+// the CRS order, so FP64 results stay bit-identical to the reference loop --
+// with no LDS and no barriers.  Consecutive lanes read consecutive positions
+// of the value and index streams, and for mesh matrices (neighbouring rows
+// have neighbouring columns) their x gathers fall in a few cache lines:
+// the CSR-stream kernel's per-lane scattered gathers, bound by the L1 tag
+// rate rather than HBM, become coalesced loads.
+template <bool C16>
+struct SellCols {
+  const int *c32;
+  const unsigned short *c16;
+  const int *base;  // per-slice column base (16-bit form)
+  __device__ __forceinline__ int col(int idx, int slice) const {
+    if constexpr (C16) return base[slice] + (int)ntload(c16 + idx);
+    else return ntload(c32 + idx);
+  }
+};
+
+// U: nonzeros per lane loaded before any of them is used
+template <typename V, typename A, int BS, int U, class SC, class Epi>
+__device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int n, const SC &sc,
+                                               const int *__restrict__ off, const V *__restrict__ vals,
+                                               const A *__restrict__ x, int nslice, Epi &epi) {
+  constexpr int WPB = BS / 64;  // slices per workgroup step
+  constexpr int NV = Epi::NV;
+  constexpr int NA = NV > 0 ? NV : 1;
+  A acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = (A)0;
+  epi.init();
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const XcdWalk w((nslice + WPB - 1) / WPB);
+  for (int gidx = w.first; gidx < w.last; gidx += w.step) {
+    const int sl = gidx * WPB + wv;  // wave-uniform
+    if (sl >= nslice) continue;
+    const int r = sl * 64 + lane;
+    const int rl = min(r, n - 1);
+    const typename Epi::Pre pre = epi.load(rl);
+    const int len = rows[rl + 1] - rows[rl];
+    const int p0 = off[sl];
+    const int wd = (off[sl + 1] - p0) >> 6;  // slice width
+    A sum = (A)0;
+    for (int k0 = 0; k0 < wd; k0 += U) {
+      V vv[U];
+      int cc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = p0 + min(k0 + u, wd - 1) * 64 + lane;  // clamped into the slice
+        vv[u] = ntload(vals + idx);
+        cc[u] = sc.col(idx, sl);
+      }
+      A xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = x[cc[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        // the product is rounded before it is added, as in the reference's
+        // loop (no fused multiply-add)
+#pragma clang fp contract(off)
+        const A t = sum + (A)vv[u] * xv[u];
+        sum = (k0 + u < len) ? t : sum;
+      }
+    }
+    if (r < n) epi.row(r, sum, pre, acc);
   }
   if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
 }

@@ -126,11 +126,36 @@ static int launch_spmv_cs(ehip_system *s, const SpmvPlan &p, const V *vals, cons
   return EHIP_EUNSUP;
 }
 
+template <typename V, typename A, int BS, int U, bool C16>
+__global__ __launch_bounds__(BS) void k_spmv_sell(const int *__restrict__ rows, int n,
+                                                  const int *__restrict__ c32, const unsigned short *__restrict__ c16,
+                                                  const int *__restrict__ sbase, const int *__restrict__ off,
+                                                  const V *__restrict__ vals, const A *__restrict__ x, int nslice,
+                                                  A *y) {
+  EpiStore<A> epi{y};
+  const SellCols<C16> sc{c32, c16, sbase};
+  spmv_sell_body<V, A, BS, U>(rows, n, sc, off, vals, x, nslice, epi);
+}
+
 template <typename V, typename A>
 static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A *x, A *y) {
   if (s->nnz == 0) {
     EHIP_HIP_TRY(hipMemsetAsync(y, 0, sizeof(A) * s->n, s->stream));
     return EHIP_OK;
+  }
+  constexpr bool F32 = sizeof(V) == 4;
+  if ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) {
+    const V *sv;
+    if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
+    if (!sv) return EHIP_ENOMEM;
+    const int g = sell_grid(s);
+    if (s->sell_c16)
+      hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, true>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                         s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
+    else
+      hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, false>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                         s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
+    return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
   }
   if (s->c16) return launch_spmv_cs<V, A, ColsU16>(s, p, vals, x, y);
   return launch_spmv_cs<V, A, ColsI32>(s, p, vals, x, y);
@@ -167,25 +192,27 @@ int ensure_work32(ehip_system *s, int nvec) {
 
 // ---------------------------------------------------------------- SpMV sampling
 
+// Sampling: prof_begin arms a pair of events for the next SpMV launch, which
+// the launch helpers pass to hipExtLaunchKernelGGL so that they timestamp the
+// kernel's own start and end; prof_end closes the sample.
 void prof_begin(ehip_system *s) {
+  s->prof_start = s->prof_stop = nullptr;
   if (s->prof_every <= 0) return;
   const bool take = (s->prof_count % s->prof_every) == 0 && s->prof_n < 512;
   if (take) {
-    hipEvent_t &e = s->prof_ev[2 * s->prof_n];
-    if (!e) (void)hipEventCreate(&e);
-    (void)hipEventRecord(e, s->stream);
+    hipEvent_t &e0 = s->prof_ev[2 * s->prof_n];
+    hipEvent_t &e1 = s->prof_ev[2 * s->prof_n + 1];
+    if (!e0) (void)hipEventCreate(&e0);
+    if (!e1) (void)hipEventCreate(&e1);
+    s->prof_start = e0;
+    s->prof_stop = e1;
   }
 }
 
 void prof_end(ehip_system *s) {
   if (s->prof_every <= 0) return;
-  const bool take = (s->prof_count % s->prof_every) == 0 && s->prof_n < 512;
-  if (take) {
-    hipEvent_t &e = s->prof_ev[2 * s->prof_n + 1];
-    if (!e) (void)hipEventCreate(&e);
-    (void)hipEventRecord(e, s->stream);
-    s->prof_n++;
-  }
+  if (s->prof_start) s->prof_n++;
+  s->prof_start = s->prof_stop = nullptr;
   s->prof_count++;
 }
 
@@ -227,6 +254,104 @@ int ensure_state(ehip_system *s) {
   EHIP_HIP_TRY(hipHostMalloc((void **)&s->h_st, sizeof(KState), hipHostMallocDefault));
   EHIP_HIP_TRY(hipHostMalloc((void **)&s->h_st32, sizeof(KState32), hipHostMallocDefault));
   return EHIP_OK;
+}
+
+// SELL-64 layout of the pattern (spmv_sell_body), kept when padding adds at
+// most a quarter to the nonzeros.  c0 holds the 0-based columns.  Padding
+// entries take the row's last column (value 0; they are never summed).
+static int build_sell(ehip_system *s, const std::vector<int> &c0) {
+  const int n = s->n;
+  const int64_t nnz = s->nnz;
+  const int *rows = s->h_rows.data();  // 1-based
+  const int S = (n + 63) / 64;
+  std::vector<int> off((size_t)S + 1, 0);
+  int64_t tot = 0;
+  for (int sl = 0; sl < S; ++sl) {
+    int wmax = 0;
+    for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) wmax = std::max(wmax, rows[r + 1] - rows[r]);
+    wmax = std::max(wmax, 1);
+    off[sl] = (int)tot;
+    tot += 64 * (int64_t)wmax;
+    if (tot >= (int64_t)INT32_MAX) return EHIP_OK;
+  }
+  off[S] = (int)tot;
+  if (tot > nnz + nnz / 4 + 64) return EHIP_OK;  // too much padding: CSR stream only
+  std::vector<int> pos((size_t)nnz), col((size_t)tot, 0);
+  std::vector<int> base((size_t)S, 0);
+  bool c16 = true;
+  for (int sl = 0; sl < S; ++sl) {
+    const int wd = (off[sl + 1] - off[sl]) / 64;
+    int lo = INT32_MAX, hi = INT32_MIN;
+    for (int l = 0; l < 64; ++l) {
+      const int r = sl * 64 + l;
+      const int len = r < n ? rows[r + 1] - rows[r] : 0;
+      const int j0 = r < n ? rows[r] - 1 : 0;
+      const int pad = len > 0 ? c0[j0 + len - 1] : std::min(r, n - 1);  // a column near the slice
+      for (int k = 0; k < wd; ++k) {
+        const int p = off[sl] + k * 64 + l;
+        const int c = k < len ? c0[j0 + k] : pad;
+        if (k < len) pos[j0 + k] = p;
+        col[p] = c;
+        lo = std::min(lo, c);
+        hi = std::max(hi, c);
+      }
+    }
+    base[sl] = lo;
+    if ((int64_t)hi - lo > 65535) c16 = false;
+  }
+  int rc;
+  if ((rc = dmalloc(s, (void **)&s->sell_off, sizeof(int) * off.size()))) return rc;
+  if ((rc = dmalloc(s, (void **)&s->sell_pos, sizeof(int) * pos.size()))) return rc;
+  if ((rc = dmalloc(s, (void **)&s->sell64, sizeof(double) * tot))) return rc;
+  EHIP_HIP_TRY(hipMemcpy(s->sell_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->sell_pos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemset(s->sell64, 0, sizeof(double) * tot));
+  if (c16) {
+    std::vector<unsigned short> c((size_t)tot);
+    for (int sl = 0; sl < S; ++sl)
+      for (int p = off[sl]; p < off[sl + 1]; ++p) c[p] = (unsigned short)(col[p] - base[sl]);
+    if ((rc = dmalloc(s, (void **)&s->sell_c16, sizeof(unsigned short) * tot))) return rc;
+    if ((rc = dmalloc(s, (void **)&s->sell_base, sizeof(int) * S))) return rc;
+    EHIP_HIP_TRY(hipMemcpy(s->sell_c16, c.data(), sizeof(unsigned short) * tot, hipMemcpyHostToDevice));
+    EHIP_HIP_TRY(hipMemcpy(s->sell_base, base.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+  } else {
+    if ((rc = dmalloc(s, (void **)&s->sell_c32, sizeof(int) * tot))) return rc;
+    EHIP_HIP_TRY(hipMemcpy(s->sell_c32, col.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
+  }
+  s->nslice = S;
+  s->sell_nnz = tot;
+  s->sell_ok = true;
+  s->fmt64 = s->fmt32 = 1;
+  return EHIP_OK;
+}
+
+template <typename TI, typename TO>
+__global__ void k_sell_fill(int64_t nnz, const int *__restrict__ pos, const TI *__restrict__ in, TO *__restrict__ out) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x)
+    out[pos[j]] = (TO)in[j];
+}
+
+const double *sell_values64(ehip_system *s) {
+  if (s->sell64_ver != s->vals_ver) {
+    hipLaunchKernelGGL((k_sell_fill<double, double>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
+                       (const double *)s->vals, s->sell64);
+    s->sell64_ver = s->vals_ver;
+  }
+  return s->sell64;
+}
+
+const float *sell_values32(ehip_system *s) {
+  if (!s->sell32) {
+    if (dmalloc(s, (void **)&s->sell32, sizeof(float) * s->sell_nnz)) return nullptr;
+    (void)hipMemsetAsync(s->sell32, 0, sizeof(float) * s->sell_nnz, s->stream);
+    s->sell32_ver = 0;
+  }
+  if (s->sell32_ver != s->vals32_ver) {
+    hipLaunchKernelGGL((k_sell_fill<float, float>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
+                       (const float *)s->vals32, s->sell32);
+    s->sell32_ver = s->vals32_ver;
+  }
+  return s->sell32;
 }
 
 }  // namespace ehip
@@ -325,6 +450,7 @@ int ehip_internal_upload_pattern(ehip_system *s) {
   if ((rc = plan_spmv(s, s->plan64, 256, 2048))) return rc;
   if ((rc = plan_spmv(s, s->plan32, 256, 2048))) return rc;
   if ((rc = build_cols16(s, tmp))) return rc;
+  if ((rc = build_sell(s, tmp))) return rc;
   return ensure_state(s);
 }
 
@@ -369,7 +495,8 @@ int ehip_system_free(ehip_system *s) {
                   s->partials,   s->st,          s->elements,  s->st32,       s->partials32,
                   s->hist,       s->slot_start,  s->slot_contrib, s->rhs_start, s->rhs_contrib,
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
-                  s->cols16,     s->seg_base};
+                  s->cols16,     s->seg_base,    s->sell_off,  s->sell_pos,   s->sell_c32,
+                  s->sell_c16,   s->sell_base,   s->sell64,    s->sell32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
@@ -405,6 +532,7 @@ int ehip_system_get_pattern(const ehip_system *s, int *perm, int *rows, int *col
 int ehip_system_set_values(ehip_system *s, const double *values, const double *rhs) {
   if (!s) return EHIP_EARG;
   if (values) EHIP_HIP_TRY(hipMemcpy(s->vals, values, sizeof(double) * s->nnz, hipMemcpyHostToDevice));
+  if (values) touch_vals(s, true, false);
   if (rhs) EHIP_HIP_TRY(hipMemcpy(s->rhs, rhs, sizeof(double) * s->n, hipMemcpyHostToDevice));
   if (values) s->has_values = true;
   return EHIP_OK;
@@ -520,14 +648,33 @@ int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call) 
   return EHIP_OK;
 }
 
-int ehip_spmv_config(ehip_system *s, int precision, int *block, int *nnz_per_block, int *chunks,
+int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int *nnz_per_block, int *chunks,
                      int *index_bits) {
   if (!s) return EHIP_EARG;
   const SpmvPlan &p = precision == 0 ? s->plan64 : s->plan32;
-  if (block) *block = p.block;
-  if (nnz_per_block) *nnz_per_block = p.nb;
-  if (chunks) *chunks = p.nchunks;
-  if (index_bits) *index_bits = s->c16 ? 16 : 32;
+  const int fmt = (precision == 0 ? s->fmt64 : s->fmt32) == 1 && s->sell_ok ? EHIP_SPMV_SELL : EHIP_SPMV_CSR;
+  if (format) *format = fmt;
+  if (fmt == EHIP_SPMV_SELL) {
+    if (block) *block = 256;
+    if (nnz_per_block) *nnz_per_block = 0;
+    if (chunks) *chunks = s->nslice;
+    if (index_bits) *index_bits = s->sell_c16 ? 16 : 32;
+  } else {
+    if (block) *block = p.block;
+    if (nnz_per_block) *nnz_per_block = p.nb;
+    if (chunks) *chunks = p.nchunks;
+    if (index_bits) *index_bits = s->c16 ? 16 : 32;
+  }
+  return EHIP_OK;
+}
+
+int ehip_spmv_set_format(ehip_system *s, int precision, int format) {
+  if (!s || (format != EHIP_SPMV_CSR && format != EHIP_SPMV_SELL)) return EHIP_EARG;
+  if (format == EHIP_SPMV_SELL && !s->sell_ok) {
+    set_error("ehip_spmv_set_format: this pattern has no SELL-64 layout (padding above the limit)");
+    return EHIP_EUNSUP;
+  }
+  (precision == 0 ? s->fmt64 : s->fmt32) = format == EHIP_SPMV_SELL ? 1 : 0;
   return EHIP_OK;
 }
 
@@ -541,6 +688,7 @@ int ehip_spmv_set_schedule(ehip_system *s, int precision, int block, int nnz_per
     set_error("ehip_spmv_set_schedule: no SpMV kernel for block %d, %d nonzeros per chunk", block, nnz_per_block);
     return EHIP_EUNSUP;
   }
+  (precision == 0 ? s->fmt64 : s->fmt32) = 0;  // a chunk schedule is a CSR-stream schedule
   return plan_spmv(s, precision == 0 ? s->plan64 : s->plan32, block, nnz_per_block);
 }
 
@@ -554,16 +702,24 @@ int ehip_spmv_tune(ehip_system *s, int precision) {
 #undef EHIP_CFG_ENTRY
   const int ncfg = (int)(sizeof(cfg) / sizeof(cfg[0]));
   SpmvPlan &plan = precision == 0 ? s->plan64 : s->plan32;
+  int &fmt = precision == 0 ? s->fmt64 : s->fmt32;
   double best = 1e30;
   int bi = 0, rc;
+  const int reps = s->nnz > 10000000 ? 20 : 200;
+  fmt = 0;
   for (int i = 0; i < ncfg; ++i) {
     if ((rc = plan_spmv(s, plan, cfg[i][0], cfg[i][1]))) return rc;
     double us = 0;
-    int reps = s->nnz > 10000000 ? 20 : 200;
     if ((rc = ehip_spmv_bench(s, precision, reps, &us))) return rc;
     if (us < best) { best = us; bi = i; }
   }
   if ((rc = plan_spmv(s, plan, cfg[bi][0], cfg[bi][1]))) return rc;
+  if (s->sell_ok) {
+    fmt = 1;
+    double us = 0;
+    if ((rc = ehip_spmv_bench(s, precision, reps, &us))) return rc;
+    if (!(us < best)) fmt = 0;
+  }
   if (precision == 0) s->tuned64 = true; else s->tuned32 = true;
   return EHIP_OK;
 }

@@ -186,11 +186,15 @@ class System:
         _lib.check(_lib.load().ehip_spmv_set_schedule(self._h, precision, block, nnz_per_block),
                    "ehip_spmv_set_schedule")
 
+    def spmv_set_format(self, fmt, precision=0):
+        _lib.check(_lib.load().ehip_spmv_set_format(self._h, precision, fmt), "ehip_spmv_set_format")
+
     def spmv_config(self, precision=0):
-        b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-        _lib.check(_lib.load().ehip_spmv_config(self._h, precision, C.byref(b), C.byref(nb), C.byref(ch),
-                                                C.byref(ib)), "ehip_spmv_config")
-        return {"block": b.value, "nnz_per_block": nb.value, "chunks": ch.value, "index_bits": ib.value}
+        f, b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.load().ehip_spmv_config(self._h, precision, C.byref(f), C.byref(b), C.byref(nb),
+                                                C.byref(ch), C.byref(ib)), "ehip_spmv_config")
+        return {"format": "sell" if f.value == _lib.SPMV_SELL else "csr", "block": b.value,
+                "nnz_per_block": nb.value, "chunks": ch.value, "index_bits": ib.value}
 
 
 def copy_bench(nbytes: int, reps: int = 20):

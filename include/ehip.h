@@ -182,13 +182,24 @@ int ehip_spmv_bench(ehip_system *s, int precision, int reps, double *us_per_call
  * times; returns the achieved GB/s (read + write) -- the measured HBM
  * ceiling, and the calibration kernel for hardware byte counters. */
 int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
-/* SpMV schedule of one precision (0 = FP64 values, 1 = FP32): workgroup
- * size, nonzeros per chunk, number of chunks, and the column index width the
- * kernels read (16: per-segment offsets, used when every segment of 2048
- * nonzeros spans < 65536 columns; 32: Cols itself). */
-int ehip_spmv_config(ehip_system *s, int precision, int *block, int *nnz_per_block, int *chunks,
+/* SpMV storage formats (the CRS arrays stay the system's reference copy):
+ *  EHIP_SPMV_CSR : CRS streamed in chunks of nonzeros, products staged in LDS;
+ *  EHIP_SPMV_SELL: SELL-64 copy (slices of 64 rows, column-major, padded to
+ *                  the slice's longest row), one lane per row -- built when
+ *                  padding adds at most 25% to the nonzeros. */
+#define EHIP_SPMV_CSR 0
+#define EHIP_SPMV_SELL 1
+/* SpMV configuration of one precision (0 = FP64 values, 1 = FP32): format,
+ * workgroup size, nonzeros per chunk (CSR; 0 for SELL), number of chunks
+ * (CSR) or slices (SELL), and the column index width the kernels read (16:
+ * offsets from a per-segment or per-slice base when every span is < 65536
+ * columns; 32: int32 indices).  Any pointer may be NULL. */
+int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int *nnz_per_block, int *chunks,
                      int *index_bits);
-/* Autotuner: times every compiled schedule on this matrix, keeps the fastest. */
+/* Selects the format for one precision (EHIP_EUNSUP: no SELL layout). */
+int ehip_spmv_set_format(ehip_system *s, int precision, int format);
+/* Autotuner: times every compiled CSR schedule and the SELL format on this
+ * matrix, keeps the fastest.  Every choice sums each row in the same order. */
 int ehip_spmv_tune(ehip_system *s, int precision);
 /* Sets a schedule explicitly (EHIP_EUNSUP if it is not compiled in).  Every
  * schedule sums each row in the same order, so results do not change. */

@@ -27,6 +27,25 @@ def row_rel_err(pat, a, b):
     return err
 
 
+def has_sell(s):
+    try:
+        s.spmv_set_format(_lib.SPMV_SELL, 0)
+        return True
+    except _lib.EhipError:
+        return False
+
+
+def set_format(s, fmt):
+    """Both precisions on one SpMV format; skips when the pattern has no
+    SELL-64 layout (small multi-DOF meshes pad their slices too much)."""
+    if fmt == _lib.SPMV_SELL and not has_sell(s):
+        s.free()
+        pytest.skip("no SELL-64 layout for this pattern (padding above the limit)")
+    s.spmv_set_format(fmt, 0)
+    s.spmv_set_format(fmt, 1)
+    assert s.spmv_config(0)["format"] == ("sell" if fmt == _lib.SPMV_SELL else "csr")
+
+
 def dense(rows, cols, vals):
     n = len(rows) - 1
     A = np.zeros((n, n))
@@ -48,9 +67,11 @@ def test_spmv_bit_exact(ehip, case):
         ref = np.zeros(pat.n)
         O.lib().or_spmv(pat.n, O._p(pat.rows), O._p(pat.cols), O._p(vals), O._p(u), O._p(ref))
         assert np.array_equal(s.spmv(u), ref)
-    for cfg in range(2):  # every schedule sums rows the same way
-        s.spmv_tune(0)
-        assert np.array_equal(s.spmv(u), ref)
+    for fmt in ((_lib.SPMV_SELL,) if has_sell(s) else ()) + (_lib.SPMV_CSR,):  # same row sums in every format
+        s.spmv_set_format(fmt)
+        assert np.array_equal(s.spmv(u), ref), fmt
+    s.spmv_tune(0)
+    assert np.array_equal(s.spmv(u), ref)
     s.free()
 
 
@@ -85,20 +106,31 @@ def test_spmv_every_schedule_and_index_width(ehip, far, bits):
     vals[diag - 1] = 4.0 + rng.random(n)
     s = System.from_csr(rows, cols, diag)
     s.set_values(vals, np.ones(n))
-    assert s.spmv_config(0)["index_bits"] == bits
     u = rng.standard_normal(n)
     ref = np.zeros(n)
     O.lib().or_spmv(n, O._p(rows), O._p(cols), O._p(vals), O._p(u), O._p(ref))
+    # SELL-64 (one lane per row); its index width follows the slice spans
+    s.spmv_set_format(_lib.SPMV_SELL)
+    cfg = s.spmv_config(0)
+    assert cfg["format"] == "sell" and cfg["index_bits"] == bits
+    assert np.array_equal(s.spmv(u), ref)
+    # every CSR-stream schedule
     for block, nb in _lib.SPMV_SCHEDULES:
         s.spmv_set_schedule(block, nb)
         cfg = s.spmv_config(0)
-        assert (cfg["block"], cfg["nnz_per_block"]) == (block, nb)
+        assert (cfg["format"], cfg["block"], cfg["nnz_per_block"], cfg["index_bits"]) == ("csr", block, nb, bits)
         assert np.array_equal(s.spmv(u), ref), (block, nb)
     with pytest.raises(_lib.EhipError):
         s.spmv_set_schedule(64, 100)
-    # FP32 values appear with a mixed-precision solve (on the scaled matrix)
+    # the tuner may pick either format: same bits
+    s.spmv_tune(0)
+    assert np.array_equal(s.spmv(u), ref)
+    # FP32 values appear with a mixed-precision solve (unscaled here, so they are Values rounded)
     s.solve(_lib.Params(0, 1, 1e-8, 50, 0, _lib.MIXED_IR, _lib.FAST, 0, 0, 0, 0, 0, 0))
     ref32 = dense_free_spmv(rows, cols, vals.astype(np.float32).astype(np.float64), u.astype(np.float32))
+    s.spmv_set_format(_lib.SPMV_SELL, precision=1)
+    got = s.spmv(u, precision=1)
+    assert np.abs(got - ref32).max() <= 1e-5 * np.abs(ref32).max()
     for block, nb in _lib.SPMV_SCHEDULES:
         s.spmv_set_schedule(block, nb, precision=1)
         got = s.spmv(u, precision=1)
@@ -132,6 +164,10 @@ def test_spmv_long_rows(ehip):
     vals = rng.standard_normal(len(cols))
     s = System.from_csr(rows, cols, diag)
     s.set_values(vals)
+    # two rows of ~5000 nonzeros would pad their slices far beyond the limit
+    assert s.spmv_config(0)["format"] == "csr"
+    with pytest.raises(_lib.EhipError):
+        s.spmv_set_format(_lib.SPMV_SELL)
     u = rng.standard_normal(n)
     assert np.allclose(s.spmv(u), dense(rows, cols, vals) @ u, rtol=1e-13, atol=1e-12)
 
@@ -243,13 +279,15 @@ def test_dirichlet_bit_exact(ehip, symmetric):
 
 @pytest.mark.parametrize("case", ["cg_poisson8", "cg_lognormal8", "cg_poisson32", "bicgstab_convdiff16",
                                   "bicgstab_poisson8", "cg_elastic3", "cg_tet4_4", "cg_none_lognormal8"])
-def test_faithful_solve_matches_reference(ehip, case):
+@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL, _lib.SPMV_CSR])
+def test_faithful_solve_matches_reference(ehip, case, fmt):
     spec = CASES[case]
     w, pat, vals, rhs, rows = assembled_case(spec)
     kw = w["kw"]
     kw.method, kw.precond, kw.tol, kw.maxit = spec["method"], spec["precond"], spec["tol"], spec["maxit"]
     s = System.from_csr(pat.rows, pat.cols, pat.diag)
     s.set_values(vals, rhs)
+    set_format(s, fmt)
     x, rep = s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, history=True)
     info, k, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, method=spec["method"],
                                                    precond=spec["precond"], tol=spec["tol"], maxit=spec["maxit"])
@@ -269,16 +307,18 @@ def test_faithful_solve_matches_reference(ehip, case):
     s.free()
 
 
+@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL, _lib.SPMV_CSR])
 @pytest.mark.parametrize("precision,mode", [(_lib.FP64, _lib.FAST), (_lib.MIXED_IR, _lib.FAST)])
 @pytest.mark.parametrize("case", ["cg_lognormal8", "cg_poisson32", "bicgstab_convdiff16", "cg_elastic3",
                                   "cg_tet4_4"])
-def test_fast_and_mixed_solve(ehip, case, precision, mode):
+def test_fast_and_mixed_solve(ehip, case, precision, mode, fmt):
     spec = CASES[case]
     w, pat, vals, rhs, rows = assembled_case(spec)
     kw = w["kw"]
     kw.method, kw.precond, kw.tol, kw.maxit = spec["method"], spec["precond"], 1e-10, 5000
     s = System.from_csr(pat.rows, pat.cols, pat.diag)
     s.set_values(vals, rhs)
+    set_format(s, fmt)
     x, rep = s.solve(kw, precision=precision, mode=mode)
     info, k, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, method=spec["method"],
                                                    precond=spec["precond"], tol=1e-12, maxit=5000)
