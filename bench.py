@@ -120,7 +120,9 @@ def form_batch(args):
     nnz = int(sum(s.nnz for s in systems))
     kw = W._kw("bicgstab", "diagonal", 1e-10, 20000)
     out = {}
-    for name, prec, mode in (("fp64_batched", _lib.FP64, _lib.FAITHFUL), ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
+    for name, prec, mode in (("fp64_faithful_batched", _lib.FP64, _lib.FAITHFUL),
+                             ("fp64_fast_batched", _lib.FP64, _lib.FAST),
+                             ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
         solve_batch(systems[:2], kw, precision=prec, mode=mode)  # warm-up
         t = time.perf_counter()
         xs, reps, ms = solve_batch(systems, kw, precision=prec, mode=mode)
@@ -131,14 +133,17 @@ def form_batch(args):
                      "iters_mean": float(it.mean()), "iters_max": int(it.max()), "converged": conv,
                      "res_true_max": float(max(r["res_true"] for r in reps))}
         log("%-14s %.2f s, %d/%d converged, %.3e DOF-iter/s" % (name, wall, conv, k, out[name]["dof_iters_per_s"]))
-    t = time.perf_counter()
-    it_seq = []
-    for s in systems:
-        _, r = s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False)
-        it_seq.append(r["iters"])
-    wall = time.perf_counter() - t
-    out["fp64_one_at_a_time"] = {"wall_s": wall, "dof_iters_per_s": float((dofs * np.array(it_seq)).sum() / wall)}
-    log("fp64 one at a time %.2f s" % wall)
+    for name, mode in (("fp64_faithful_one_at_a_time", _lib.FAITHFUL), ("fp64_fast_one_at_a_time", _lib.FAST)):
+        t = time.perf_counter()
+        it_seq, conv = [], 0
+        for s in systems:
+            _, r = s.solve(kw, precision=_lib.FP64, mode=mode, return_x=False)
+            it_seq.append(r["iters"])
+            conv += r["info"] == 1
+        wall = time.perf_counter() - t
+        out[name] = {"wall_s": wall, "dof_iters_per_s": float((dofs * np.array(it_seq)).sum() / wall),
+                     "converged": conv}
+        log("%s %.2f s, %d/%d converged" % (name, wall, conv, k))
     head = out["mixed_batched"]
     rec = {"metric": "DOF-iters/s, batched BiCGStab to 1e-10 (mixed precision)", "value": head["dof_iters_per_s"],
            "unit": "DOF-iter/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": 1000.0 * head["wall_s"],
@@ -148,8 +153,11 @@ def form_batch(args):
                       "nnz_total": nnz, "side_min": int(min(sizes)), "side_max": int(max(sizes)),
                       "seed": args.seed, "method": "bicgstab", "precond": "diagonal", "tol": 1e-10},
            "forms": out,
-           "speedup_vs_one_at_a_time": {n: out["fp64_one_at_a_time"]["wall_s"] / out[n]["wall_s"]
-                                        for n in ("fp64_batched", "mixed_batched")}}
+           "note": "HUTI-order FP64 BiCGStab stops with HUTI_BICGSTAB_SNORM (36) on some convection-dominated "
+                   "systems, as the reference HUTI does on the same matrices (checked against the compiled "
+                   "reference); the fast forms confirm convergence on the true residual and restart",
+           "speedup_vs_fp64_fast_one_at_a_time": {n: out["fp64_fast_one_at_a_time"]["wall_s"] / out[n]["wall_s"]
+                                                  for n in ("fp64_fast_batched", "mixed_batched")}}
     print(json.dumps(rec), flush=True)
     for s in systems:
         s.free()
@@ -189,62 +197,79 @@ def form_serve(args):
         seq_lat.append(time.perf_counter() - t1)
     seq_rate = P / (time.perf_counter() - t)
     log("one-at-a-time FP64: %.1f solves/s" % seq_rate)
-    rate = args.rate if args.rate > 0 else args.load * seq_rate
+    srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)))
+    if args.lane_workers > 0:
+        srv.set_lane(args.lane_dofs, args.lane_workers)
     R = args.requests
+
+    def serve(arrivals):
+        """Runs the request stream; returns (elapsed s, latencies s, infos, res_true, max live)."""
+        busy, queue = {}, []
+        lat = np.zeros(R)
+        infos = np.zeros(R, dtype=int)
+        res_true = np.zeros(R)
+        nxt = done = max_live = 0
+        t0 = time.perf_counter()
+        while done < R:
+            now = time.perf_counter() - t0
+            while nxt < R and arrivals[nxt] <= now:
+                queue.append(nxt)
+                nxt += 1
+            # admit every waiting request that fits (a large one waiting for a
+            # lane worker does not hold back small ones behind it)
+            still = []
+            for req in queue:
+                s = pool[req % P]
+                if s._h.value not in busy and srv.submit(s):
+                    busy[s._h.value] = req
+                else:
+                    still.append(req)
+            queue = still
+            live, _ = srv.status()
+            max_live = max(max_live, live)
+            if live == 0:
+                if nxt < R:
+                    time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.005)))
+                continue
+            for s, rep in srv.step():
+                req = busy.pop(s._h.value)
+                lat[req] = time.perf_counter() - t0 - arrivals[req]
+                infos[req] = rep["info"]
+                res_true[req] = rep["res_true"]
+                done += 1
+        return time.perf_counter() - t0, lat, infos, res_true, max_live
+
+    # capacity: every request waiting at t = 0
+    el, lat_sat, infos, rt, ml = serve(np.zeros(R))
+    capacity = R / el
+    log("server capacity %.1f solves/s (%.1fx one-at-a-time FP64)" % (capacity, capacity / seq_rate))
+    # latency under load: Poisson arrivals at a fixed fraction of that capacity
+    rate = args.rate if args.rate > 0 else args.load * capacity
     rng = np.random.default_rng(args.seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
-    srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)))
-    busy = {}
-    queue = []
-    lat = np.zeros(R)
-    infos = np.zeros(R, dtype=int)
-    res_true = np.zeros(R)
-    nxt = 0
-    done = 0
-    max_live = 0
-    t0 = time.perf_counter()
-    while done < R:
-        now = time.perf_counter() - t0
-        while nxt < R and arrivals[nxt] <= now:
-            queue.append(nxt)
-            nxt += 1
-        while queue:
-            req = queue[0]
-            s = pool[req % P]
-            if s._h.value in busy or not srv.submit(s):
-                break
-            busy[s._h.value] = req
-            queue.pop(0)
-        live, _ = srv.status()
-        max_live = max(max_live, live)
-        if live == 0:
-            if nxt < R:
-                time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.01)))
-            continue
-        for s, rep in srv.step():
-            req = busy.pop(s._h.value)
-            lat[req] = time.perf_counter() - t0 - arrivals[req]
-            infos[req] = rep["info"]
-            res_true[req] = rep["res_true"]
-            done += 1
-    elapsed = time.perf_counter() - t0
+    el2, lat, infos2, rt2, max_live = serve(arrivals)
     srv.free()
-    served_rate = R / elapsed
-    rec = {"metric": "solves/s and p50/p99 latency, continuous-batching mixed-precision CG to 1e-10",
-           "value": served_rate, "unit": "solves/s", "n_gpus": 1, "steps": R, "warmup": 0,
-           "ms_per_step": 1000.0 * elapsed / R, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+    log("at %.1f arrivals/s: p50 %.2f ms, p99 %.2f ms" % (rate, 1000 * np.percentile(lat, 50),
+                                                          1000 * np.percentile(lat, 99)))
+    rec = {"metric": "solves/s (capacity) and p50/p99 latency at a fixed arrival rate, continuous-batching "
+                     "mixed-precision CG to 1e-10",
+           "value": capacity, "unit": "solves/s", "n_gpus": 1, "steps": R, "warmup": 0,
+           "ms_per_step": 1000.0 * el / R, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 inner / f64 refinement", "data": "synthetic",
-           "config": {"workload": "heat_tet4_serving", "requests": R, "arrival_rate_per_s": rate,
-                      "pool_systems": P, "n_min": int(min(sizes)), "n_max": int(max(sizes)),
+           "config": {"workload": "heat_tet4_serving", "requests": R, "pool_systems": P,
+                      "n_min": int(min(sizes)), "n_max": int(max(sizes)),
                       "dofs_mean": float(np.mean([s.n for s in pool])), "slots": args.slots,
+                      "lane_min_dofs": args.lane_dofs, "lane_workers": args.lane_workers,
                       "mem_budget_gib": args.budget_gib, "seed": args.seed, "tol": 1e-10},
-           "latency_ms": {"p50": 1000.0 * float(np.percentile(lat, 50)), "p99": 1000.0 * float(np.percentile(lat, 99)),
-                          "max": 1000.0 * float(lat.max())},
-           "one_at_a_time_fp64": {"solves_per_s": seq_rate,
+           "arrival_rate_per_s": rate, "served_rate_under_load": R / el2,
+           "latency_ms": {"p50": 1000.0 * float(np.percentile(lat, 50)),
+                          "p99": 1000.0 * float(np.percentile(lat, 99)), "max": 1000.0 * float(lat.max())},
+           "one_at_a_time_fp64": {"solves_per_s": seq_rate, "method": "FP64 CG in HUTI order",
                                   "latency_ms_p50": 1000.0 * float(np.percentile(seq_lat, 50)),
                                   "latency_ms_p99": 1000.0 * float(np.percentile(seq_lat, 99))},
-           "speedup_solves_per_s": served_rate / seq_rate, "converged": int((infos == 1).sum()),
-           "res_true_max": float(res_true.max()), "max_live": max_live}
+           "speedup_solves_per_s": capacity / seq_rate,
+           "converged": int((infos == 1).sum() + (infos2 == 1).sum()), "requests_total": 2 * R,
+           "res_true_max": float(max(rt.max(), rt2.max())), "max_live": max_live}
     print(json.dumps(rec), flush=True)
     for s in pool:
         s.free()
@@ -312,8 +337,11 @@ def main():
     ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
     ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request systems")
     ap.add_argument("--requests", type=int, default=1000, help="serve form: requests")
-    ap.add_argument("--rate", type=float, default=0.0, help="serve form: arrivals per second (0: load x FP64 rate)")
-    ap.add_argument("--load", type=float, default=8.0, help="serve form: arrival rate / one-at-a-time FP64 rate")
+    ap.add_argument("--rate", type=float, default=0.0, help="serve form: arrivals per second (0: load x capacity)")
+    ap.add_argument("--load", type=float, default=0.7, help="serve form: arrival rate / measured capacity")
+    ap.add_argument("--lane-dofs", type=int, default=30000, help="serve form: unknowns from which a request "
+                                                                   "takes the concurrent lane")
+    ap.add_argument("--lane-workers", type=int, default=4, help="serve form: concurrent-lane workers (0: off)")
     ap.add_argument("--slots", type=int, default=64, help="serve form: server slots")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="serve form: HBM admission budget")
     ap.add_argument("--gpus", type=int, default=1)

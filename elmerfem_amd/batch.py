@@ -51,6 +51,11 @@ class Server:
         self.max_slots = max_slots
         self._live = {}
 
+    def set_lane(self, min_dofs: int, workers: int = 4):
+        """Systems of at least min_dofs unknowns are solved whole on `workers`
+        host threads (overlapping on the device) instead of being batched."""
+        _lib.check(_lib.load().ehip_server_set_lane(self._h, int(min_dofs), int(workers)), "ehip_server_set_lane")
+
     def submit(self, system) -> bool:
         """Admits the system; False when no slot or budget is free (keep it queued)."""
         slot = C.c_int()
@@ -63,7 +68,7 @@ class Server:
 
     def step(self):
         """One block of iterations for every live system; returns [(system, report)] of finished ones."""
-        k = self.max_slots
+        k = self.max_slots + 64  # batched slots plus the concurrent lane's workers
         nd = C.c_int()
         sys_out = (C.c_void_p * k)()
         rep_out = (_lib.Report * k)()

@@ -26,7 +26,10 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <condition_variable>
 #include <map>
+#include <mutex>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -71,7 +74,8 @@ struct BSys {
   float rho32, rho_new32, alpha32, omega32, beta32, rhsnorm32, res32, itol;
   int iter, info, done, phase, maxit, outer, inner_iter, inner_total, inner_maxit, snorm_exit;
   unsigned ticket;
-  int pad;
+  int inner_bad;   // the inner solve ended on a non-finite residual: its correction is dropped
+  int bad_count;   // consecutive such inner solves
 };
 
 struct BatchArgs {
@@ -486,6 +490,7 @@ struct CtxM {
   double scale;
   double ratio;  // previous residual scale / current one (0 before the first replacement)
   float alpha, beta, omega;
+  int bad;       // the inner solve broke down: no correction
 };
 
 __device__ __forceinline__ CtxM ctxm(const BSys &st, const SlotPtrs &sp) {
@@ -502,6 +507,7 @@ __device__ __forceinline__ CtxM ctxm(const BSys &st, const SlotPtrs &sp) {
   c.alpha = st.alpha32;
   c.beta = st.beta32;
   c.omega = st.omega32;
+  c.bad = st.inner_bad;
   return c;
 }
 
@@ -589,8 +595,9 @@ __device__ __forceinline__ void inner_step_done(BSys &st, float res, bool breakd
   st.res32 = res;
   st.inner_iter += 1;
   st.inner_total += 1;
-  if (res < st.itol || breakdown || !(res == res) || st.inner_iter >= st.inner_maxit ||
-      st.inner_total >= st.maxit)
+  const bool finite = res == res && res <= 3.4e38f;
+  if (!finite) st.inner_bad = 1;
+  if (res < st.itol || breakdown || !finite || st.inner_iter >= st.inner_maxit || st.inner_total >= st.maxit)
     st.phase = PH_CORR;
 }
 
@@ -745,9 +752,27 @@ struct OpCorr : OpBase<double> {
   __device__ bool active(const BSys &st) const { return st.phase == PH_CORR; }
   __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
   __device__ void row(Ctx &c, int i, double, double (&)[1]) const {
-    c.x64[i] = c.x64[i] + c.scale * (double)c.v.x[i];
+    if (!c.bad) c.x64[i] = c.x64[i] + c.scale * (double)c.v.x[i];
   }
-  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&)[1]) const { st.phase = PH_RES; }
+  // An inner solve that broke down with a non-finite residual leaves x as it
+  // was; the next refinement step restarts from the FP64 residual with the
+  // inner iteration cap halved below the breakdown point (the restart is
+  // deterministic, so the same cap would break down the same way).  Three in
+  // a row end the solve as diverged.
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&)[1]) const {
+    if (st.inner_bad) {
+      st.inner_bad = 0;
+      st.inner_maxit = st.inner_iter / 2 > 1 ? st.inner_iter / 2 : 1;
+      st.scale = 0.0;  // so the next replacement starts CG afresh: no search direction carried over
+      if (++st.bad_count >= 3) {
+        slot_done(a, st, 3);
+        return;
+      }
+    } else {
+      st.bad_count = 0;
+    }
+    st.phase = PH_RES;
+  }
 };
 
 // ---------------------------------------------------------------- per-slot setup kernels
@@ -815,6 +840,28 @@ struct ehip_batch {
   int64_t bytes = 0;
   int64_t budget = 0;              // serving: HBM budget for solver workspace (0 = none)
   std::vector<int> batch_members;  // for ehip_batch_* : slot of each system
+  struct Lane *lane = nullptr;     // serving: concurrent lane for large systems (null = off)
+};
+
+// Serving's concurrent lane: a system of at least min_dofs unknowns is not
+// batched -- one iteration of a large system fills the GPU on its own -- but
+// solved by ehip_solve on one of `workers` host threads, each on the
+// system's own stream, so large solves overlap one another and the batched
+// small ones on the device.  A request waits (EHIP_ECAP) until a worker is
+// free and the HBM budget allows its workspace.
+struct Lane {
+  std::vector<std::thread> th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ehip_system *> q;
+  std::vector<std::pair<ehip_system *, ehip_report>> done;
+  std::map<ehip_system *, int64_t> need;
+  int running = 0;  // queued or being solved
+  int64_t bytes = 0;
+  int64_t min_dofs = 0;
+  int workers = 0;
+  bool stop = false;
+  ehip_params p{};
 };
 
 namespace ehip {
@@ -1079,8 +1126,49 @@ static int engine_step(ehip_batch *b) {
   return EHIP_OK;
 }
 
+static void lane_stop(ehip_batch *b) {
+  Lane *L = b->lane;
+  if (!L) return;
+  {
+    std::lock_guard<std::mutex> lk(L->mu);
+    L->stop = true;
+  }
+  L->cv_work.notify_all();
+  for (auto &t : L->th) t.join();
+  delete L;
+  b->lane = nullptr;
+}
+
+static void lane_worker(Lane *L) {
+  for (;;) {
+    ehip_system *s = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(L->mu);
+      L->cv_work.wait(lk, [&] { return L->stop || !L->q.empty(); });
+      if (L->q.empty()) return;  // stop requested and nothing left
+      s = L->q.front();
+      L->q.pop_front();
+    }
+    ehip_report r{};
+    if (hipSetDevice(s->dev) != hipSuccess || ehip_solve(s, &L->p, nullptr, &r, nullptr) != EHIP_OK) r.info = 3;
+    {
+      std::lock_guard<std::mutex> lk(L->mu);
+      L->done.push_back({s, r});
+    }
+    L->cv_done.notify_all();
+  }
+}
+
+// workspace a single-system mixed or FP64 solve keeps on the device
+static int64_t lane_bytes(const ehip_system *s, bool mixed) {
+  const int64_t n = s->n, nnz = s->nnz;
+  if (!mixed) return n * 8 * 12;
+  return n * (4 * 8 + 8 * 2) + (s->vals32 ? 0 : 4 * nnz) + (s->sell_ok && !s->sell32 ? 4 * s->sell_nnz : 0);
+}
+
 static void engine_free(ehip_batch *b) {
   if (!b) return;
+  lane_stop(b);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto &sl : b->slot) {
     if (sl.w64) (void)hipFree(sl.w64);
@@ -1197,8 +1285,33 @@ extern "C" int ehip_server_create(const ehip_params *p, int max_slots, int64_t m
 extern "C" int ehip_server_submit(ehip_batch *b, ehip_system *s, int *slot_out) {
   if (!b || !s) return EHIP_EARG;
   const int64_t budget = b->budget;
+  Lane *L = b->lane;
+  if (L && s->n >= L->min_dofs) {
+    if (!s->has_values) {
+      set_error("server: the system has no values");
+      return EHIP_ESTATE;
+    }
+    const int64_t need = lane_bytes(s, b->mixed != 0);
+    std::lock_guard<std::mutex> lk(L->mu);
+    if (L->running >= L->workers) {
+      set_error("server: every concurrent-lane worker is busy");
+      return EHIP_ECAP;
+    }
+    if (budget > 0 && b->bytes + L->bytes + need > budget) {
+      set_error("server: HBM budget %lld bytes would be exceeded", (long long)budget);
+      return EHIP_ECAP;
+    }
+    L->q.push_back(s);
+    L->need[s] = need;
+    L->bytes += need;
+    L->running++;
+    L->cv_work.notify_one();
+    if (slot_out) *slot_out = -1;
+    return EHIP_OK;
+  }
   const int64_t need = slot_bytes(s, b->mixed != 0);
-  if (budget > 0 && b->bytes + need > budget) {
+  const int64_t lane_b = L ? L->bytes : 0;
+  if (budget > 0 && b->bytes + lane_b + need > budget) {
     set_error("server: HBM budget %lld bytes would be exceeded", (long long)budget);
     return EHIP_ECAP;
   }
@@ -1213,15 +1326,42 @@ extern "C" int ehip_server_submit(ehip_batch *b, ehip_system *s, int *slot_out) 
 // Runs one graph replay (graph_iters iterations of every live system), then
 // retires the finished ones: their systems get back-scaled solutions and
 // reports.  done_systems/done_reports receive up to max_done entries.
+// Collects finished concurrent-lane solves into the output arrays; with
+// `wait`, blocks up to wait_ms for one when none is ready.
+static void lane_collect(ehip_batch *b, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
+                         int max_done, bool wait, int wait_ms) {
+  Lane *L = b->lane;
+  if (!L) return;
+  std::unique_lock<std::mutex> lk(L->mu);
+  if (wait && L->done.empty() && L->running > 0)
+    L->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), [&] { return !L->done.empty(); });
+  size_t k = 0;
+  for (; k < L->done.size() && *ndone < max_done; ++k) {
+    auto &d = L->done[k];
+    if (done_systems) done_systems[*ndone] = d.first;
+    if (done_reports) done_reports[*ndone] = d.second;
+    (*ndone)++;
+    L->running--;
+    L->bytes -= L->need[d.first];
+    L->need.erase(d.first);
+  }
+  L->done.erase(L->done.begin(), L->done.begin() + (long)k);
+}
+
 extern "C" int ehip_server_step(ehip_batch *b, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
                                 int max_done) {
   if (!b || !ndone) return EHIP_EARG;
   *ndone = 0;
   int live = 0;
   for (auto &sl : b->slot) live += sl.used;
-  if (!live) return EHIP_OK;
+  if (!live) {
+    // nothing batched: wait briefly for a concurrent-lane solve instead of spinning
+    lane_collect(b, ndone, done_systems, done_reports, max_done, true, 2);
+    return EHIP_OK;
+  }
   int rc = engine_step(b);
   if (rc) return rc;
+  lane_collect(b, ndone, done_systems, done_reports, max_done, false, 0);
   bool changed = false;
   for (int i = 0; i < b->capacity; ++i) {
     auto &sl = b->slot[i];
@@ -1243,8 +1383,37 @@ extern "C" int ehip_server_status(const ehip_batch *b, int *live, int64_t *bytes
   if (!b) return EHIP_EARG;
   int l = 0;
   for (auto &sl : b->slot) l += sl.used;
+  int64_t by = b->bytes;
+  if (b->lane) {
+    std::lock_guard<std::mutex> lk(b->lane->mu);
+    l += b->lane->running;
+    by += b->lane->bytes;
+  }
   if (live) *live = l;
-  if (bytes) *bytes = b->bytes;
+  if (bytes) *bytes = by;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_server_set_lane(ehip_batch *b, int64_t min_dofs, int workers) {
+  if (!b || workers < 0 || workers > 64 || min_dofs < 0) return EHIP_EARG;
+  if (b->lane) {
+    std::lock_guard<std::mutex> lk(b->lane->mu);
+    if (b->lane->running) {
+      set_error("server: the concurrent lane is busy");
+      return EHIP_ESTATE;
+    }
+  }
+  lane_stop(b);
+  if (workers == 0) return EHIP_OK;
+  Lane *L = new Lane();
+  L->min_dofs = min_dofs;
+  L->workers = workers;
+  L->p = b->p;
+  L->p.mode = EHIP_FAST;
+  L->p.record_history = 0;
+  L->p.profile_every = 0;
+  for (int i = 0; i < workers; ++i) L->th.emplace_back(lane_worker, L);
+  b->lane = L;
   return EHIP_OK;
 }
 

@@ -232,6 +232,12 @@ int ehip_server_submit(ehip_server *srv, ehip_system *s, int *slot);
 int ehip_server_step(ehip_server *srv, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
                      int max_done);
 int ehip_server_status(const ehip_server *srv, int *live, int64_t *bytes);
+/* Concurrent lane: systems with at least min_dofs unknowns are not batched
+ * but solved whole (ehip_solve) on one of `workers` host threads, each on
+ * the system's own stream, overlapping one another and the batched systems
+ * on the device; a submission waits (EHIP_ECAP) until a worker is free.
+ * workers = 0 turns the lane off (the default: everything is batched). */
+int ehip_server_set_lane(ehip_server *srv, int64_t min_dofs, int workers);
 int ehip_server_free(ehip_server *srv);
 
 #ifdef __cplusplus

@@ -138,3 +138,47 @@ def test_server_admission_budget(ehip):
         srv.step()
     assert srv.submit(s2)
     srv.free()
+
+
+def test_server_concurrent_lane(ehip):
+    """Large systems go to the concurrent lane (whole single-system solves on
+    worker threads) while small ones are batched; every result equals its
+    path's result alone, bit for bit."""
+    ws = [make("tet4", i) for i in range(6)]
+    kw = ws[0]["kw"]
+    systems = [W.build_system(w, optimize_bandwidth=True)[0] for w in ws]
+    sizes = sorted(s.n for s in systems)
+    cut = sizes[len(sizes) // 2]          # the larger half takes the lane
+    alone = []
+    for w, s0 in zip(ws, systems):
+        s = W.build_system(w, optimize_bandwidth=True)[0]
+        if s.n >= cut:
+            x, rep = s.solve(kw, precision=_lib.MIXED_IR, mode=_lib.FAST)
+        else:
+            xs, reps, _ = solve_batch([s], kw, precision=_lib.MIXED_IR, mode=_lib.FAST)
+            x = xs[0]
+        alone.append(x)
+        s.free()
+    srv = Server(kw, precision=_lib.MIXED_IR, max_slots=2)
+    srv.set_lane(cut, workers=2)
+    pending = list(enumerate(systems))
+    ids, results, steps = {}, {}, 0
+    while pending or srv.status()[0]:
+        while pending:
+            i, s = pending[0]
+            if not srv.submit(s):
+                break
+            ids[s._h.value] = i
+            pending.pop(0)
+        for s, rep in srv.step():
+            x = np.zeros(s.n)
+            _lib.check(_lib.load().ehip_get_solution(s._h, x.ctypes.data_as(__import__("ctypes").c_void_p)), "sol")
+            results[ids[s._h.value]] = (x, rep)
+        steps += 1
+        assert steps < 100000
+    assert srv.status() == (0, 0)
+    for i in range(len(ws)):
+        x, rep = results[i]
+        assert rep["info"] == 1
+        assert np.array_equal(x, alone[i]), i
+    srv.free()
