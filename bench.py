@@ -304,7 +304,7 @@ def form_elastic(args):
                              ("mixed", _lib.MIXED_IR, _lib.FAST)):
         sys_.set_values(vals, rhs)
         t = time.perf_counter()
-        x, r = sys_.solve(kw, precision=prec, mode=mode)
+        x, r = sys_.solve(kw, precision=prec, mode=mode, inner_tol=args.inner_tol if prec == _lib.MIXED_IR else 0.0)
         wall = time.perf_counter() - t
         xs[name] = x
         out[name] = {"iters": r["iters"], "ms": 1000.0 * wall, "info": r["info"], "res_true": r["res_true"],
@@ -334,6 +334,8 @@ def main():
                     help="headline: the 128^3 solve; batch / serve: the batching and serving forms; "
                          "elastic: the precision form on elasticity (--n 96, or --quadratic --n 48)")
     ap.add_argument("--quadratic", action="store_true", help="elastic form: Hex27 elements")
+    ap.add_argument("--inner-tol", type=float, default=0.0,
+                    help="mixed CG: residual drop between FP64 replacements (0: the library default)")
     ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
     ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request systems")
     ap.add_argument("--requests", type=int, default=1000, help="serve form: requests")
@@ -399,7 +401,8 @@ def main():
         out = []
         for _ in range(reps):
             t = time.perf_counter()
-            x, rep = sys_.solve(kw, precision=precision, mode=mode, return_x=False, profile_every=profile_every)
+            x, rep = sys_.solve(kw, precision=precision, mode=mode, return_x=False, profile_every=profile_every,
+                                inner_tol=args.inner_tol if precision == _lib.MIXED_IR else 0.0)
             rep["wall_ms"] = 1000.0 * (time.perf_counter() - t)
             out.append(rep)
             log("%-14s info=%d iters=%d outer=%d res_scaled=%.3e res_true=%.3e wall=%.1f ms krylov=%.1f ms" % (

@@ -1034,9 +1034,13 @@ static void launch_epi_cs(ehip_system *s, const SpmvPlan &p, const T *vals, cons
                           const KStateT<T> *st) {
 #define EHIP_EPI_CASE(BS, NB)                                                                               \
   if (p.block == BS && p.nb == NB) {                                                                        \
-    hipExtLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream,           \
-                          s->prof_start, s->prof_stop, 0, s->rows, s->cols, s->cols16, s->seg_base, vals,    \
-                          xin, p.chunk_row, p.nchunks, e, st);                                               \
+    if (s->prof_start)                                                                                      \
+      hipExtLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream,         \
+                            s->prof_start, s->prof_stop, 0, s->rows, s->cols, s->cols16, s->seg_base, vals,  \
+                            xin, p.chunk_row, p.nchunks, e, st);                                             \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows,   \
+                         s->cols, s->cols16, s->seg_base, vals, xin, p.chunk_row, p.nchunks, e, st);        \
     return;                                                                                                 \
   }
   EHIP_SPMV_CONFIGS(EHIP_EPI_CASE)
@@ -1073,15 +1077,24 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
     const T *sv;
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
     const int g = sell_grid(s);
-    // (a sampled launch carries its timing events: s->prof_start/stop)
-    if (s->sell_c16)
-      hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream,
-                            s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
-                            s->sell_off, sv, xin, s->nslice, e, st);
-    else
-      hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), 0, s->stream,
-                            s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
-                            s->sell_off, sv, xin, s->nslice, e, st);
+    // a sampled launch carries its timing events (s->prof_start/stop); other
+    // launches -- including those captured into graphs -- are plain launches
+    if (s->prof_start) {
+      if (s->sell_c16)
+        hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream,
+                              s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
+                              s->sell_off, sv, xin, s->nslice, e, st);
+      else
+        hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), 0, s->stream,
+                              s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
+                              s->sell_off, sv, xin, s->nslice, e, st);
+    } else if (s->sell_c16) {
+      hipLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream, s->rows,
+                         s->n, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
+    } else {
+      hipLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), 0, s->stream, s->rows,
+                         s->n, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
+    }
     return;
   }
   if (s->c16)
@@ -1322,6 +1335,71 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
 // Default residual drop between two replacements of the reliable-update CG.
 constexpr double kRuDelta = 1e-2;
 
+// Enqueues `count` FP32 fast-CG iterations (fused SpMV step, vector step,
+// finalize) of the mixed form.
+static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, bool prof) {
+  const int n = s->n, g = vec_grid(n), pg = max_grid();
+  KState32 *st32 = s->st32;
+  for (int k = 0; k < count; ++k) {
+    EpiCgFast1<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], 0.f};
+    if (prof) prof_begin(s);
+    launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
+    if (prof) prof_end(s);
+    hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32,
+                       (const float *)s->partials32, epi_grid<float>(s, s->plan32), s->partials32 + kVecPartOff,
+                       (double *)nullptr);
+    fin<float>(s, FinCgFast2<float>{st32, s->partials32 + kVecPartOff, g, nullptr}, g, st32);
+  }
+}
+
+// What a captured block bakes in: buffers, SpMV format and schedule.
+static uint64_t cg32_key(const ehip_system *s) {
+  const uint64_t parts[] = {(uint64_t)(uintptr_t)s->work32, (uint64_t)(uintptr_t)s->vals32,
+                            (uint64_t)(uintptr_t)s->sell32, (uint64_t)(uintptr_t)s->partials32,
+                            (uint64_t)(uintptr_t)s->st32, (uint64_t)(uintptr_t)s->plan32.chunk_row,
+                            (uint64_t)s->fmt32, (uint64_t)s->plan32.block, (uint64_t)s->plan32.nb,
+                            (uint64_t)s->plan32.grid, (uint64_t)s->n};
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t x : parts) {
+    h ^= x;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// Enqueues `count` iterations as replays of captured blocks (one graph launch
+// per power-of-two piece instead of three kernel launches per iteration).
+// The blocks hold no memset; every iteration kernel reads its scalars from
+// the device state, so a replay is the same work as direct launches.
+static int cg32_iterations_graph(ehip_system *s, const Vecs<float> &v, int count) {
+  const uint64_t key = cg32_key(s);
+  if (key != s->cg32_key) {
+    for (hipGraphExec_t &g : s->cg32_graph)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    s->cg32_key = key;
+  }
+  for (int b = 5; b >= 0; --b) {
+    const int piece = 1 << b;
+    while (count >= piece) {
+      if (!s->cg32_graph[b]) {
+        hipGraph_t graph = nullptr;
+        EHIP_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        cg32_iterations(s, v, piece, false);
+        EHIP_HIP_TRY(hipStreamEndCapture(s->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&s->cg32_graph[b], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        EHIP_HIP_TRY(e);
+      }
+      EHIP_HIP_TRY(hipGraphLaunch(s->cg32_graph[b], s->stream));
+      count -= piece;
+    }
+  }
+  return EHIP_OK;
+}
+
 // Mixed-precision CG with reliable updates: one FP32 CG sequence (FP32
 // values, vectors and accumulation, fused one-SpMV steps) whose residual is
 // replaced by the FP64 true residual b - A x64 each time the FP32 recurrence
@@ -1358,18 +1436,15 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   double lres_prev = std::log(std::max(s->h_st->res, 1e-300));
   double rate = 0.0;  // log residual change per iteration (< 0)
   int batch = std::min(c.check, 8);
+  // the SELL copy of the FP32 values is refreshed here, outside any capture
+  if (s->fmt32 == 1 && s->sell_ok && !sell_values32(s)) return EHIP_ENOMEM;
   while (!s->h_st->done) {
-    for (int k = 0; k < batch; ++k) {
-      EpiCgFast1<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], 0.f};
-      if (c.prof) prof_begin(s);
-      launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
-      if (c.prof) prof_end(s);
-      hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32,
-                         (const float *)s->partials32, epi_grid<float>(s, s->plan32), s->partials32 + kVecPartOff,
-                         (double *)nullptr);
-      fin<float>(s, FinCgFast2<float>{st32, s->partials32 + kVecPartOff, g, nullptr}, g, st32);
-      c.spmv++;
+    if (c.prof) {
+      cg32_iterations(s, v, batch, true);  // sampled launches need their own events
+    } else if ((rc = cg32_iterations_graph(s, v, batch))) {
+      return rc;
     }
+    c.spmv += batch;
     if ((rc = read_state(s, st32, s->h_st32))) return rc;
     const KState32 h = *s->h_st32;
     if (h.iter > it_prev && h.res > 0.f) {

@@ -501,6 +501,8 @@ int ehip_system_free(ehip_system *s) {
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipGraphExec_t g : s->cg32_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   if (s->h_st) (void)hipHostFree(s->h_st);
   if (s->h_st32) (void)hipHostFree(s->h_st32);
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -388,3 +388,22 @@ def test_fortran_host_program(ehip):
     r = subprocess.run([exe, "48"], capture_output=True, text=True, timeout=60)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "matvec bit-exact: T" in r.stdout and "ehip_demo: ok" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["cg_poisson32", "cg_lognormal8"])
+def test_mixed_graph_replay_equals_direct_launches(ehip, case):
+    """The mixed CG enqueues its FP32 iterations as replays of captured
+    blocks; with launch sampling on it launches them one by one.  Both give
+    the same bits, on the first solve (graphs captured) and on repeated
+    solves (graphs replayed from the cache)."""
+    w, pat, vals, rhs, rows = assembled_case(CASES[case])
+    s = System.from_csr(pat.rows, pat.cols, pat.diag)
+    out = []
+    for prof in (1, 0, 0, 1):
+        s.set_values(vals, rhs)
+        x, rep = s.solve(w["kw"], precision=_lib.MIXED_IR, mode=_lib.FAST, profile_every=prof)
+        assert rep["info"] == 1
+        out.append((x, rep["iters"]))
+    for x, it in out[1:]:
+        assert it == out[0][1] and np.array_equal(x, out[0][0])
+    s.free()
