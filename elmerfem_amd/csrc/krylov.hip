@@ -25,10 +25,13 @@
 #include <vector>
 
 #include "ehip_internal.h"
+#include "launch.h"
 #include "linalg_kernels.h"
 #include "spmv_core.h"
 
 namespace ehip {
+
+thread_local GraphRec *tl_graph_rec = nullptr;
 
 // ---------------------------------------------------------------- scaling
 
@@ -329,44 +332,103 @@ struct EpiCgFast1 {
   }
 };
 
-// step 2: alpha = rho / p.q, summed from step 1's partials by every
-// workgroup in the same fixed order (so all agree bit for bit; workgroup 0
-// records it) -- this replaces a one-workgroup finalize launch;
-// x += alpha p ; r -= alpha q ; z = r/dg ; partials r.z, r.r
+// One tile of the CG vector step: x, p, r, q and the Jacobi diagonal of
+// kVI elements per lane.
 template <typename T>
-__global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
-                                                        int pq_g, T *partials, double *hist) {
-  if (solve_done(st)) return;
-  T pq[1];
-  block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
-  const T alpha = st->rho / pq[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->pq = pq[0];
-    st->alpha = alpha;
-  }
-  T acc[2] = {(T)0, (T)0};
-  for (int base = blockIdx.x * kVecBlock * kVI; base < n; base += gridDim.x * kVecBlock * kVI) {
-    T xk[kVI], pk[kVI], rk[kVI], qk[kVI], dk[kVI];
+struct CgTile {
+  T x[kVI], p[kVI], r[kVI], q[kVI], d[kVI];
+  __device__ __forceinline__ void load(const Vecs<T> &v, int n, int base) {
 #pragma unroll
     for (int k = 0; k < kVI; ++k) {
       const int i = base + k * kVecBlock + threadIdx.x;
-      if (i < n) { xk[k] = v.x[i]; pk[k] = v.p[i]; rk[k] = v.r[i]; qk[k] = v.q[i]; dk[k] = v.dg[i]; }
+      if (i < n) { x[k] = v.x[i]; p[k] = v.p[i]; r[k] = v.r[i]; q[k] = v.q[i]; d[k] = v.dg[i]; }
     }
+  }
+  // x += alpha p ; r -= alpha q ; z = r/dg ; acc += r.z (and r.r when NA = 2)
+  template <int NA>
+  __device__ __forceinline__ void step(const Vecs<T> &v, int n, int base, T alpha, T (&acc)[NA]) const {
 #pragma unroll
     for (int k = 0; k < kVI; ++k) {
       const int i = base + k * kVecBlock + threadIdx.x;
       if (i < n) {
-        v.x[i] = xk[k] + alpha * pk[k];
-        const T ri = rk[k] - alpha * qk[k];
+        v.x[i] = x[k] + alpha * p[k];
+        const T ri = r[k] - alpha * q[k];
         v.r[i] = ri;
-        const T zi = ri / dk[k];
+        const T zi = ri / d[k];
         v.z[i] = zi;
         acc[0] += ri * zi;
-        acc[1] += ri * ri;
+        if constexpr (NA > 1) acc[1] += ri * ri;
       }
     }
   }
-  block_publish<T, 2>(acc, partials, gridDim.x);
+};
+
+// alpha = rho / p.q, summed from the SpMV step's partials by every workgroup
+// in the same fixed order (so all agree bit for bit; workgroup 0 records it)
+// -- this replaces a one-workgroup finalize launch.  The first tile's loads
+// do not depend on alpha and go out before the reduction, so the two
+// latencies overlap.  Then x += alpha p ; r -= alpha q ; z = r/dg with the
+// partials of r.z (and r.r when NA = 2).
+template <typename T, int NA, bool Fold = false>
+__device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<T> *st, const T *pq_part, int pq_g,
+                                               T *partials) {
+  const int stride = gridDim.x * kVecBlock * kVI;
+  int base = blockIdx.x * kVecBlock * kVI;
+  CgTile<T> cur, nxt;
+  cur.load(v, n, base);
+  T pq[1];
+  block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
+  const T alpha = (Fold ? st->rho_next : st->rho) / pq[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->pq = pq[0];
+    st->alpha = alpha;
+    if constexpr (Fold) {
+      st->rho = st->rho_next;
+      st->fresh = 0;
+      st->iter = st->iter + 1;
+    }
+  }
+  T acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = (T)0;
+  while (base < n) {
+    const int nb = base + stride;
+    if (nb < n) nxt.load(v, n, nb);
+    cur.template step<NA>(v, n, base, alpha, acc);
+    cur = nxt;
+    base = nb;
+  }
+  block_publish<T, NA>(acc, partials, gridDim.x);
+}
+
+// step 2 of the one-SpMV CG: partials r.z, r.r
+template <typename T, bool Fold = false>
+__global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
+                                                        int pq_g, T *partials) {
+  if (solve_done(st)) return;
+  cg_vector_step<T, 2, Fold>(n, v, st, pq_part, pq_g, partials);
+}
+
+// The stopping tests on the recurrence residual after a CG vector step, in
+// the order HUTI applies them: converged, diverged, rho = 0, out of
+// iterations, and (mixed form) a reliable update due.
+struct CgCheck {
+  int done, info, rec_conv;
+};
+template <typename T>
+__device__ __forceinline__ CgCheck cg_check(T res, T rz, int it, const KStateT<T> *st) {
+  if (res < st->tol) return {1, 0, 1};
+  if (!(res == res) || (st->divtol > (T)0 && res > st->divtol)) return {1, 3, 0};
+  if (rz == (T)0) return {1, 20, 0};
+  if (it >= st->maxit) return {1, 2, 0};
+  if (res < st->ru) return {1, 0, 2};  // reliable update due
+  return {0, 0, 0};
+}
+template <typename T>
+__device__ __forceinline__ void cg_apply(const CgCheck &c, KStateT<T> *st) {
+  if (c.rec_conv) st->rec_conv = c.rec_conv;
+  if (c.info) st->info = c.info;
+  if (c.done) st->done = 1;
 }
 
 template <typename T>
@@ -388,22 +450,86 @@ struct FinCgFast2 {
     const int it = st->iter + 1;  // iterations done
     st->iter = it;
     if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
-    if (res < st->tol) {
-      st->rec_conv = 1;
-      st->done = 1;
-    } else if (!(res == res) || (st->divtol > (T)0 && res > st->divtol)) {
-      st->info = 3;
-      st->done = 1;
-    } else if (tot[0] == (T)0) {
-      st->info = 20;
-      st->done = 1;
-    } else if (it >= st->maxit) {
-      st->info = 2;
-      st->done = 1;
-    } else if (res < st->ru) {
-      st->rec_conv = 2;  // reliable update due
-      st->done = 1;
+    cg_apply(cg_check(res, tot[0], it, st), st);
+  }
+};
+
+// ---- folded CG steps (FP32 inner sequence of the mixed form): two launches
+// per iteration instead of three.  The SpMV step sums the previous vector
+// step's partials itself (every workgroup, same order, so all agree bit for
+// bit) and runs the stopping tests there; workgroup 0 records the scalars.
+// No launch reads a field that its own workgroup 0 writes:
+//   SpMV step   reads rho, iter, fresh (and beta when fresh);
+//               writes rho_new, rho_next, beta, rr, res, flags
+//   vector step reads rho_next; writes rho = rho_next, pq, alpha, iter, fresh
+// A block of iterations ends with FinCgFold, which leaves rho and beta
+// current (fresh = 1), as a residual replacement does.
+// the vector step's grid is at most kFoldPer x 256 (fused_grid), so the
+// SpMV step's prologue sums its partials with kFoldPer loads per thread
+constexpr int kFoldPer = 4;
+static_assert(kFusedCap <= kFoldPer * 256, "vector-step partials must fit the SpMV prologue's loads");
+template <typename T>
+struct EpiCgFold {
+  static constexpr int NV = 1;
+  static constexpr bool kStops = true;
+  EHIP_EPI_MEMBERS(T)
+  const T *rz_part;  // the vector step's partials: r.z at [0, rz_g), r.r at [rz_g, 2 rz_g)
+  int rz_g;
+  T beta;
+  bool stop;
+  __device__ void init() {
+    stop = false;
+    if (st->fresh) {
+      beta = st->beta;
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->rho_next = st->rho;
+      return;
     }
+    T tot[2];
+    block_reduce_few<T, 2, kFoldPer>(rz_part, rz_g, rz_g, tot);
+    const T rho_old = st->rho;
+    const T res = sqrt(tot[1]) / st->rhsnorm;
+    const CgCheck c = cg_check(res, tot[0], st->iter, st);
+    beta = tot[0] / rho_old;
+    stop = c.done != 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->rho_new = rho_old;
+      st->rho_next = tot[0];
+      st->beta = beta;
+      st->rr = tot[1];
+      st->res = res;
+      cg_apply(c, st);
+    }
+  }
+  struct Pre { T q, z, p; };
+  __device__ Pre load(int i) const { return Pre{v.q[i], v.z[i], v.p[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+    const T qi = s + beta * pr.q;
+    const T pi = pr.z + beta * pr.p;
+    v.q[i] = qi;
+    v.p[i] = pi;
+    acc[0] += pi * qi;
+  }
+};
+
+// end of a block of folded iterations: the SpMV step's summation and tests
+// for the last vector step, leaving rho and beta current
+template <typename T>
+struct FinCgFold {
+  static constexpr int NV = 2;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  __device__ void finish(T (&tot)[2]) {
+    if (threadIdx.x != 0) return;
+    const T rho_old = st->rho;
+    st->rho_new = rho_old;
+    st->rho = tot[0];
+    st->beta = tot[0] / rho_old;
+    st->rr = tot[1];
+    const T res = sqrt(tot[1]) / st->rhsnorm;
+    st->res = res;
+    cg_apply(cg_check(res, tot[0], st->iter, st), st);
+    st->fresh = 1;
   }
 };
 
@@ -477,41 +603,13 @@ struct EpiCgQ {
   }
 };
 
-// alpha = rho / p.q from Kc's partials, summed by every workgroup in the
-// same order (workgroup 0 records it), in place of a finalize launch
+// HUTI order: alpha = rho / p.q from Kc's partials, then the vector step
+// with the partials of rho_new = r.z
 template <typename T>
 __global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
                                                      int pq_g, T *partials) {
   if (solve_done(st)) return;
-  T pq[1];
-  block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
-  const T alpha = st->rho / pq[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->pq = pq[0];
-    st->alpha = alpha;
-  }
-  T acc[1] = {(T)0};
-  for (int base = blockIdx.x * kVecBlock * kVI; base < n; base += gridDim.x * kVecBlock * kVI) {
-    T xk[kVI], pk[kVI], rk[kVI], qk[kVI], dk[kVI];
-#pragma unroll
-    for (int k = 0; k < kVI; ++k) {
-      const int i = base + k * kVecBlock + threadIdx.x;
-      if (i < n) { xk[k] = v.x[i]; pk[k] = v.p[i]; rk[k] = v.r[i]; qk[k] = v.q[i]; dk[k] = v.dg[i]; }
-    }
-#pragma unroll
-    for (int k = 0; k < kVI; ++k) {
-      const int i = base + k * kVecBlock + threadIdx.x;
-      if (i < n) {
-        v.x[i] = xk[k] + alpha * pk[k];
-        const T ri = rk[k] - alpha * qk[k];
-        v.r[i] = ri;
-        const T zi = ri / dk[k];
-        v.z[i] = zi;
-        acc[0] += ri * zi;
-      }
-    }
-  }
-  block_publish<T, 1>(acc, partials, gridDim.x);
+  cg_vector_step<T, 1>(n, v, st, pq_part, pq_g, partials);
 }
 
 template <typename T>
@@ -848,6 +946,7 @@ __global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int ite
   st->res_true = (T)-1;
   st->pad0 = 0;
   st->pad1 = 0;
+  st->fresh = 1;
 }
 
 // ---- mixed precision
@@ -982,6 +1081,7 @@ struct EpiRU {
     } else {
       st32->ru = (float)(res * delta);
       st32->rec_conv = 0;
+      st32->fresh = 1;
       st32->done = 0;
     }
   }
@@ -1039,8 +1139,8 @@ static void launch_epi_cs(ehip_system *s, const SpmvPlan &p, const T *vals, cons
                             s->prof_start, s->prof_stop, 0, s->rows, s->cols, s->cols16, s->seg_base, vals,  \
                             xin, p.chunk_row, p.nchunks, e, st);                                             \
     else                                                                                                    \
-      hipLaunchKernelGGL((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), 0, s->stream, s->rows,   \
-                         s->cols, s->cols16, s->seg_base, vals, xin, p.chunk_row, p.nchunks, e, st);        \
+      launch_k((k_spmv_epi<T, BS, NB, CS, Epi>), dim3(p.grid), dim3(BS), s->stream, s->rows, s->cols,       \
+               s->cols16, s->seg_base, vals, xin, p.chunk_row, p.nchunks, e, st);                           \
     return;                                                                                                 \
   }
   EHIP_SPMV_CONFIGS(EHIP_EPI_CASE)
@@ -1089,11 +1189,11 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
                               s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
                               s->sell_off, sv, xin, s->nslice, e, st);
     } else if (s->sell_c16) {
-      hipLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream, s->rows,
-                         s->n, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
+      launch_k((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n,
+               s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
     } else {
-      hipLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), 0, s->stream, s->rows,
-                         s->n, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
+      launch_k((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n,
+               s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
     }
     return;
   }
@@ -1108,14 +1208,13 @@ template <typename T, class Epi>
 static void launch_epi_fin(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
                            const KStateT<T> *st) {
   launch_epi<T, Epi>(s, p, vals, xin, e, st);
-  if constexpr (Epi::NV > 0)
-    hipLaunchKernelGGL((k_fin<T, Epi>), dim3(1), dim3(256), 0, s->stream, e, epi_grid<T>(s, p), st);
+  if constexpr (Epi::NV > 0) launch_k((k_fin<T, Epi>), dim3(1), dim3(256), s->stream, e, epi_grid<T>(s, p), st);
 }
 
 // vector kernel's finalize
 template <typename T, class Fin>
 static void fin(ehip_system *s, const Fin &f, int g, const KStateT<T> *st) {
-  hipLaunchKernelGGL((k_fin<T, Fin>), dim3(1), dim3(256), 0, s->stream, f, g, st);
+  launch_k((k_fin<T, Fin>), dim3(1), dim3(256), s->stream, f, g, st);
 }
 
 // *out = sum of x_i^2 (two launches, no in-kernel fence)
@@ -1165,6 +1264,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
                   KStateT<T> *st, KStateT<T> *hst, T *part, bool init_residual) {
   const int n = s->n;
   const int g = vec_grid(n);
+  const int gf = fused_grid(n);
   const int pg = max_grid();
   int rc;
   const int iter0 = (c.mode == EHIP_FAITHFUL || c.method == EHIP_BICGSTAB) ? 1 : 0;
@@ -1186,9 +1286,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.p, e1, st);
           if (c.prof) prof_end(s);
-          hipLaunchKernelGGL(k_cg_da<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
+          hipLaunchKernelGGL(k_cg_da<T>, dim3(gf), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
                              epi_grid<T>(s, plan), part + kVecPartOff);
-          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, part + kVecPartOff, g, (T)0};
+          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, part + kVecPartOff, gf, (T)0};
           launch_epi_fin<T>(s, plan, vals, v.x, e2, st);
           c.spmv += 2;
         }
@@ -1203,9 +1303,9 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.z, e1, st);
           if (c.prof) prof_end(s);
-          hipLaunchKernelGGL(k_cg_fast2<T>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
-                             epi_grid<T>(s, plan), part + kVecPartOff, c.hist);
-          fin<T>(s, FinCgFast2<T>{st, part + kVecPartOff, g, c.hist}, g, st);
+          hipLaunchKernelGGL((k_cg_fast2<T, false>), dim3(gf), dim3(kVecBlock), 0, s->stream, n, v, st,
+                             (const T *)part, epi_grid<T>(s, plan), part + kVecPartOff);
+          fin<T>(s, FinCgFast2<T>{st, part + kVecPartOff, gf, c.hist}, gf, st);
           c.spmv++;
         }
         if ((rc = read_state(s, st, hst))) return rc;
@@ -1335,21 +1435,21 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
 // Default residual drop between two replacements of the reliable-update CG.
 constexpr double kRuDelta = 1e-2;
 
-// Enqueues `count` FP32 fast-CG iterations (fused SpMV step, vector step,
-// finalize) of the mixed form.
+// Enqueues `count` folded FP32 CG iterations (SpMV step, vector step) of the
+// mixed form and the finalize that closes the block.
 static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, bool prof) {
-  const int n = s->n, g = vec_grid(n), pg = max_grid();
+  const int n = s->n, gf = fused_grid(n), pg = max_grid();
   KState32 *st32 = s->st32;
+  float *vpart = s->partials32 + kVecPartOff;
   for (int k = 0; k < count; ++k) {
-    EpiCgFast1<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], 0.f};
+    EpiCgFold<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], vpart, gf, 0.f, false};
     if (prof) prof_begin(s);
     launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
     if (prof) prof_end(s);
-    hipLaunchKernelGGL(k_cg_fast2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st32,
-                       (const float *)s->partials32, epi_grid<float>(s, s->plan32), s->partials32 + kVecPartOff,
-                       (double *)nullptr);
-    fin<float>(s, FinCgFast2<float>{st32, s->partials32 + kVecPartOff, g, nullptr}, g, st32);
+    launch_k((k_cg_fast2<float, true>), dim3(gf), dim3(kVecBlock), s->stream, n, v, st32,
+             (const float *)s->partials32, epi_grid<float>(s, s->plan32), vpart);
   }
+  fin<float>(s, FinCgFold<float>{st32, vpart, gf}, gf, st32);
 }
 
 // What a captured block bakes in: buffers, SpMV format and schedule.
@@ -1367,10 +1467,11 @@ static uint64_t cg32_key(const ehip_system *s) {
   return h;
 }
 
-// Enqueues `count` iterations as replays of captured blocks (one graph launch
-// per power-of-two piece instead of three kernel launches per iteration).
-// The blocks hold no memset; every iteration kernel reads its scalars from
-// the device state, so a replay is the same work as direct launches.
+// Enqueues `count` iterations as replays of recorded blocks (one graph launch
+// per power-of-two piece instead of two kernel launches per iteration).  The
+// blocks are built node by node (launch_k under a recorder), not captured,
+// and hold kernels only; every iteration kernel reads its scalars from the
+// device state, so a replay is the same work as direct launches.
 static int cg32_iterations_graph(ehip_system *s, const Vecs<float> &v, int count) {
   const uint64_t key = cg32_key(s);
   if (key != s->cg32_key) {
@@ -1385,12 +1486,14 @@ static int cg32_iterations_graph(ehip_system *s, const Vecs<float> &v, int count
     const int piece = 1 << b;
     while (count >= piece) {
       if (!s->cg32_graph[b]) {
-        hipGraph_t graph = nullptr;
-        EHIP_HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+        GraphRec rec;
+        EHIP_HIP_TRY(hipGraphCreate(&rec.graph, 0));
+        tl_graph_rec = &rec;
         cg32_iterations(s, v, piece, false);
-        EHIP_HIP_TRY(hipStreamEndCapture(s->stream, &graph));
-        const hipError_t e = hipGraphInstantiate(&s->cg32_graph[b], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
+        tl_graph_rec = nullptr;
+        hipError_t e = rec.err;
+        if (e == hipSuccess) e = hipGraphInstantiate(&s->cg32_graph[b], rec.graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(rec.graph);
         EHIP_HIP_TRY(e);
       }
       EHIP_HIP_TRY(hipGraphLaunch(s->cg32_graph[b], s->stream));
