@@ -847,8 +847,12 @@ struct ehip_batch {
 // batched -- one iteration of a large system fills the GPU on its own -- but
 // solved by ehip_solve on one of `workers` host threads, each on the
 // system's own stream, so large solves overlap one another and the batched
-// small ones on the device.  A request waits (EHIP_ECAP) until a worker is
-// free and the HBM budget allows its workspace.
+// small ones on the device.  Admitted systems queue in the lane (up to
+// kLaneDepth per worker) so that a worker starts its next solve the moment
+// it finishes one, not at the caller's next server step; a request waits
+// (EHIP_ECAP) when the lane queue is full or the HBM budget would not hold
+// its workspace.
+constexpr int kLaneDepth = 4;
 struct Lane {
   std::vector<std::thread> th;
   std::mutex mu;
@@ -1293,8 +1297,8 @@ extern "C" int ehip_server_submit(ehip_batch *b, ehip_system *s, int *slot_out) 
     }
     const int64_t need = lane_bytes(s, b->mixed != 0);
     std::lock_guard<std::mutex> lk(L->mu);
-    if (L->running >= L->workers) {
-      set_error("server: every concurrent-lane worker is busy");
+    if (L->running >= L->workers * kLaneDepth) {
+      set_error("server: the concurrent lane's queue is full");
       return EHIP_ECAP;
     }
     if (budget > 0 && b->bytes + L->bytes + need > budget) {

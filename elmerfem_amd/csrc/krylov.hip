@@ -1234,6 +1234,7 @@ static int read_state(ehip_system *s, KStateT<T> *dst, KStateT<T> *hst) {
 
 struct SolveCtx {
   int method, precond, maxit, check, mode;
+  bool check_set;  // check_every given by the caller (else the loops choose)
   double tol, divtol;
   double *hist;  // device, may be null
   int spmv;
@@ -1434,6 +1435,9 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
 
 // Default residual drop between two replacements of the reliable-update CG.
 constexpr double kRuDelta = 1e-2;
+// Longest block of FP32 iterations between two host checks (graph pieces of
+// 32 + 16 + ... iterations).
+constexpr int kRuMaxBlock = 63;
 
 // Enqueues `count` folded FP32 CG iterations (SpMV step, vector step) of the
 // mixed form and the finalize that closes the block.
@@ -1534,11 +1538,17 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   int nupd = 0;
   if ((rc = read_state(s, st, s->h_st))) return rc;
   // iterations per check: predicted from the observed convergence rate so
-  // that few no-op launches follow the step that asks for a replacement
+  // that few no-op launches follow the step that asks for a replacement.
+  // A check costs a host round trip (tens of microseconds with the device
+  // idle) and a no-op iteration past the replacement point a few
+  // microseconds, so the prediction aims a little long and, unless the caller
+  // fixed check_every, blocks run up to kRuMaxBlock iterations.
+  const int cap = c.check_set ? c.check : std::max(c.check, kRuMaxBlock);
+  auto predict = [](double iters) { return (int)std::ceil(1.1 * iters) + 2; };
   int it_prev = 0;
   double lres_prev = std::log(std::max(s->h_st->res, 1e-300));
   double rate = 0.0;  // log residual change per iteration (< 0)
-  int batch = std::min(c.check, 8);
+  int batch = std::min(cap, 8);
   // the SELL copy of the FP32 values is refreshed here, outside any capture
   if (s->fmt32 == 1 && s->sell_ok && !sell_values32(s)) return EHIP_ENOMEM;
   while (!s->h_st->done) {
@@ -1559,10 +1569,9 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
     }
     if (!h.done) {
       // predicted iterations until the replacement threshold
-      int want = c.check;
-      if (rate < 0.0 && h.ru > 0.f && h.res > h.ru)
-        want = (int)std::ceil(std::log((double)h.ru / (double)h.res) / rate) + 1;
-      batch = std::max(2, std::min(c.check, want));
+      int want = cap;
+      if (rate < 0.0 && h.ru > 0.f && h.res > h.ru) want = predict(std::log((double)h.ru / (double)h.res) / rate);
+      batch = std::max(2, std::min(cap, want));
       continue;
     }
     // FP32 stopped: replacement due, recurrence converged, or failure
@@ -1580,7 +1589,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
     }
     it_prev = h.iter;
     lres_prev = std::log(std::max(s->h_st->res, 1e-300));
-    batch = std::max(2, std::min(c.check, rate < 0.0 ? (int)std::ceil(std::log(delta) / rate) + 1 : c.check));
+    batch = std::max(2, std::min(cap, rate < 0.0 ? predict(std::log(delta) / rate) : cap));
   }
   if ((rc = read_state(s, st32, s->h_st32))) return rc;
   *inner_total = s->h_st32->iter;
@@ -1678,6 +1687,7 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   c.tol = p->tol;
   c.divtol = p->divtol;
   c.check = p->check_every > 0 ? p->check_every : (n > 200000 ? 25 : 10);
+  c.check_set = p->check_every > 0;
   c.hist = dhist;
   c.prof = p->profile_every > 0;
   s->prof_every = p->profile_every;
