@@ -95,6 +95,13 @@ struct ehip_system {
   int *sell_c32 = nullptr;            // sell_nnz column indices, or
   unsigned short *sell_c16 = nullptr; // 16-bit offsets from sell_base[slice]
   int *sell_base = nullptr;           // nslice
+  // FP64 values use the pair-packed layout (spmv_sell2_body) when the 16-bit
+  // offsets fit: slice s at [sell2_off[s], sell2_off[s+1]), with its own
+  // offsets array; sell64 then holds sell2_nnz values in that layout
+  int64_t sell2_nnz = 0;
+  int *sell2_off = nullptr;
+  unsigned short *sell2_c16 = nullptr;
+  bool sell2_ok = false;
   double *sell64 = nullptr;
   float *sell32 = nullptr;
   bool sell_ok = false;
@@ -164,6 +171,7 @@ const float *sell_values32(ehip_system *s);
 // SELL-64 launch shape: nonzeros per lane in flight, and the grid (slice
 // groups of four per workgroup step, at most 2048 workgroups, a multiple of 8)
 constexpr int kSellU = 8;
+constexpr int kSell2U = 4;  // pairs per lane in flight (pair-packed FP64)
 inline int sell_grid(const ehip_system *s) {
   int g = (s->nslice + 3) / 4;
   if (g > 2048) g = 2048;

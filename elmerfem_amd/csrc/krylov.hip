@@ -1159,6 +1159,16 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ ro
   spmv_sell_body<T, T, BS, U>(rows, n, sc, off, vals, xin, nslice, epi);
 }
 
+template <int BS, int U, class Epi>
+__global__ __launch_bounds__(BS) void k_spmv_epi_sell2(const int *__restrict__ rows, int n,
+                                                       const unsigned short *__restrict__ c16,
+                                                       const int *__restrict__ sbase, const int *__restrict__ off2,
+                                                       const double *__restrict__ vals, const double *__restrict__ xin,
+                                                       int nslice, Epi epi, const KState *st) {
+  if (st && solve_done(st)) return;
+  spmv_sell2_body<double, BS, U>(rows, n, c16, sbase, off2, vals, xin, nslice, epi);
+}
+
 // Workgroups (and so published partials) of an epilogue SpMV launch
 template <typename T>
 static int epi_grid(const ehip_system *s, const SpmvPlan &p) {
@@ -1177,8 +1187,20 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
     const T *sv;
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
     const int g = sell_grid(s);
+    if constexpr (!F32) {
+      if (s->sell2_ok) {  // pair-packed FP64 layout
+        if (s->prof_start)
+          hipExtLaunchKernelGGL((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), 0, s->stream,
+                                s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell2_c16, s->sell_base,
+                                s->sell2_off, sv, xin, s->nslice, e, st);
+        else
+          launch_k((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n,
+                   s->sell2_c16, s->sell_base, s->sell2_off, sv, xin, s->nslice, e, st);
+        return;
+      }
+    }
     // a sampled launch carries its timing events (s->prof_start/stop); other
-    // launches -- including those captured into graphs -- are plain launches
+    // launches -- including those recorded into graphs -- are plain launches
     if (s->prof_start) {
       if (s->sell_c16)
         hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream,

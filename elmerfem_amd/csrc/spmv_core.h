@@ -307,4 +307,70 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
   if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
 }
 
+// Pair-packed SELL-64 for 8-byte values (the FP64 copy): inside slice s the
+// k-th nonzero of row 64 s + l sits at off2[s] + 128 (k/2) + 2 l + (k%2),
+// each slice's width rounded up to even.  A lane loads its row's next two
+// values as one 16-byte load and their two 16-bit column offsets as one
+// 4-byte load -- half the load instructions of the unpacked layout, which
+// for FP64 are the limit -- and still adds k = 0, 1, ... in order.
+typedef double EhipDouble2 __attribute__((ext_vector_type(2)));
+template <typename A, int BS, int U, class Epi>
+__device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, int n,
+                                                const unsigned short *__restrict__ c16,
+                                                const int *__restrict__ base, const int *__restrict__ off2,
+                                                const double *__restrict__ vals, const A *__restrict__ x, int nslice,
+                                                Epi &epi) {
+  constexpr int WPB = BS / 64;
+  constexpr int NV = Epi::NV;
+  constexpr int NA = NV > 0 ? NV : 1;
+  A acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = (A)0;
+  epi.init();
+  if constexpr (EpiStops<Epi>::value) {
+    if (epi.stop) return;
+  }
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const XcdWalk w((nslice + WPB - 1) / WPB);
+  for (int gidx = w.first; gidx < w.last; gidx += w.step) {
+    const int sl = gidx * WPB + wv;  // wave-uniform
+    if (sl >= nslice) continue;
+    const int r = sl * 64 + lane;
+    const int rl = min(r, n - 1);
+    const typename Epi::Pre pre = epi.load(rl);
+    const int len = rows[rl + 1] - rows[rl];
+    const int p0 = off2[sl];
+    const int np = (off2[sl + 1] - p0) >> 7;  // pairs per row in this slice
+    const int b = base[sl];
+    A sum = (A)0;
+    for (int k0 = 0; k0 < np; k0 += U) {
+      EhipDouble2 vv[U];
+      unsigned int cc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = p0 + min(k0 + u, np - 1) * 128 + 2 * lane;  // clamped into the slice
+        vv[u] = ntload(reinterpret_cast<const EhipDouble2 *>(vals + idx));
+        cc[u] = ntload(reinterpret_cast<const unsigned int *>(c16 + idx));
+      }
+      A xv[U][2];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        xv[u][0] = x[b + (int)(cc[u] & 0xffffu)];
+        xv[u][1] = x[b + (int)(cc[u] >> 16)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma clang fp contract(off)
+          const A t = sum + (A)vv[u][h] * xv[u][h];
+          sum = (2 * (k0 + u) + h < len) ? t : sum;
+        }
+    }
+    if (r < n) epi.row(r, sum, pre, acc);
+  }
+  if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
+}
+
 }  // namespace ehip

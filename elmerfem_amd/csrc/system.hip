@@ -137,6 +137,16 @@ __global__ __launch_bounds__(BS) void k_spmv_sell(const int *__restrict__ rows, 
   spmv_sell_body<V, A, BS, U>(rows, n, sc, off, vals, x, nslice, epi);
 }
 
+template <typename A, int BS, int U>
+__global__ __launch_bounds__(BS) void k_spmv_sell2(const int *__restrict__ rows, int n,
+                                                   const unsigned short *__restrict__ c16,
+                                                   const int *__restrict__ sbase, const int *__restrict__ off2,
+                                                   const double *__restrict__ vals, const A *__restrict__ x,
+                                                   int nslice, A *y) {
+  EpiStore<A> epi{y};
+  spmv_sell2_body<A, BS, U>(rows, n, c16, sbase, off2, vals, x, nslice, epi);
+}
+
 template <typename V, typename A>
 static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A *x, A *y) {
   if (s->nnz == 0) {
@@ -149,6 +159,13 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
     if (!sv) return EHIP_ENOMEM;
     const int g = sell_grid(s);
+    if constexpr (!F32) {
+      if (s->sell2_ok) {
+        hipLaunchKernelGGL((k_spmv_sell2<A, 256, kSell2U>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                           s->sell2_c16, s->sell_base, s->sell2_off, sv, x, s->nslice, y);
+        return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
+      }
+    }
     if (s->sell_c16)
       hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, true>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
                          s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
@@ -302,10 +319,9 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   int rc;
   if ((rc = dmalloc(s, (void **)&s->sell_off, sizeof(int) * off.size()))) return rc;
   if ((rc = dmalloc(s, (void **)&s->sell_pos, sizeof(int) * pos.size()))) return rc;
-  if ((rc = dmalloc(s, (void **)&s->sell64, sizeof(double) * tot))) return rc;
   EHIP_HIP_TRY(hipMemcpy(s->sell_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->sell_pos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
-  EHIP_HIP_TRY(hipMemset(s->sell64, 0, sizeof(double) * tot));
+  int64_t tot64 = tot;  // FP64 values: unpacked, or pair-packed below
   if (c16) {
     std::vector<unsigned short> c((size_t)tot);
     for (int sl = 0; sl < S; ++sl)
@@ -314,10 +330,41 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     if ((rc = dmalloc(s, (void **)&s->sell_base, sizeof(int) * S))) return rc;
     EHIP_HIP_TRY(hipMemcpy(s->sell_c16, c.data(), sizeof(unsigned short) * tot, hipMemcpyHostToDevice));
     EHIP_HIP_TRY(hipMemcpy(s->sell_base, base.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+    // pair-packed copy of the offsets for the FP64 values: width rounded up to
+    // even, entry (k, l) of slice sl at off2[sl] + 128 (k/2) + 2 l + (k%2)
+    std::vector<int> off2((size_t)S + 1, 0);
+    int64_t tot2 = 0;
+    for (int sl = 0; sl < S; ++sl) {
+      const int wd = (off[sl + 1] - off[sl]) / 64;
+      off2[sl] = (int)tot2;
+      tot2 += 64 * (int64_t)((wd + 1) & ~1);
+    }
+    off2[S] = (int)tot2;
+    if (tot2 < (int64_t)INT32_MAX) {
+      std::vector<unsigned short> c2((size_t)tot2);
+      for (int sl = 0; sl < S; ++sl) {
+        const int wd = (off[sl + 1] - off[sl]) / 64;
+        const int wd2 = (off2[sl + 1] - off2[sl]) / 64;
+        for (int l = 0; l < 64; ++l)
+          for (int k = 0; k < wd2; ++k) {
+            const int kk = std::min(k, wd - 1);  // the extra entry repeats the row's padding column
+            c2[(size_t)off2[sl] + (size_t)(k >> 1) * 128 + 2 * l + (k & 1)] = c[(size_t)off[sl] + kk * 64 + l];
+          }
+      }
+      if ((rc = dmalloc(s, (void **)&s->sell2_off, sizeof(int) * off2.size()))) return rc;
+      if ((rc = dmalloc(s, (void **)&s->sell2_c16, sizeof(unsigned short) * tot2))) return rc;
+      EHIP_HIP_TRY(hipMemcpy(s->sell2_off, off2.data(), sizeof(int) * off2.size(), hipMemcpyHostToDevice));
+      EHIP_HIP_TRY(hipMemcpy(s->sell2_c16, c2.data(), sizeof(unsigned short) * tot2, hipMemcpyHostToDevice));
+      s->sell2_nnz = tot2;
+      s->sell2_ok = true;
+      tot64 = tot2;
+    }
   } else {
     if ((rc = dmalloc(s, (void **)&s->sell_c32, sizeof(int) * tot))) return rc;
     EHIP_HIP_TRY(hipMemcpy(s->sell_c32, col.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
   }
+  if ((rc = dmalloc(s, (void **)&s->sell64, sizeof(double) * tot64))) return rc;
+  EHIP_HIP_TRY(hipMemset(s->sell64, 0, sizeof(double) * tot64));
   s->nslice = S;
   s->sell_nnz = tot;
   s->sell_ok = true;
@@ -331,10 +378,25 @@ __global__ void k_sell_fill(int64_t nnz, const int *__restrict__ pos, const TI *
     out[pos[j]] = (TO)in[j];
 }
 
+// FP64 values into the pair-packed layout: one thread per row (the layout
+// position follows from row, slice and k; padding entries stay zero)
+__global__ void k_sell2_fill(int n, const int *__restrict__ rows, const int *__restrict__ off2,
+                             const double *__restrict__ in, double *__restrict__ out) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const int j0 = rows[r], len = rows[r + 1] - j0;
+    const int64_t p0 = off2[r >> 6] + 2 * (r & 63);
+    for (int k = 0; k < len; ++k) out[p0 + (int64_t)(k >> 1) * 128 + (k & 1)] = in[j0 + k];
+  }
+}
+
 const double *sell_values64(ehip_system *s) {
   if (s->sell64_ver != s->vals_ver) {
-    hipLaunchKernelGGL((k_sell_fill<double, double>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
-                       (const double *)s->vals, s->sell64);
+    if (s->sell2_ok)
+      hipLaunchKernelGGL(k_sell2_fill, dim3(std::min(4096, (s->n + 255) / 256)), dim3(256), 0, s->stream, s->n,
+                         s->rows, s->sell2_off, (const double *)s->vals, s->sell64);
+    else
+      hipLaunchKernelGGL((k_sell_fill<double, double>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
+                         (const double *)s->vals, s->sell64);
     s->sell64_ver = s->vals_ver;
   }
   return s->sell64;
@@ -496,7 +558,8 @@ int ehip_system_free(ehip_system *s) {
                   s->hist,       s->slot_start,  s->slot_contrib, s->rhs_start, s->rhs_contrib,
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
                   s->cols16,     s->seg_base,    s->sell_off,  s->sell_pos,   s->sell_c32,
-                  s->sell_c16,   s->sell_base,   s->sell64,    s->sell32};
+                  s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
+                  s->sell2_c16};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
