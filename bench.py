@@ -215,14 +215,19 @@ def form_serve(args):
             while nxt < R and arrivals[nxt] <= now:
                 queue.append(nxt)
                 nxt += 1
-            # admit every waiting request that fits (a large one waiting for a
-            # lane worker does not hold back small ones behind it)
-            still = []
+            # admit every waiting request that fits (a large one waiting for
+            # the lane does not hold back small ones behind it); once a request
+            # of one path is refused, later ones of that path are not tried
+            still, full = [], set()
             for req in queue:
                 s = pool[req % P]
-                if s._h.value not in busy and srv.submit(s):
+                path = args.lane_workers > 0 and s.n >= args.lane_dofs
+                if path in full or s._h.value in busy:
+                    still.append(req)
+                elif srv.submit(s):
                     busy[s._h.value] = req
                 else:
+                    full.add(path)
                     still.append(req)
             queue = still
             live, _ = srv.status()

@@ -92,6 +92,7 @@ SIGNATURES = {
     "ehip_server_status": (C.c_int, [_vp, _vp, _vp]),
     "ehip_server_free": (C.c_int, [_vp]),
     "ehip_server_set_lane": (C.c_int, [_vp, _i64, C.c_int]),
+    "ehip_batch_set_lane": (C.c_int, [_vp, _i64, C.c_int]),
 }
 
 _lib = None

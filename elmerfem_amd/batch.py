@@ -16,13 +16,19 @@ from . import _lib
 
 
 def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=0.0, inner_maxit=0,
-                check_every=0):
-    """Returns (solutions, reports, device ms)."""
+                check_every=0, lane_dofs=None, lane_workers=None):
+    """Returns (solutions, reports, ms).  With lane_workers > 0, systems of
+    at least lane_dofs unknowns (default 40000) are solved whole on that many
+    concurrent threads instead of in the multi-system launches."""
     L = _lib.init()
     arr = (C.c_void_p * len(systems))(*[s._h for s in systems])
     h = C.c_void_p()
     _lib.check(L.ehip_batch_create(len(systems), arr, C.byref(h)), "ehip_batch_create")
     try:
+        if lane_dofs is not None or lane_workers is not None:
+            _lib.check(L.ehip_batch_set_lane(h, 40000 if lane_dofs is None else int(lane_dofs),
+                                             0 if lane_workers is None else int(lane_workers)),
+                       "ehip_batch_set_lane")
         p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every)
         reps = (_lib.Report * len(systems))()
         ms = C.c_double()

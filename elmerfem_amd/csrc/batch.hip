@@ -839,8 +839,13 @@ struct ehip_batch {
   int mixed = 0, method = 0;
   int64_t bytes = 0;
   int64_t budget = 0;              // serving: HBM budget for solver workspace (0 = none)
-  std::vector<int> batch_members;  // for ehip_batch_* : slot of each system
+  std::vector<int> batch_members;  // for ehip_batch_* : slot of each system (-1: concurrent lane)
   struct Lane *lane = nullptr;     // serving: concurrent lane for large systems (null = off)
+  // ehip_batch_solve: systems of at least lane_min_dofs unknowns are solved
+  // whole on lane_workers concurrent threads instead of in the multi-system
+  // launches (ehip_batch_set_lane)
+  int64_t lane_min_dofs = 40000;
+  int lane_workers = 0;
 };
 
 // Serving's concurrent lane: a system of at least min_dofs unknowns is not
@@ -1214,52 +1219,106 @@ extern "C" int ehip_batch_free(ehip_batch *b) {
   return EHIP_OK;
 }
 
+// Systems of at least lane_min_dofs unknowns do not join the multi-system
+// launches: an iteration of one of them already fills the GPU, and in a
+// shared launch sequence every op would also walk their chunks for the
+// phases they are not in.  They are solved whole by ehip_solve on
+// lane_workers threads (largest first), each on the system's own stream,
+// concurrently with the multi-system launches of the small ones; each such
+// result is exactly the system's ehip_solve result.
 extern "C" int ehip_batch_solve(ehip_batch *bh, const ehip_params *p, ehip_report *reports, double *t_ms) {
   if (!bh || !p) return EHIP_EARG;
   const int nsys = bh->capacity;
   std::vector<ehip_system *> sys(nsys);
   for (int i = 0; i < nsys; ++i) sys[i] = bh->slot[i].s;
+  std::vector<int> lane_idx, eng_idx;
+  for (int i = 0; i < nsys; ++i)
+    (bh->lane_workers > 0 && sys[i]->n >= bh->lane_min_dofs ? lane_idx : eng_idx).push_back(i);
+  for (int i : lane_idx)
+    if (!sys[i]->has_values) {
+      set_error("batch: system has no values");
+      return EHIP_ESTATE;
+    }
   ehip_batch *e = nullptr;
-  int rc = engine_create(nsys, p, &e);
+  int rc = engine_create(std::max(1, (int)eng_idx.size()), p, &e);
   if (rc) return rc;
   hipEvent_t e0, e1;
   EHIP_HIP_TRY(hipEventCreate(&e0));
   EHIP_HIP_TRY(hipEventCreate(&e1));
   auto t0 = std::chrono::steady_clock::now();
-  for (int i = 0; i < nsys; ++i) {
+  Lane *L = nullptr;
+  if (!lane_idx.empty()) {
+    std::sort(lane_idx.begin(), lane_idx.end(), [&](int a, int b) { return sys[a]->n > sys[b]->n; });
+    L = new Lane();
+    L->min_dofs = bh->lane_min_dofs;
+    L->workers = std::min(bh->lane_workers, (int)lane_idx.size());
+    L->p = *p;
+    L->p.record_history = 0;
+    L->p.profile_every = 0;
+    for (int i : lane_idx) {
+      L->q.push_back(sys[i]);
+      L->running++;
+      bh->batch_members[i] = -1;
+    }
+    e->lane = L;  // engine_free joins the workers
+    for (int w = 0; w < L->workers; ++w) L->th.emplace_back(lane_worker, L);
+  }
+  for (int i : eng_idx) {
     int slot = engine_admit(e, sys[i]);
     if (slot < 0) { engine_free(e); return slot; }
     bh->batch_members[i] = slot;
   }
   if ((rc = upload_worklist(e))) { engine_free(e); return rc; }
   EHIP_HIP_TRY(hipEventRecord(e0, e->stream));
-  // every replay runs graph_iters iterations of every live slot
-  const int gi = p->check_every > 0 ? p->check_every : 8;
-  const int64_t cap = (int64_t)p->maxit * (p->precision == EHIP_MIXED_IR ? 2 : 1) / gi + 64;
-  int64_t replays = 0;
-  for (;;) {
-    if ((rc = engine_step(e))) { engine_free(e); return rc; }
-    ++replays;
-    if (*e->h_active == 0 || replays > cap) break;
+  if (!eng_idx.empty()) {
+    // every replay runs graph_iters iterations of every live slot
+    const int gi = p->check_every > 0 ? p->check_every : 8;
+    const int64_t cap = (int64_t)p->maxit * (p->precision == EHIP_MIXED_IR ? 2 : 1) / gi + 64;
+    int64_t replays = 0;
+    for (;;) {
+      if ((rc = engine_step(e))) { engine_free(e); return rc; }
+      ++replays;
+      if (*e->h_active == 0 || replays > cap) break;
+    }
   }
   EHIP_HIP_TRY(hipEventRecord(e1, e->stream));
   EHIP_HIP_TRY(hipEventSynchronize(e1));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, e0, e1);
+  std::map<ehip_system *, ehip_report> lane_rep;
+  if (L) {
+    std::unique_lock<std::mutex> lk(L->mu);
+    L->cv_done.wait(lk, [&] { return L->done.size() == lane_idx.size(); });
+    for (auto &d : L->done) lane_rep[d.first] = d.second;
+  }
+  const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (int i = 0; i < nsys; ++i) {
     const int slot = bh->batch_members[i];
     ehip_report r{};
-    if (!e->h_st[slot].done) {
-      BSys &h = e->h_st[slot];
-      h.info = 2;  // replay cap reached before the slot finished
+    if (slot < 0) {
+      r = lane_rep[sys[i]];
+    } else {
+      if (!e->h_st[slot].done) {
+        BSys &h = e->h_st[slot];
+        h.info = 2;  // replay cap reached before the slot finished
+      }
+      if ((rc = engine_retire(e, slot, &r))) { engine_free(e); return rc; }
+      r.t_krylov_ms = ms;
+      r.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    if ((rc = engine_retire(e, slot, &r))) { engine_free(e); return rc; }
-    r.t_krylov_ms = ms;
-    r.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (reports) reports[i] = r;
   }
-  if (t_ms) *t_ms = ms;
+  if (t_ms) *t_ms = L ? std::max((double)ms, wall) : ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   engine_free(e);
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_set_lane(ehip_batch *b, int64_t min_dofs, int workers) {
+  if (!b || workers < 0 || workers > 64 || min_dofs < 0) return EHIP_EARG;
+  b->lane_min_dofs = min_dofs;
+  b->lane_workers = workers;
   return EHIP_OK;
 }
 

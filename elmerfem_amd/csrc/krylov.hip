@@ -1412,6 +1412,8 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
   *inner_total = 0;
   int outer = 0;
+  int bad = 0;             // consecutive inner solves that ended on a non-finite residual
+  bool diverged = false;
   SolveCtx ci = c;
   ci.mode = EHIP_FAST;
   ci.hist = nullptr;
@@ -1435,14 +1437,32 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
     if ((rc = krylov<float>(s, ci, s->plan32, s->vals32, v, s->st32, s->h_st32, s->partials32, false))) return rc;
     c.spmv += ci.spmv;
     *inner_total += s->h_st32->iter;
-    if (s->h_st32->info == 3 || s->h_st32->iter == 0) {
-      // inner breakdown without progress: stop with the outer residual
-      break;
+    if (s->h_st32->info == 3) {
+      // The inner solve ended on a non-finite residual (FP32 overflow on a
+      // hard system): its correction is dropped and the next refinement step
+      // restarts from the same FP64 residual with the inner cap halved below
+      // the breakdown point (the restart is deterministic, so the same cap
+      // would break down the same way).  Three in a row, or a breakdown at
+      // the first inner step, end the solve as diverged.
+      if (s->h_st32->iter <= 1 || ++bad >= 3) {
+        diverged = true;
+        break;
+      }
+      inner_maxit = std::max(1, s->h_st32->iter / 2);
+      continue;
     }
+    if (s->h_st32->iter == 0) break;  // inner breakdown without progress: stop with the outer residual
+    bad = 0;
     hipLaunchKernelGGL(k_correct, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->x, v.x, scale);
     ++outer;
   }
   if ((rc = read_state(s, st, s->h_st))) return rc;
+  if (diverged && !s->h_st->done) {
+    KState h = *s->h_st;
+    h.info = 3;
+    h.done = 1;
+    *s->h_st = h;
+  }
   if (!s->h_st->done) {
     // ran out of iterations
     KState h = *s->h_st;

@@ -206,15 +206,23 @@ int ehip_spmv_tune(ehip_system *s, int precision);
 int ehip_spmv_set_schedule(ehip_system *s, int precision, int block, int nnz_per_block);
 
 /* ---- batching: many independent systems in one multi-system launch ----
+ * No counterpart in the reference (a new form of the same solves).
  * Systems keep their own device memory.  The batch builds one device work
  * list of (system, row range) chunks over all of them; every launch of an
  * iteration walks that list, each system keeping its own scalars (alpha,
  * beta, omega, rho, convergence) and phase, and the iteration is replayed
- * from a captured hipGraph. */
+ * from a captured hipGraph.  Optionally (ehip_batch_set_lane) systems with
+ * at least lane_min_dofs unknowns are instead solved whole by ehip_solve on
+ * lane_workers host threads, concurrently with the multi-system launches;
+ * their results are exactly their ehip_solve results.  Off by default. */
 typedef struct ehip_batch ehip_batch;
 int ehip_batch_create(int nsys, ehip_system **systems, ehip_batch **out);
 int ehip_batch_free(ehip_batch *b);
-/* reports[nsys]; solutions stay on the device (ehip_batch_get_solution). */
+/* workers = 0 (the default): every system joins the multi-system launches */
+int ehip_batch_set_lane(ehip_batch *b, int64_t lane_min_dofs, int lane_workers);
+/* reports[nsys]; solutions stay on the device (ehip_batch_get_solution).
+ * t_ms: device time of the multi-system launches, or the wall time of the
+ * whole batch when some systems took the concurrent lane. */
 int ehip_batch_solve(ehip_batch *b, const ehip_params *p, ehip_report *reports, double *t_ms);
 int ehip_batch_get_solution(ehip_batch *b, int i, double *x);
 
@@ -235,8 +243,10 @@ int ehip_server_status(const ehip_server *srv, int *live, int64_t *bytes);
 /* Concurrent lane: systems with at least min_dofs unknowns are not batched
  * but solved whole (ehip_solve) on one of `workers` host threads, each on
  * the system's own stream, overlapping one another and the batched systems
- * on the device; a submission waits (EHIP_ECAP) until a worker is free.
- * workers = 0 turns the lane off (the default: everything is batched). */
+ * on the device.  Admitted systems queue in the lane (up to 4 per worker)
+ * so a worker starts its next solve as soon as it finishes one; beyond that
+ * a submission waits (EHIP_ECAP).  workers = 0 turns the lane off (the
+ * default: everything is batched). */
 int ehip_server_set_lane(ehip_server *srv, int64_t min_dofs, int workers);
 int ehip_server_free(ehip_server *srv);
 

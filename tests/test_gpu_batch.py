@@ -88,6 +88,36 @@ def test_alone_equals_in_mix(ehip):
         assert ra["iters"] == r["iters"] and ra["outer_iters"] == r["outer_iters"]
 
 
+@pytest.mark.parametrize("precision,method", [(_lib.MIXED_IR, _lib.CG), (_lib.FP64, _lib.BICGSTAB)])
+def test_batch_concurrent_lane(ehip, precision, method):
+    """A batch whose larger systems take the concurrent lane: those results
+    equal the system's own ehip_solve result bit for bit, the others their
+    multi-system result alone."""
+    kind = "tet4" if method == _lib.CG else "convdiff"
+    ws = [make(kind, i) for i in range(5)]
+    kw = ws[0]["kw"]
+    kw.method, kw.tol, kw.maxit = method, 1e-10, 20000
+    mode = _lib.FAST if precision == _lib.MIXED_IR else _lib.FAITHFUL
+    sizes = sorted(W.build_system(w, optimize_bandwidth=True)[0].n for w in ws)
+    cut = sizes[2]  # the three largest take the lane
+    alone = []
+    for w in ws:
+        s = W.build_system(w, optimize_bandwidth=True)[0]
+        if s.n >= cut:
+            x, r = s.solve(kw, precision=precision, mode=mode)
+        else:
+            xs1, reps1, _ = solve_batch([s], kw, precision=precision, mode=mode, lane_workers=0)
+            x, r = xs1[0], reps1[0]
+        alone.append((x, r))
+        s.free()
+    systems = [W.build_system(w, optimize_bandwidth=True)[0] for w in ws]
+    xs, reps, _ = solve_batch(systems, kw, precision=precision, mode=mode, lane_dofs=cut, lane_workers=2)
+    for (xa, ra), x, r in zip(alone, xs, reps):
+        assert r["info"] == ra["info"] == 1, (r, ra)
+        assert ra["iters"] == r["iters"]
+        assert np.array_equal(xa, x)
+
+
 def test_server_continuous_batching(ehip):
     """Systems join while others run; each result equals the batch-alone result bit for bit."""
     ws = [make("tet4", i) for i in range(6)]
