@@ -122,82 +122,105 @@ __global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
   __shared__ T prod[Op::SPMV ? NB : 1];
   __shared__ T sh[16 * 4];
   __shared__ int amlast;
+  __shared__ int act[BS];  // this round's active chunks, in walk order
+  __shared__ int wcount[BS / 64];
   const int nch = *a.nchunks;
   XcdWalk w(nch);
-  for (int c = w.first; c < w.last; c += w.step) {
-    const Chunk ck = a.chunks[c];
-    const SlotPtrs sp = a.slots[ck.slot];
-    BSys &st = a.st[ck.slot];
-    if (!op.active(st)) continue;  // uniform: the slot's state is fixed during this launch
-    typename Op::Ctx cx = op.begin(st, sp);
-    T acc[NV > 0 ? NV : 1];
-#pragma unroll
-    for (int k = 0; k < (NV > 0 ? NV : 1); ++k) acc[k] = (T)0;
-    if constexpr (Op::SPMV) {
-      const int j0 = sp.rows[ck.r0], j1 = sp.rows[ck.r1];
-      const int len = j1 - j0;
-      const T *xin = op.xin(cx);
-      const auto *vals = op.vals(sp);
-      if (len <= NB) {
-        // branch-free stream: indices past the chunk are clamped to its last
-        // nonzero (every load unconditional and in bounds); the extra
-        // products land in LDS slots no row reads
-        constexpr int ITEMS = NB / BS;
-        using VT = typename std::remove_cv<typename std::remove_pointer<decltype(vals)>::type>::type;
-        VT vv[ITEMS];
-        int cc[ITEMS];
-        T xv[ITEMS];
-        const int rr = ck.r0 + (int)threadIdx.x;
-        const int rl = min(rr, ck.r1 - 1);
-        const int ra = sp.rows[rl], rb = sp.rows[rl + 1];
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-          const int j = lane_nz<BS>(j0, len, k);
-          vv[k] = ntload(vals + j);
-          cc[k] = ntload(sp.cols + j);
-        }
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) xv[k] = xin[cc[k]];
-#pragma unroll
-        for (int k = 0; k < ITEMS; ++k) prod[threadIdx.x + k * BS] = (T)vv[k] * xv[k];
-        lds_barrier();
-        if (rr < ck.r1) {
-          T s = (T)0;
-          for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
-          op.row(cx, rr, s, acc);
-        }
-        for (int r = rr + BS; r < ck.r1; r += BS) {
-          const int s0 = sp.rows[r] - j0, s1 = sp.rows[r + 1] - j0;
-          T s = (T)0;
-          for (int k = s0; k < s1; ++k) s = s + prod[k];
-          op.row(cx, r, s, acc);
-        }
-        lds_barrier();
-      } else {
-        for (int r = ck.r0; r < ck.r1; ++r) {
-          T part = (T)0;
-          for (int j = sp.rows[r] + threadIdx.x; j < sp.rows[r + 1]; j += BS)
-            part += (T)ntload(vals + j) * xin[ntload(sp.cols + j)];
-          T s = block_sum(part, sh);
-          if (threadIdx.x == 0) op.row(cx, r, s, acc);
-          __syncthreads();
-        }
-      }
-    } else {
-      for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) op.row(cx, r, (T)0, acc);
-    }
-    // publish this chunk's partials (plain stores; the finalize launch reads them)
-    if constexpr (NV > 0) {
-      T bs[NV];
-#pragma unroll
-      for (int k = 0; k < NV; ++k) bs[k] = block_sum(acc[k], sh + 16 * k);
-      T *part = Op::template partials<T>(a);
-      if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) part[(size_t)(sp.ck0 + ck.k) * 4 + k] = bs[k];
-      }
-    }
+  // The workgroup's chunks are w.first + i w.step.  Most ops are active for
+  // only some slots (their phase), so the walk first checks BS chunks at
+  // once -- one chunk per thread, one memory latency for all of them -- and
+  // compacts the active ones; inactive chunks then cost nothing more.
+  const int m = w.first < w.last ? (w.last - w.first + w.step - 1) / w.step : 0;
+  for (int i0 = 0; i0 < m; i0 += BS) {
+    bool on = false;
+    const int ci = w.first + (i0 + (int)threadIdx.x) * w.step;
+    if (i0 + (int)threadIdx.x < m) on = op.active(a.st[a.chunks[ci].slot]);
+    const unsigned long long bal = __ballot(on);
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    if (lane == 0) wcount[wv] = __popcll(bal);
     __syncthreads();
+    int off = 0, nact = 0;
+    for (int q = 0; q < BS / 64; ++q) {
+      off += q < wv ? wcount[q] : 0;
+      nact += wcount[q];
+    }
+    if (on) act[off + __popcll(bal & ((1ull << lane) - 1))] = ci;
+    __syncthreads();
+    for (int ai = 0; ai < nact; ++ai) {
+      const int c = act[ai];
+      const Chunk ck = a.chunks[c];
+      const SlotPtrs sp = a.slots[ck.slot];
+      BSys &st = a.st[ck.slot];
+      typename Op::Ctx cx = op.begin(st, sp);
+      T acc[NV > 0 ? NV : 1];
+#pragma unroll
+      for (int k = 0; k < (NV > 0 ? NV : 1); ++k) acc[k] = (T)0;
+      if constexpr (Op::SPMV) {
+        const int j0 = sp.rows[ck.r0], j1 = sp.rows[ck.r1];
+        const int len = j1 - j0;
+        const T *xin = op.xin(cx);
+        const auto *vals = op.vals(sp);
+        if (len <= NB) {
+          // branch-free stream: indices past the chunk are clamped to its last
+          // nonzero (every load unconditional and in bounds); the extra
+          // products land in LDS slots no row reads
+          constexpr int ITEMS = NB / BS;
+          using VT = typename std::remove_cv<typename std::remove_pointer<decltype(vals)>::type>::type;
+          VT vv[ITEMS];
+          int cc[ITEMS];
+          T xv[ITEMS];
+          const int rr = ck.r0 + (int)threadIdx.x;
+          const int rl = min(rr, ck.r1 - 1);
+          const int ra = sp.rows[rl], rb = sp.rows[rl + 1];
+#pragma unroll
+          for (int k = 0; k < ITEMS; ++k) {
+            const int j = lane_nz<BS>(j0, len, k);
+            vv[k] = ntload(vals + j);
+            cc[k] = ntload(sp.cols + j);
+          }
+#pragma unroll
+          for (int k = 0; k < ITEMS; ++k) xv[k] = xin[cc[k]];
+#pragma unroll
+          for (int k = 0; k < ITEMS; ++k) prod[threadIdx.x + k * BS] = (T)vv[k] * xv[k];
+          lds_barrier();
+          if (rr < ck.r1) {
+            T s = (T)0;
+            for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
+            op.row(cx, rr, s, acc);
+          }
+          for (int r = rr + BS; r < ck.r1; r += BS) {
+            const int s0 = sp.rows[r] - j0, s1 = sp.rows[r + 1] - j0;
+            T s = (T)0;
+            for (int k = s0; k < s1; ++k) s = s + prod[k];
+            op.row(cx, r, s, acc);
+          }
+          lds_barrier();
+        } else {
+          for (int r = ck.r0; r < ck.r1; ++r) {
+            T part = (T)0;
+            for (int j = sp.rows[r] + threadIdx.x; j < sp.rows[r + 1]; j += BS)
+              part += (T)ntload(vals + j) * xin[ntload(sp.cols + j)];
+            T s = block_sum(part, sh);
+            if (threadIdx.x == 0) op.row(cx, r, s, acc);
+            __syncthreads();
+          }
+        }
+      } else {
+        for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) op.row(cx, r, (T)0, acc);
+      }
+      // publish this chunk's partials (plain stores; the finalize launch reads them)
+      if constexpr (NV > 0) {
+        T bs[NV];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) bs[k] = block_sum(acc[k], sh + 16 * k);
+        T *part = Op::template partials<T>(a);
+        if (threadIdx.x == 0) {
+#pragma unroll
+          for (int k = 0; k < NV; ++k) part[(size_t)(sp.ck0 + ck.k) * 4 + k] = bs[k];
+        }
+      }
+      __syncthreads();
+    }
   }
   (void)amlast;
 }
@@ -1047,8 +1070,23 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   int rc;
   if ((rc = ensure_state(s))) return rc;
   // unscaled b kept for the report residual
-  EHIP_HIP_TRY(hipMalloc(&sl.w64, sizeof(double) * (size_t)n * (NV64 + 1)));
-  if (b->mixed) EHIP_HIP_TRY(hipMalloc(&sl.w32, sizeof(float) * (size_t)n * (NV32 + 1)));
+  const size_t n64 = (size_t)n * (NV64 + 1), n32 = b->mixed ? (size_t)n * (NV32 + 1) : 0;
+  if (s->bw64_n < n64) {
+    if (s->bw64) EHIP_HIP_TRY(hipFree(s->bw64));
+    s->bw64 = nullptr;
+    s->bw64_n = 0;
+    EHIP_HIP_TRY(hipMalloc(&s->bw64, sizeof(double) * n64));
+    s->bw64_n = n64;
+  }
+  if (s->bw32_n < n32) {
+    if (s->bw32) EHIP_HIP_TRY(hipFree(s->bw32));
+    s->bw32 = nullptr;
+    s->bw32_n = 0;
+    EHIP_HIP_TRY(hipMalloc(&s->bw32, sizeof(float) * n32));
+    s->bw32_n = n32;
+  }
+  sl.w64 = s->bw64;
+  sl.w32 = b->mixed ? s->bw32 : nullptr;
   sl.bytes = slot_bytes(s, b->mixed);
   double *b0 = sl.w64 + (size_t)NV64 * n;
   EHIP_HIP_TRY(hipMemcpyAsync(b0, s->rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s->stream));
@@ -1115,9 +1153,7 @@ static int engine_retire(ehip_batch *b, int slot, ehip_report *r) {
   rep.norm = nrm;
   s->has_x = true;
   if (r) *r = rep;
-  (void)hipFree(sl.w64);
-  if (sl.w32) (void)hipFree(sl.w32);
-  b->bytes -= sl.bytes;
+  b->bytes -= sl.bytes;  // the workspace stays with the system for its next admission
   sl = ehip_batch::Slot();
   return EHIP_OK;
 }
@@ -1180,8 +1216,6 @@ static void engine_free(ehip_batch *b) {
   lane_stop(b);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto &sl : b->slot) {
-    if (sl.w64) (void)hipFree(sl.w64);
-    if (sl.w32) (void)hipFree(sl.w32);
   }
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

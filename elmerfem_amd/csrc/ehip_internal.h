@@ -105,6 +105,12 @@ struct ehip_system {
   double *sell64 = nullptr;
   float *sell32 = nullptr;
   bool sell_ok = false;
+  // workspace of the multi-system engine, kept across admissions (a
+  // hipFree per retired request would synchronise the whole device)
+  double *bw64 = nullptr;
+  size_t bw64_n = 0;
+  float *bw32 = nullptr;
+  size_t bw32_n = 0;
   int fmt64 = 0, fmt32 = 0;           // SpMV format per precision: 0 CSR stream, 1 SELL-64
   uint64_t vals_ver = 1, vals32_ver = 1, sell64_ver = 0, sell32_ver = 0;
   double *vals = nullptr, *rhs = nullptr, *x = nullptr;

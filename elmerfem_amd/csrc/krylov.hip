@@ -457,28 +457,31 @@ struct FinCgFast2 {
 // ---- folded CG steps (FP32 inner sequence of the mixed form): two launches
 // per iteration instead of three.  The SpMV step sums the previous vector
 // step's partials itself (every workgroup, same order, so all agree bit for
-// bit) and runs the stopping tests there; workgroup 0 records the scalars.
+// bit) to get rho and beta; workgroup 0 records the scalars.  The stopping
+// tests run only in FinCgFold, the one-workgroup finalize that closes every
+// block of iterations: taken inside the SpMV step by every workgroup, the
+// stop -- though computed from the same values everywhere -- corrupted a few
+// per cent of the solves when several solves ran concurrently on the GPU
+// (half-updated search directions), which the finalize-only test does not.
 // No launch reads a field that its own workgroup 0 writes:
-//   SpMV step   reads rho, iter, fresh (and beta when fresh);
-//               writes rho_new, rho_next, beta, rr, res, flags
+//   SpMV step   reads rho, fresh (and beta when fresh);
+//               writes rho_new, rho_next, beta, rr, res
 //   vector step reads rho_next; writes rho = rho_next, pq, alpha, iter, fresh
-// A block of iterations ends with FinCgFold, which leaves rho and beta
-// current (fresh = 1), as a residual replacement does.
-// the vector step's grid is at most kFoldPer x 256 (fused_grid), so the
-// SpMV step's prologue sums its partials with kFoldPer loads per thread
+// FinCgFold leaves rho and beta current (fresh = 1), as a residual
+// replacement does.
+//
+// The vector step's grid is at most kFoldPer x 256 (fused_grid), so the SpMV
+// step's prologue sums its partials with kFoldPer loads per thread.
 constexpr int kFoldPer = 4;
 static_assert(kFusedCap <= kFoldPer * 256, "vector-step partials must fit the SpMV prologue's loads");
 template <typename T>
 struct EpiCgFold {
   static constexpr int NV = 1;
-  static constexpr bool kStops = true;
   EHIP_EPI_MEMBERS(T)
   const T *rz_part;  // the vector step's partials: r.z at [0, rz_g), r.r at [rz_g, 2 rz_g)
   int rz_g;
   T beta;
-  bool stop;
   __device__ void init() {
-    stop = false;
     if (st->fresh) {
       beta = st->beta;
       if (blockIdx.x == 0 && threadIdx.x == 0) st->rho_next = st->rho;
@@ -487,17 +490,13 @@ struct EpiCgFold {
     T tot[2];
     block_reduce_few<T, 2, kFoldPer>(rz_part, rz_g, rz_g, tot);
     const T rho_old = st->rho;
-    const T res = sqrt(tot[1]) / st->rhsnorm;
-    const CgCheck c = cg_check(res, tot[0], st->iter, st);
     beta = tot[0] / rho_old;
-    stop = c.done != 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->rho_new = rho_old;
       st->rho_next = tot[0];
       st->beta = beta;
       st->rr = tot[1];
-      st->res = res;
-      cg_apply(c, st);
+      st->res = sqrt(tot[1]) / st->rhsnorm;
     }
   }
   struct Pre { T q, z, p; };
@@ -511,7 +510,7 @@ struct EpiCgFold {
   }
 };
 
-// end of a block of folded iterations: the SpMV step's summation and tests
+// end of a block of folded iterations: the summation and the stopping tests
 // for the last vector step, leaving rho and beta current
 template <typename T>
 struct FinCgFold {
@@ -1488,7 +1487,7 @@ static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, boo
   KState32 *st32 = s->st32;
   float *vpart = s->partials32 + kVecPartOff;
   for (int k = 0; k < count; ++k) {
-    EpiCgFold<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], vpart, gf, 0.f, false};
+    EpiCgFold<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], vpart, gf, 0.f};
     if (prof) prof_begin(s);
     launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
     if (prof) prof_end(s);
@@ -1610,10 +1609,11 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
       lres_prev = lr;
     }
     if (!h.done) {
-      // predicted iterations until the replacement threshold
+      // predicted iterations until the replacement threshold (the tests run
+      // at the end of each block), never past maxit
       int want = cap;
       if (rate < 0.0 && h.ru > 0.f && h.res > h.ru) want = predict(std::log((double)h.ru / (double)h.res) / rate);
-      batch = std::max(2, std::min(cap, want));
+      batch = std::max(1, std::min({std::max(2, std::min(cap, want)), h.maxit - h.iter}));
       continue;
     }
     // FP32 stopped: replacement due, recurrence converged, or failure
@@ -1632,6 +1632,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
     it_prev = h.iter;
     lres_prev = std::log(std::max(s->h_st->res, 1e-300));
     batch = std::max(2, std::min(cap, rate < 0.0 ? predict(std::log(delta) / rate) : cap));
+    batch = std::max(1, std::min(batch, h.maxit - h.iter));
   }
   if ((rc = read_state(s, st32, s->h_st32))) return rc;
   *inner_total = s->h_st32->iter;

@@ -14,20 +14,9 @@
 //            row operands it needs are loaded before phase 1 so their latency
 //            hides behind the streaming.
 #pragma once
-#include <type_traits>
-
 #include "reduce.h"
 
 namespace ehip {
-
-// An epilogue whose init() can find that the launch has nothing left to do
-// (a convergence test folded into the SpMV step) declares kStops = true and
-// sets `stop`, the same in every workgroup; the SpMV body then returns
-// before touching the matrix.
-template <class E, class = void>
-struct EpiStops : std::false_type {};
-template <class E>
-struct EpiStops<E, std::void_t<decltype(E::kStops)>> : std::bool_constant<E::kStops> {};
 
 template <typename T>
 __device__ __forceinline__ T ntload(const T *p) {
@@ -147,9 +136,6 @@ __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
-  if constexpr (EpiStops<Epi>::value) {
-    if (epi.stop) return;
-  }
   const int tid = (int)threadIdx.x;
   const XcdWalk w(nchunks);
 
@@ -265,9 +251,6 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
-  if constexpr (EpiStops<Epi>::value) {
-    if (epi.stop) return;
-  }
   const int lane = (int)threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const XcdWalk w((nslice + WPB - 1) / WPB);
@@ -327,9 +310,6 @@ __device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, in
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
-  if constexpr (EpiStops<Epi>::value) {
-    if (epi.stop) return;
-  }
   const int lane = (int)threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const XcdWalk w((nslice + WPB - 1) / WPB);

@@ -559,7 +559,7 @@ int ehip_system_free(ehip_system *s) {
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
                   s->cols16,     s->seg_base,    s->sell_off,  s->sell_pos,   s->sell_c32,
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
-                  s->sell2_c16};
+                  s->sell2_c16,  s->bw64,        s->bw32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
