@@ -463,9 +463,9 @@ def main():
     c32 = sys_.spmv_config(1)
     bits = c32["index_bits"]
     if c32["format"] == "sell":
-        kname = "k_spmv_epi_sell<float,256,8,%s,EpiCgFold<float>>" % ("true" if bits == 16 else "false")
+        kname = "k_spmv_epi_sell<float,256,8,%s,EpiCgFast1<float>>" % ("true" if bits == 16 else "false")
     else:
-        kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFold<float>>" % (c32["block"], c32["nnz_per_block"],
+        kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
                                                                   "ColsU16" if bits == 16 else "ColsI32")
     us = float(np.mean([r["spmv_us"] for r in sampled if r["spmv_samples"] > 0]))
     bytes32 = fused_cg32_bytes(nnz, n, bits, c32["format"], c32["chunks"])

@@ -52,10 +52,7 @@ constexpr int kVecPartOff = 8 * 2048;
 template <typename T>
 struct KStateT {
   T rho, rho_new, alpha, beta, omega, pq, rr, rhsnorm, res, ts, tt, snorm, res_true, scale, ru;
-  T rho_next;  // folded CG: r.z of the latest vector step, summed by the SpMV step after it
   int iter, info, done, maxit, snorm_exit, rec_conv, pad0, pad1;
-  int fresh;   // folded CG: rho and beta are current (no vector step since the last finalize)
-  int pad2;
   T tol, divtol;
   unsigned int ticket[4];
 };

@@ -369,7 +369,7 @@ struct CgTile {
 // do not depend on alpha and go out before the reduction, so the two
 // latencies overlap.  Then x += alpha p ; r -= alpha q ; z = r/dg with the
 // partials of r.z (and r.r when NA = 2).
-template <typename T, int NA, bool Fold = false>
+template <typename T, int NA>
 __device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<T> *st, const T *pq_part, int pq_g,
                                                T *partials) {
   const int stride = gridDim.x * kVecBlock * kVI;
@@ -378,15 +378,10 @@ __device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<
   cur.load(v, n, base);
   T pq[1];
   block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
-  const T alpha = (Fold ? st->rho_next : st->rho) / pq[0];
+  const T alpha = st->rho / pq[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pq = pq[0];
     st->alpha = alpha;
-    if constexpr (Fold) {
-      st->rho = st->rho_next;
-      st->fresh = 0;
-      st->iter = st->iter + 1;
-    }
   }
   T acc[NA];
 #pragma unroll
@@ -402,11 +397,11 @@ __device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<
 }
 
 // step 2 of the one-SpMV CG: partials r.z, r.r
-template <typename T, bool Fold = false>
+template <typename T>
 __global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
                                                         int pq_g, T *partials) {
   if (solve_done(st)) return;
-  cg_vector_step<T, 2, Fold>(n, v, st, pq_part, pq_g, partials);
+  cg_vector_step<T, 2>(n, v, st, pq_part, pq_g, partials);
 }
 
 // The stopping tests on the recurrence residual after a CG vector step, in
@@ -451,84 +446,6 @@ struct FinCgFast2 {
     st->iter = it;
     if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
     cg_apply(cg_check(res, tot[0], it, st), st);
-  }
-};
-
-// ---- folded CG steps (FP32 inner sequence of the mixed form): two launches
-// per iteration instead of three.  The SpMV step sums the previous vector
-// step's partials itself (every workgroup, same order, so all agree bit for
-// bit) to get rho and beta; workgroup 0 records the scalars.  The stopping
-// tests run only in FinCgFold, the one-workgroup finalize that closes every
-// block of iterations: taken inside the SpMV step by every workgroup, the
-// stop -- though computed from the same values everywhere -- corrupted a few
-// per cent of the solves when several solves ran concurrently on the GPU
-// (half-updated search directions), which the finalize-only test does not.
-// No launch reads a field that its own workgroup 0 writes:
-//   SpMV step   reads rho, fresh (and beta when fresh);
-//               writes rho_new, rho_next, beta, rr, res
-//   vector step reads rho_next; writes rho = rho_next, pq, alpha, iter, fresh
-// FinCgFold leaves rho and beta current (fresh = 1), as a residual
-// replacement does.
-//
-// The vector step's grid is at most kFoldPer x 256 (fused_grid), so the SpMV
-// step's prologue sums its partials with kFoldPer loads per thread.
-constexpr int kFoldPer = 4;
-static_assert(kFusedCap <= kFoldPer * 256, "vector-step partials must fit the SpMV prologue's loads");
-template <typename T>
-struct EpiCgFold {
-  static constexpr int NV = 1;
-  EHIP_EPI_MEMBERS(T)
-  const T *rz_part;  // the vector step's partials: r.z at [0, rz_g), r.r at [rz_g, 2 rz_g)
-  int rz_g;
-  T beta;
-  __device__ void init() {
-    if (st->fresh) {
-      beta = st->beta;
-      if (blockIdx.x == 0 && threadIdx.x == 0) st->rho_next = st->rho;
-      return;
-    }
-    T tot[2];
-    block_reduce_few<T, 2, kFoldPer>(rz_part, rz_g, rz_g, tot);
-    const T rho_old = st->rho;
-    beta = tot[0] / rho_old;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      st->rho_new = rho_old;
-      st->rho_next = tot[0];
-      st->beta = beta;
-      st->rr = tot[1];
-      st->res = sqrt(tot[1]) / st->rhsnorm;
-    }
-  }
-  struct Pre { T q, z, p; };
-  __device__ Pre load(int i) const { return Pre{v.q[i], v.z[i], v.p[i]}; }
-  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
-    const T qi = s + beta * pr.q;
-    const T pi = pr.z + beta * pr.p;
-    v.q[i] = qi;
-    v.p[i] = pi;
-    acc[0] += pi * qi;
-  }
-};
-
-// end of a block of folded iterations: the summation and the stopping tests
-// for the last vector step, leaving rho and beta current
-template <typename T>
-struct FinCgFold {
-  static constexpr int NV = 2;
-  KStateT<T> *st;
-  T *partials;
-  int stride;
-  __device__ void finish(T (&tot)[2]) {
-    if (threadIdx.x != 0) return;
-    const T rho_old = st->rho;
-    st->rho_new = rho_old;
-    st->rho = tot[0];
-    st->beta = tot[0] / rho_old;
-    st->rr = tot[1];
-    const T res = sqrt(tot[1]) / st->rhsnorm;
-    st->res = res;
-    cg_apply(cg_check(res, tot[0], st->iter, st), st);
-    st->fresh = 1;
   }
 };
 
@@ -945,7 +862,6 @@ __global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int ite
   st->res_true = (T)-1;
   st->pad0 = 0;
   st->pad1 = 0;
-  st->fresh = 1;
 }
 
 // ---- mixed precision
@@ -1080,7 +996,6 @@ struct EpiRU {
     } else {
       st32->ru = (float)(res * delta);
       st32->rec_conv = 0;
-      st32->fresh = 1;
       st32->done = 0;
     }
   }
@@ -1325,8 +1240,8 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_begin(s);
           launch_epi<T>(s, plan, vals, v.z, e1, st);
           if (c.prof) prof_end(s);
-          hipLaunchKernelGGL((k_cg_fast2<T, false>), dim3(gf), dim3(kVecBlock), 0, s->stream, n, v, st,
-                             (const T *)part, epi_grid<T>(s, plan), part + kVecPartOff);
+          hipLaunchKernelGGL(k_cg_fast2<T>, dim3(gf), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
+                             epi_grid<T>(s, plan), part + kVecPartOff);
           fin<T>(s, FinCgFast2<T>{st, part + kVecPartOff, gf, c.hist}, gf, st);
           c.spmv++;
         }
@@ -1480,21 +1395,25 @@ constexpr double kRuDelta = 1e-2;
 // 32 + 16 + ... iterations).
 constexpr int kRuMaxBlock = 63;
 
-// Enqueues `count` folded FP32 CG iterations (SpMV step, vector step) of the
-// mixed form and the finalize that closes the block.
+// Enqueues `count` FP32 fast-CG iterations of the mixed form: the fused SpMV
+// step, the vector step, and the one-workgroup finalize that runs the
+// stopping tests after every step (the replacement must follow the step
+// that crosses its threshold: running on past it lets the FP32 recurrence
+// drift from the true residual, and the continued search direction then
+// carries a badly scaled beta).
 static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, bool prof) {
   const int n = s->n, gf = fused_grid(n), pg = max_grid();
   KState32 *st32 = s->st32;
   float *vpart = s->partials32 + kVecPartOff;
   for (int k = 0; k < count; ++k) {
-    EpiCgFold<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], vpart, gf, 0.f};
+    EpiCgFast1<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], 0.f};
     if (prof) prof_begin(s);
     launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
     if (prof) prof_end(s);
-    launch_k((k_cg_fast2<float, true>), dim3(gf), dim3(kVecBlock), s->stream, n, v, st32,
-             (const float *)s->partials32, epi_grid<float>(s, s->plan32), vpart);
+    launch_k((k_cg_fast2<float>), dim3(gf), dim3(kVecBlock), s->stream, n, v, st32, (const float *)s->partials32,
+             epi_grid<float>(s, s->plan32), vpart);
+    fin<float>(s, FinCgFast2<float>{st32, vpart, gf, nullptr}, gf, st32);
   }
-  fin<float>(s, FinCgFold<float>{st32, vpart, gf}, gf, st32);
 }
 
 // What a captured block bakes in: buffers, SpMV format and schedule.
@@ -1513,7 +1432,7 @@ static uint64_t cg32_key(const ehip_system *s) {
 }
 
 // Enqueues `count` iterations as replays of recorded blocks (one graph launch
-// per power-of-two piece instead of two kernel launches per iteration).  The
+// per power-of-two piece instead of three kernel launches per iteration).  The
 // blocks are built node by node (launch_k under a recorder), not captured,
 // and hold kernels only; every iteration kernel reads its scalars from the
 // device state, so a replay is the same work as direct launches.
@@ -1609,8 +1528,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
       lres_prev = lr;
     }
     if (!h.done) {
-      // predicted iterations until the replacement threshold (the tests run
-      // at the end of each block), never past maxit
+      // predicted iterations until the replacement threshold, never past maxit
       int want = cap;
       if (rate < 0.0 && h.ru > 0.f && h.res > h.ru) want = predict(std::log((double)h.ru / (double)h.res) / rate);
       batch = std::max(1, std::min({std::max(2, std::min(cap, want)), h.maxit - h.iter}));

@@ -29,11 +29,9 @@ inline int vec_grid(int n) {
 
 // Grid for the CG vector steps that open with a reduction of the SpMV's
 // partial sums (every workgroup reads all of them): capped so that those
-// reads stay small beside the vector traffic, and so that the next SpMV
-// step can sum the vector step's own partials with 4 loads per thread.
-// Each workgroup walks several tiles, the next tile's loads in flight while
-// the current one is computed.  (1024 measured best of 256/512/1024/4096 on
-// the 2.1M-row Hex8 system.)
+// reads stay small beside the vector traffic.  Each workgroup walks several
+// tiles, the next tile's loads in flight while the current one is computed.
+// (1024 measured best of 256/512/1024/4096 on the 2.1M-row Hex8 system.)
 constexpr int kFusedCap = 1024;
 inline int fused_grid(int n) { return std::min(vec_grid(n), kFusedCap); }
 

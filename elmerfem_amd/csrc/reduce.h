@@ -88,31 +88,6 @@ __device__ __forceinline__ void block_reduce_all(const T *partials, int stride, 
   }
 }
 
-// The same sums for a prologue, where registers are scarce: g <= PER *
-// blockDim.x partials per value, PER loads per thread (only those that
-// exist), all issued before any add.  Each thread adds its values in the
-// order block_reduce_all does, so the totals agree with it.
-template <typename T, int NV, int PER>
-__device__ __forceinline__ void block_reduce_few(const T *partials, int stride, int g, T (&tot)[NV]) {
-  __shared__ T sh[16 * NV];
-  const int nt = blockDim.x;
-  T v[NV][PER];
-#pragma unroll
-  for (int k = 0; k < NV; ++k)
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int i = threadIdx.x + u * nt;
-      v[k][u] = i < g ? partials[k * stride + i] : (T)0;
-    }
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    T a = (T)0;
-#pragma unroll
-    for (int u = 0; u < PER; ++u) a += v[k][u];
-    tot[k] = block_sum(a, sh + 16 * k);
-  }
-}
-
 // XCD-aware chunk walk: gridDim.x is a multiple of 8 and the hardware
 // dispatches workgroup b to XCD b % 8, so XCD x owns the contiguous chunk
 // range [x*C/8, (x+1)*C/8) and its L2 keeps that range's x-vector window.

@@ -1,10 +1,10 @@
 """Concurrent single-system solves: several host threads, each solving its
 own system on its own stream, the way the server's concurrent lane runs
 them.  Every solve must converge and repeat its own result exactly, however
-the solves overlap on the device.  (A stopping test taken inside the folded
-CG SpMV step by every workgroup once corrupted a few per cent of such solves
--- half-updated search directions ending in iteration-limit stops or NaN --
-while sequential solves were unaffected; this is its regression test.)"""
+the solves overlap on the device.  (A CG variant that took its stopping test
+inside the SpMV step, in every workgroup, once corrupted a few per cent of
+such solves -- iteration-limit stops or NaN -- while sequential solves were
+unaffected; this is its regression test.)"""
 import queue
 import threading
 
