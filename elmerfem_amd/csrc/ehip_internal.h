@@ -38,10 +38,15 @@ struct SpmvPlan {
 #define EHIP_SPMV_CONFIGS(X) X(256, 2048) X(256, 1024) X(512, 2048)
 
 // Per-system partial-sum buffer (entries): SpMV epilogues publish at
-// [0, 4 * max_grid), vector kernels whose producer's partials are still
+// [0, 4 * kEpiStride), vector kernels whose producer's partials are still
 // being read in the same launch publish at [kVecPartOff, kPartials).
 constexpr int kPartials = 16 * 2048;
 constexpr int kVecPartOff = 8 * 2048;
+// Stride of an SpMV epilogue's partials (value k of workgroup b at
+// k * kEpiStride + b): at least the largest SpMV grid (sell_grid, plan
+// grids), and 4 values fit below kVecPartOff.
+constexpr int kEpiStride = 4096;
+static_assert(4 * kEpiStride <= kVecPartOff, "epilogue partials overlap the vector-step partials");
 
 // Device-resident Krylov scalars.  Kernels publish per-workgroup partial
 // sums; a one-workgroup finalize launch adds them in a fixed order and
@@ -172,12 +177,13 @@ const double *sell_values64(ehip_system *s);
 const float *sell_values32(ehip_system *s);
 
 // SELL-64 launch shape: nonzeros per lane in flight, and the grid (slice
-// groups of four per workgroup step, at most 2048 workgroups, a multiple of 8)
+// groups of four per workgroup step, at most 4096 workgroups -- measured
+// 2-3% faster than 2048 for FP32 and FP64 at 128^3 -- a multiple of 8)
 constexpr int kSellU = 8;
 constexpr int kSell2U = 4;  // pairs per lane in flight (pair-packed FP64)
 inline int sell_grid(const ehip_system *s) {
   int g = (s->nslice + 3) / 4;
-  if (g > 2048) g = 2048;
+  if (g > kEpiStride) g = kEpiStride;
   if (g < 8) g = 8;
   return (g + 7) / 8 * 8;
 }

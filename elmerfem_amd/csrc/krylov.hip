@@ -1202,7 +1202,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
   const int n = s->n;
   const int g = vec_grid(n);
   const int gf = fused_grid(n);
-  const int pg = max_grid();
+  const int pg = kEpiStride;
   int rc;
   const int iter0 = (c.mode == EHIP_FAITHFUL || c.method == EHIP_BICGSTAB) ? 1 : 0;
   if (init_residual) {
@@ -1313,7 +1313,7 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   Vecs<float> v = make_vecs<float>(s->work32, n, c.method, s->work32 + (size_t)nv * n, nullptr);
   v.b = v.r;  // the inner right-hand side is the normalised residual held in r
   const int g = vec_grid(n);
-  const int pg = max_grid();
+  const int pg = kEpiStride;
   hipLaunchKernelGGL(k_make_dg<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, v.dg,
                      c.precond);
   KState *st = s->st;
@@ -1402,7 +1402,7 @@ constexpr int kRuMaxBlock = 63;
 // drift from the true residual, and the continued search direction then
 // carries a badly scaled beta).
 static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, bool prof) {
-  const int n = s->n, gf = fused_grid(n), pg = max_grid();
+  const int n = s->n, gf = fused_grid(n), pg = kEpiStride;
   KState32 *st32 = s->st32;
   float *vpart = s->partials32 + kVecPartOff;
   for (int k = 0; k < count; ++k) {
@@ -1481,7 +1481,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   if ((rc = ensure_work32(s, nv + 1))) return rc;
   Vecs<float> v = make_vecs<float>(s->work32, n, EHIP_CG, s->work32 + (size_t)nv * n, nullptr);
   const int g = vec_grid(n);
-  const int pg = max_grid();
+  const int pg = kEpiStride;
   KState *st = s->st;
   KState32 *st32 = s->st32;
   hipLaunchKernelGGL(k_make_dg<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, v.dg,
@@ -1688,7 +1688,7 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   // unscaled FP64 true residual ||b - A x|| / ||b|| and ComputeNorm
   {
     double *out = reinterpret_cast<double *>(&s->st->ts);
-    EpiResid er{b0, s->partials, max_grid(), &s->st->ticket[0], out};
+    EpiResid er{b0, s->partials, kEpiStride, &s->st->ticket[0], out};
     launch_epi_fin<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
     sumsq(s, n, s->x, &s->st->rr);
     if ((rc = read_state(s, s->st, s->h_st))) return rc;
@@ -1741,7 +1741,7 @@ extern "C" int ehip_internal_report(ehip_system *s, const double *b0, double *re
   if ((rc = ensure_state(s))) return rc;
   const int n = s->n;
   double *out = reinterpret_cast<double *>(&s->st->ts);
-  EpiResid er{b0, s->partials, max_grid(), &s->st->ticket[0], out};
+  EpiResid er{b0, s->partials, kEpiStride, &s->st->ticket[0], out};
   launch_epi_fin<double>(s, s->plan64, s->vals, s->x, er, (const KState *)nullptr);
   sumsq(s, n, s->x, &s->st->rr);
   if ((rc = read_state(s, s->st, s->h_st))) return rc;
