@@ -5,7 +5,7 @@
 // is a device-resident array of (slot, row chunk) items walked by a fixed
 // grid; every slot carries its own Krylov scalars and a phase, so systems
 // of very different size and iteration count share one launch sequence.
-// That sequence (one iteration of every active slot) is captured once in a
+// That sequence (one iteration of every active slot) is recorded once in a
 // hipGraph and replayed; admitting or retiring a system only rewrites the
 // work list and the slot table between replays, so the graph never changes.
 //
@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "ehip_internal.h"
+#include "launch.h"
 #include "linalg_kernels.h"
 #include "spmv_core.h"
 
@@ -930,9 +931,9 @@ static BatchArgs args_of(ehip_batch *b) {
 template <class Op>
 static void launch_op(ehip_batch *b, const Op &op) {
   const int grid = max_grid();
-  hipLaunchKernelGGL((bk_run<Op, 256, kChunkNB>), dim3(grid), dim3(256), 0, b->stream, args_of(b), op);
+  launch_k((bk_run<Op, 256, kChunkNB>), dim3(grid), dim3(256), b->stream, args_of(b), op);
   const int fgrid = std::min(b->capacity, 1024);
-  hipLaunchKernelGGL((bk_fin<Op>), dim3(fgrid), dim3(256), 0, b->stream, args_of(b), op, b->capacity);
+  launch_k((bk_fin<Op>), dim3(fgrid), dim3(256), b->stream, args_of(b), op, b->capacity);
 }
 
 // one iteration of every active slot
@@ -973,9 +974,15 @@ static void enqueue_iteration(ehip_batch *b) {
 static int build_graph(ehip_batch *b, int iters) {
   if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
-  EHIP_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  // built node by node, not by stream capture, which would make the runtime
+  // reject legacy-stream work from other threads (the concurrent lane's)
+  GraphRec rec;
+  EHIP_HIP_TRY(hipGraphCreate(&rec.graph, 0));
+  b->graph = rec.graph;
+  tl_graph_rec = &rec;
   for (int k = 0; k < iters; ++k) enqueue_iteration(b);
-  EHIP_HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+  tl_graph_rec = nullptr;
+  EHIP_HIP_TRY(rec.err);
   EHIP_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   b->graph_iters = iters;
   b->graph_ok = true;

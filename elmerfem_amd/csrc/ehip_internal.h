@@ -151,7 +151,7 @@ struct ehip_system {
   // events for the next SpMV launch when it is a sample (hipExtLaunchKernelGGL
   // timestamps the kernel itself), else null
   hipEvent_t prof_start = nullptr, prof_stop = nullptr;
-  // captured blocks of 1, 2, 4, ..., 32 FP32 CG iterations (mixed form),
+  // recorded blocks of 1, 2, 4, ..., 32 FP32 CG iterations (mixed form),
   // valid while cg32_key (the buffers and SpMV schedule they bake in) holds
   hipGraphExec_t cg32_graph[6] = {};
   uint64_t cg32_key = 0;

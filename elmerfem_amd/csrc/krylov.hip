@@ -1416,7 +1416,7 @@ static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, boo
   }
 }
 
-// What a captured block bakes in: buffers, SpMV format and schedule.
+// What a recorded block bakes in: buffers, SpMV format and schedule.
 static uint64_t cg32_key(const ehip_system *s) {
   const uint64_t parts[] = {(uint64_t)(uintptr_t)s->work32, (uint64_t)(uintptr_t)s->vals32,
                             (uint64_t)(uintptr_t)s->sell32, (uint64_t)(uintptr_t)s->partials32,
@@ -1509,7 +1509,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   double lres_prev = std::log(std::max(s->h_st->res, 1e-300));
   double rate = 0.0;  // log residual change per iteration (< 0)
   int batch = std::min(cap, 8);
-  // the SELL copy of the FP32 values is refreshed here, outside any capture
+  // the SELL copy of the FP32 values is refreshed here, outside any recording
   if (s->fmt32 == 1 && s->sell_ok && !sell_values32(s)) return EHIP_ENOMEM;
   while (!s->h_st->done) {
     if (c.prof) {

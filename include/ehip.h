@@ -211,7 +211,7 @@ int ehip_spmv_set_schedule(ehip_system *s, int precision, int block, int nnz_per
  * list of (system, row range) chunks over all of them; every launch of an
  * iteration walks that list, each system keeping its own scalars (alpha,
  * beta, omega, rho, convergence) and phase, and the iteration is replayed
- * from a captured hipGraph.  Optionally (ehip_batch_set_lane) systems with
+ * from a recorded hipGraph.  Optionally (ehip_batch_set_lane) systems with
  * at least lane_min_dofs unknowns are instead solved whole by ehip_solve on
  * lane_workers host threads, concurrently with the multi-system launches;
  * their results are exactly their ehip_solve results.  Off by default. */
