@@ -86,11 +86,14 @@ struct ehip_system {
   int *seg_base = nullptr;
   int nseg = 0;
   bool c16 = false;
-  // SELL-64 copy for SpMV (see spmv_sell_body): slice s holds rows
+  // SELL-64 copy for SpMV (see spmv_sell_body): slice s holds slice rows
   // [64 s, 64 s + 64) column-major at [sell_off[s], sell_off[s+1]), padded to
-  // its longest row.  Built at pattern upload when the padding stays small;
-  // its values are refreshed from Values / the FP32 copy when those change.
+  // its longest row.  Slice row i is row i, or sell_row[i] when rows were
+  // sorted by length inside windows (sell_row[i] >= n: padding lane).  Built
+  // at pattern upload when the padding stays small; its values are refreshed
+  // from Values / the FP32 copy when those change.
   int nslice = 0;
+  int *sell_row = nullptr;            // nslice * 64, or null
   int64_t sell_nnz = 0;
   int *sell_off = nullptr;            // nslice + 1
   int *sell_pos = nullptr;            // nnz: SELL position of each CRS nonzero

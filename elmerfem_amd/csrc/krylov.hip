@@ -1062,7 +1062,7 @@ static void launch_epi_cs(ehip_system *s, const SpmvPlan &p, const T *vals, cons
 }
 
 template <typename T, int BS, int U, bool C16, class Epi>
-__global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ rows, int n,
+__global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                       const int *__restrict__ c32,
                                                       const unsigned short *__restrict__ c16,
                                                       const int *__restrict__ sbase, const int *__restrict__ off,
@@ -1070,17 +1070,17 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ ro
                                                       int nslice, Epi epi, const KStateT<T> *st) {
   if (st && solve_done(st)) return;
   const SellCols<C16> sc{c32, c16, sbase};
-  spmv_sell_body<T, T, BS, U>(rows, n, sc, off, vals, xin, nslice, epi);
+  spmv_sell_body<T, T, BS, U>(rows, n, rmap, sc, off, vals, xin, nslice, epi);
 }
 
 template <int BS, int U, class Epi>
-__global__ __launch_bounds__(BS) void k_spmv_epi_sell2(const int *__restrict__ rows, int n,
+__global__ __launch_bounds__(BS) void k_spmv_epi_sell2(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                        const unsigned short *__restrict__ c16,
                                                        const int *__restrict__ sbase, const int *__restrict__ off2,
                                                        const double *__restrict__ vals, const double *__restrict__ xin,
                                                        int nslice, Epi epi, const KState *st) {
   if (st && solve_done(st)) return;
-  spmv_sell2_body<double, BS, U>(rows, n, c16, sbase, off2, vals, xin, nslice, epi);
+  spmv_sell2_body<double, BS, U>(rows, n, rmap, c16, sbase, off2, vals, xin, nslice, epi);
 }
 
 // Workgroups (and so published partials) of an epilogue SpMV launch
@@ -1105,10 +1105,10 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
       if (s->sell2_ok) {  // pair-packed FP64 layout
         if (s->prof_start)
           hipExtLaunchKernelGGL((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), 0, s->stream,
-                                s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell2_c16, s->sell_base,
+                                s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sell2_c16, s->sell_base,
                                 s->sell2_off, sv, xin, s->nslice, e, st);
         else
-          launch_k((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n,
+          launch_k((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n, s->sell_row,
                    s->sell2_c16, s->sell_base, s->sell2_off, sv, xin, s->nslice, e, st);
         return;
       }
@@ -1118,17 +1118,17 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
     if (s->prof_start) {
       if (s->sell_c16)
         hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), 0, s->stream,
-                              s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
+                              s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sell_c32, s->sell_c16, s->sell_base,
                               s->sell_off, sv, xin, s->nslice, e, st);
       else
         hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), 0, s->stream,
-                              s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_c32, s->sell_c16, s->sell_base,
+                              s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sell_c32, s->sell_c16, s->sell_base,
                               s->sell_off, sv, xin, s->nslice, e, st);
     } else if (s->sell_c16) {
-      launch_k((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n,
+      launch_k((k_spmv_epi_sell<T, 256, kSellU, true, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n, s->sell_row,
                s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
     } else {
-      launch_k((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n,
+      launch_k((k_spmv_epi_sell<T, 256, kSellU, false, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n, s->sell_row,
                s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
     }
     return;

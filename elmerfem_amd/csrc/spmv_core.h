@@ -219,8 +219,12 @@ __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
 
 // ---------------------------------------------------------------- SELL-64
 // Sliced ELLPACK with slices of 64 rows (one wavefront): inside slice s the
-// k-th nonzero of row 64 s + l sits at position off[s] + 64 k + l; rows
-// shorter than the slice's longest are padded (value 0, a valid column).
+// k-th nonzero of slice row 64 s + l sits at position off[s] + 64 k + l;
+// rows shorter than the slice's longest are padded (value 0, a valid
+// column).  Slice row i is matrix row i, or rmap[i] when the rows were
+// sorted by length inside windows to keep the padding small (SELL-C-sigma:
+// patterns whose neighbouring rows differ in length, like Hex27 elements);
+// rmap[i] >= n marks a padding lane.
 // One lane owns one row and adds its products for k = 0, 1, ... in order --
 // the CRS order, so FP64 results stay bit-identical to the reference loop --
 // with no LDS and no barriers.  Consecutive lanes read consecutive positions
@@ -241,7 +245,8 @@ struct SellCols {
 
 // U: nonzeros per lane loaded before any of them is used
 template <typename V, typename A, int BS, int U, class SC, class Epi>
-__device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int n, const SC &sc,
+__device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
+                                               const SC &sc,
                                                const int *__restrict__ off, const V *__restrict__ vals,
                                                const A *__restrict__ x, int nslice, Epi &epi) {
   constexpr int WPB = BS / 64;  // slices per workgroup step
@@ -257,10 +262,10 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
   for (int gidx = w.first; gidx < w.last; gidx += w.step) {
     const int sl = gidx * WPB + wv;  // wave-uniform
     if (sl >= nslice) continue;
-    const int r = sl * 64 + lane;
+    const int r = rmap ? rmap[sl * 64 + lane] : sl * 64 + lane;  // >= n: a padding lane
     const int rl = min(r, n - 1);
     const typename Epi::Pre pre = epi.load(rl);
-    const int len = rows[rl + 1] - rows[rl];
+    const int len = r < n ? rows[rl + 1] - rows[rl] : 0;
     const int p0 = off[sl];
     const int wd = (off[sl + 1] - p0) >> 6;  // slice width
     A sum = (A)0;
@@ -298,7 +303,7 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
 // for FP64 are the limit -- and still adds k = 0, 1, ... in order.
 typedef double EhipDouble2 __attribute__((ext_vector_type(2)));
 template <typename A, int BS, int U, class Epi>
-__device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, int n,
+__device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                 const unsigned short *__restrict__ c16,
                                                 const int *__restrict__ base, const int *__restrict__ off2,
                                                 const double *__restrict__ vals, const A *__restrict__ x, int nslice,
@@ -316,10 +321,10 @@ __device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, in
   for (int gidx = w.first; gidx < w.last; gidx += w.step) {
     const int sl = gidx * WPB + wv;  // wave-uniform
     if (sl >= nslice) continue;
-    const int r = sl * 64 + lane;
+    const int r = rmap ? rmap[sl * 64 + lane] : sl * 64 + lane;  // >= n: a padding lane
     const int rl = min(r, n - 1);
     const typename Epi::Pre pre = epi.load(rl);
-    const int len = rows[rl + 1] - rows[rl];
+    const int len = r < n ? rows[rl + 1] - rows[rl] : 0;
     const int p0 = off2[sl];
     const int np = (off2[sl + 1] - p0) >> 7;  // pairs per row in this slice
     const int b = base[sl];

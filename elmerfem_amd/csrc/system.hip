@@ -127,24 +127,24 @@ static int launch_spmv_cs(ehip_system *s, const SpmvPlan &p, const V *vals, cons
 }
 
 template <typename V, typename A, int BS, int U, bool C16>
-__global__ __launch_bounds__(BS) void k_spmv_sell(const int *__restrict__ rows, int n,
+__global__ __launch_bounds__(BS) void k_spmv_sell(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                   const int *__restrict__ c32, const unsigned short *__restrict__ c16,
                                                   const int *__restrict__ sbase, const int *__restrict__ off,
                                                   const V *__restrict__ vals, const A *__restrict__ x, int nslice,
                                                   A *y) {
   EpiStore<A> epi{y};
   const SellCols<C16> sc{c32, c16, sbase};
-  spmv_sell_body<V, A, BS, U>(rows, n, sc, off, vals, x, nslice, epi);
+  spmv_sell_body<V, A, BS, U>(rows, n, rmap, sc, off, vals, x, nslice, epi);
 }
 
 template <typename A, int BS, int U>
-__global__ __launch_bounds__(BS) void k_spmv_sell2(const int *__restrict__ rows, int n,
+__global__ __launch_bounds__(BS) void k_spmv_sell2(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                    const unsigned short *__restrict__ c16,
                                                    const int *__restrict__ sbase, const int *__restrict__ off2,
                                                    const double *__restrict__ vals, const A *__restrict__ x,
                                                    int nslice, A *y) {
   EpiStore<A> epi{y};
-  spmv_sell2_body<A, BS, U>(rows, n, c16, sbase, off2, vals, x, nslice, epi);
+  spmv_sell2_body<A, BS, U>(rows, n, rmap, c16, sbase, off2, vals, x, nslice, epi);
 }
 
 template <typename V, typename A>
@@ -162,16 +162,16 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
     if constexpr (!F32) {
       if (s->sell2_ok) {
         hipLaunchKernelGGL((k_spmv_sell2<A, 256, kSell2U>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
-                           s->sell2_c16, s->sell_base, s->sell2_off, sv, x, s->nslice, y);
+                           s->sell_row, s->sell2_c16, s->sell_base, s->sell2_off, sv, x, s->nslice, y);
         return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
       }
     }
     if (s->sell_c16)
       hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, true>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
-                         s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
+                         s->sell_row, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
     else
       hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, false>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
-                         s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
+                         s->sell_row, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
     return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
   }
   if (s->c16) return launch_spmv_cs<V, A, ColsU16>(s, p, vals, x, y);
@@ -274,25 +274,49 @@ int ensure_state(ehip_system *s) {
 }
 
 // SELL-64 layout of the pattern (spmv_sell_body), kept when padding adds at
-// most a quarter to the nonzeros.  c0 holds the 0-based columns.  Padding
-// entries take the row's last column (value 0; they are never summed).
+// most a quarter to the nonzeros.  Rows go into slices in their own order
+// when that stays under the limit, else sorted by length (longest first,
+// stable) inside windows of sigma rows, the smallest window of 256, 1024 or
+// 4096 rows that meets the limit: the slice-row-to-row map is then kept on
+// the device.  c0 holds the 0-based columns.  Padding entries take the row's
+// last column (value 0; they are never summed).
+static int64_t sell_slices(int n, const int *rows, const std::vector<int> *ord, std::vector<int> &off) {
+  const int S = (n + 63) / 64;
+  off.assign((size_t)S + 1, 0);
+  int64_t tot = 0;
+  for (int sl = 0; sl < S; ++sl) {
+    int wmax = 1;
+    for (int i = sl * 64; i < std::min(n, sl * 64 + 64); ++i) {
+      const int r = ord ? (*ord)[i] : i;
+      wmax = std::max(wmax, rows[r + 1] - rows[r]);
+    }
+    off[sl] = (int)std::min<int64_t>(tot, INT32_MAX);
+    tot += 64 * (int64_t)wmax;
+  }
+  off[S] = (int)std::min<int64_t>(tot, INT32_MAX);
+  return tot;
+}
+
 static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   const int n = s->n;
   const int64_t nnz = s->nnz;
   const int *rows = s->h_rows.data();  // 1-based
   const int S = (n + 63) / 64;
-  std::vector<int> off((size_t)S + 1, 0);
-  int64_t tot = 0;
-  for (int sl = 0; sl < S; ++sl) {
-    int wmax = 0;
-    for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) wmax = std::max(wmax, rows[r + 1] - rows[r]);
-    wmax = std::max(wmax, 1);
-    off[sl] = (int)tot;
-    tot += 64 * (int64_t)wmax;
-    if (tot >= (int64_t)INT32_MAX) return EHIP_OK;
+  const int64_t limit = std::min<int64_t>(nnz + nnz / 4 + 64, INT32_MAX - 1);
+  std::vector<int> off;
+  std::vector<int> ord;  // slice row -> row (empty: identity)
+  int64_t tot = sell_slices(n, rows, nullptr, off);
+  for (int sigma : {256, 1024, 4096}) {
+    if (tot <= limit) break;
+    ord.resize((size_t)n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    for (int w0 = 0; w0 < n; w0 += sigma)
+      std::stable_sort(ord.begin() + w0, ord.begin() + std::min(n, w0 + sigma),
+                       [rows](int a, int b) { return rows[a + 1] - rows[a] > rows[b + 1] - rows[b]; });
+    tot = sell_slices(n, rows, &ord, off);
   }
-  off[S] = (int)tot;
-  if (tot > nnz + nnz / 4 + 64) return EHIP_OK;  // too much padding: CSR stream only
+  if (tot > limit) return EHIP_OK;  // too much padding: CSR stream only
+  const bool mapped = !ord.empty();
   std::vector<int> pos((size_t)nnz), col((size_t)tot, 0);
   std::vector<int> base((size_t)S, 0);
   bool c16 = true;
@@ -300,7 +324,8 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     const int wd = (off[sl + 1] - off[sl]) / 64;
     int lo = INT32_MAX, hi = INT32_MIN;
     for (int l = 0; l < 64; ++l) {
-      const int r = sl * 64 + l;
+      const int i = sl * 64 + l;
+      const int r = i < n ? (mapped ? ord[i] : i) : n;  // n: a padding lane
       const int len = r < n ? rows[r + 1] - rows[r] : 0;
       const int j0 = r < n ? rows[r] - 1 : 0;
       const int pad = len > 0 ? c0[j0 + len - 1] : std::min(r, n - 1);  // a column near the slice
@@ -317,6 +342,12 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     if ((int64_t)hi - lo > 65535) c16 = false;
   }
   int rc;
+  if (mapped) {
+    std::vector<int> rm((size_t)S * 64, n);
+    std::copy(ord.begin(), ord.end(), rm.begin());
+    if ((rc = dmalloc(s, (void **)&s->sell_row, sizeof(int) * rm.size()))) return rc;
+    EHIP_HIP_TRY(hipMemcpy(s->sell_row, rm.data(), sizeof(int) * rm.size(), hipMemcpyHostToDevice));
+  }
   if ((rc = dmalloc(s, (void **)&s->sell_off, sizeof(int) * off.size()))) return rc;
   if ((rc = dmalloc(s, (void **)&s->sell_pos, sizeof(int) * pos.size()))) return rc;
   EHIP_HIP_TRY(hipMemcpy(s->sell_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
@@ -378,13 +409,14 @@ __global__ void k_sell_fill(int64_t nnz, const int *__restrict__ pos, const TI *
     out[pos[j]] = (TO)in[j];
 }
 
-// FP64 values into the pair-packed layout: one thread per row (the layout
-// position follows from row, slice and k; padding entries stay zero)
-__global__ void k_sell2_fill(int n, const int *__restrict__ rows, const int *__restrict__ off2,
-                             const double *__restrict__ in, double *__restrict__ out) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+// FP64 values into the pair-packed layout: one thread per slice row (the
+// layout position follows from slice, lane and k; padding entries stay zero)
+__global__ void k_sell2_fill(int n, const int *__restrict__ rows, const int *__restrict__ rmap,
+                             const int *__restrict__ off2, const double *__restrict__ in, double *__restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int r = rmap ? rmap[i] : i;
     const int j0 = rows[r], len = rows[r + 1] - j0;
-    const int64_t p0 = off2[r >> 6] + 2 * (r & 63);
+    const int64_t p0 = off2[i >> 6] + 2 * (i & 63);
     for (int k = 0; k < len; ++k) out[p0 + (int64_t)(k >> 1) * 128 + (k & 1)] = in[j0 + k];
   }
 }
@@ -393,7 +425,7 @@ const double *sell_values64(ehip_system *s) {
   if (s->sell64_ver != s->vals_ver) {
     if (s->sell2_ok)
       hipLaunchKernelGGL(k_sell2_fill, dim3(std::min(4096, (s->n + 255) / 256)), dim3(256), 0, s->stream, s->n,
-                         s->rows, s->sell2_off, (const double *)s->vals, s->sell64);
+                         s->rows, s->sell_row, s->sell2_off, (const double *)s->vals, s->sell64);
     else
       hipLaunchKernelGGL((k_sell_fill<double, double>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
                          (const double *)s->vals, s->sell64);
@@ -559,7 +591,7 @@ int ehip_system_free(ehip_system *s) {
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
                   s->cols16,     s->seg_base,    s->sell_off,  s->sell_pos,   s->sell_c32,
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
-                  s->sell2_c16,  s->bw64,        s->bw32};
+                  s->sell2_c16,  s->sell_row,    s->bw64,      s->bw32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)

@@ -172,6 +172,77 @@ def test_spmv_long_rows(ehip):
     assert np.allclose(s.spmv(u), dense(rows, cols, vals) @ u, rtol=1e-13, atol=1e-12)
 
 
+def _ragged_matrix(n, far, rng):
+    """Rows of 3-5 nonzeros and rows of 40-48 nonzeros in blocks of 16, so
+    every slice of 64 consecutive rows mixes both: in row order the slices
+    pad far beyond the limit, sorted by length inside windows they do not.
+    Columns lie within `far` of the row (far >= 65536 forces int32 columns)."""
+    rows, cols = [1], []
+    for i in range(n):
+        k = int(rng.integers(40, 49)) if (i // 16) % 2 else int(rng.integers(3, 6))
+        lo, hi = max(0, i - far), min(n - 1, i + far)
+        c = np.unique(np.concatenate([[i], rng.integers(lo, hi + 1, k)]))
+        cols.extend((c + 1).tolist())
+        rows.append(rows[-1] + len(c))
+    rows = np.array(rows, np.int32)
+    cols = np.array(cols, np.int32)
+    diag = np.array([rows[i] + np.nonzero(cols[rows[i] - 1:rows[i + 1] - 1] == i + 1)[0][0] for i in range(n)],
+                    np.int32)
+    return rows, cols, diag
+
+
+@pytest.mark.parametrize("n,far,bits", [(50003, 2000, 16), (140001, 70000, 32)])
+def test_spmv_sorted_sell_layout(ehip, n, far, bits):
+    """Ragged row lengths: the SELL-64 layout exists only with rows sorted by
+    length inside windows; row sums keep the reference's bits in FP64 (pair-
+    packed values with 16-bit offsets, plain values with int32 columns), the
+    FP32 copy stays within FP32 rounding, and CG converges the same in both
+    formats."""
+    rng = np.random.default_rng(7)
+    rows, cols, diag = _ragged_matrix(n, far, rng)
+    r = np.repeat(np.arange(n), np.diff(rows))
+    vals = -rng.random(len(cols))
+    vals[diag - 1] = 0.0
+    rowsum = np.zeros(n)
+    np.add.at(rowsum, r, np.abs(vals))
+    vals[diag - 1] = rowsum + 1.0 + rng.random(n)  # diagonally dominant, not symmetric
+    s = System.from_csr(rows, cols, diag)
+    s.set_values(vals, np.ones(n))
+    s.spmv_set_format(_lib.SPMV_SELL)
+    cfg = s.spmv_config(0)
+    assert cfg["format"] == "sell" and cfg["index_bits"] == bits
+    u = rng.standard_normal(n)
+    ref = np.zeros(n)
+    O.lib().or_spmv(n, O._p(rows), O._p(cols), O._p(vals), O._p(u), O._p(ref))
+    assert np.array_equal(s.spmv(u), ref)
+    vals2 = vals * (1.0 + 0.5 * rng.random(len(vals)))  # refreshed values reach the layout
+    s.set_values(vals2, np.ones(n))
+    ref2 = np.zeros(n)
+    O.lib().or_spmv(n, O._p(rows), O._p(cols), O._p(vals2), O._p(u), O._p(ref2))
+    assert np.array_equal(s.spmv(u), ref2)
+    s.spmv_set_format(_lib.SPMV_CSR)
+    assert np.array_equal(s.spmv(u), ref2)
+    kw = _lib.Params(1, 1, 1e-10, 500, 0, _lib.FP64, _lib.FAST, 0, 0, 0, 0, 0, 0)
+    got = {}
+    for fmt in (_lib.SPMV_CSR, _lib.SPMV_SELL):
+        s.set_values(vals2, np.ones(n))
+        s.spmv_set_format(fmt, 0)
+        s.spmv_set_format(fmt, 1)
+        x, rep = s.solve(kw)
+        assert rep["info"] == 1, (fmt, rep)
+        got[fmt] = (rep["iters"], x)
+    assert abs(got[_lib.SPMV_CSR][0] - got[_lib.SPMV_SELL][0]) <= 1
+    assert np.abs(got[_lib.SPMV_CSR][1] - got[_lib.SPMV_SELL][1]).max() <= 1e-8 * np.abs(got[_lib.SPMV_CSR][1]).max()
+    # FP32 copy (made by a mixed-precision solve; unscaled, so Values rounded)
+    s.set_values(vals2, np.ones(n))
+    s.spmv_set_format(_lib.SPMV_SELL, 1)
+    s.solve(_lib.Params(1, 1, 1e-8, 50, 0, _lib.MIXED_IR, _lib.FAST, 0, 0, 0, 0, 0, 0))
+    ref32 = dense_free_spmv(rows, cols, vals2.astype(np.float32).astype(np.float64), u.astype(np.float32))
+    got32 = s.spmv(u, precision=1)
+    assert np.abs(got32 - ref32).max() <= 1e-5 * np.abs(ref32).max()
+    s.free()
+
+
 def test_matvec_abi(ehip):
     """The exact Matrix Vector Proc signature (Load.c:734-747) with host arrays."""
     w, pat, vals, rhs, rows = assembled_case(CASES["cg_poisson8"])
