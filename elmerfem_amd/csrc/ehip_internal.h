@@ -107,6 +107,14 @@ struct ehip_system {
   int *sell2_off = nullptr;
   unsigned short *sell2_c16 = nullptr;
   bool sell2_ok = false;
+  // node-blocked SELL (spmv_sellb_body) when the pattern is made of whole
+  // dofs x dofs blocks and a slice's node span fits 16 bits: sellb_d = dofs,
+  // one offset per block in sellb_c16 (sell_nnz / dofs entries) from
+  // sellb_base[slice]; the values then use the plain SELL layout
+  int sellb_d = 0;
+  unsigned short *sellb_c16 = nullptr;
+  int *sellb_base = nullptr;
+  int sell_sigma = 0;                 // window of the length sort (0: row order)
   double *sell64 = nullptr;
   float *sell32 = nullptr;
   bool sell_ok = false;
@@ -184,6 +192,7 @@ const float *sell_values32(ehip_system *s);
 // 2-3% faster than 2048 for FP32 and FP64 at 128^3 -- a multiple of 8)
 constexpr int kSellU = 8;
 constexpr int kSell2U = 4;  // pairs per lane in flight (pair-packed FP64)
+constexpr int kSellBU = 4;  // node blocks per lane in flight (node-blocked)
 inline int sell_grid(const ehip_system *s) {
   int g = (s->nslice + 3) / 4;
   if (g > kEpiStride) g = kEpiStride;

@@ -1083,6 +1083,28 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell2(const int *__restrict__ r
   spmv_sell2_body<double, BS, U>(rows, n, rmap, c16, sbase, off2, vals, xin, nslice, epi);
 }
 
+template <typename T, int BS, int U, int D, class Epi>
+__global__ __launch_bounds__(BS) void k_spmv_epi_sellb(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
+                                                       const unsigned short *__restrict__ cb,
+                                                       const int *__restrict__ nbase, const int *__restrict__ off,
+                                                       const T *__restrict__ vals, const T *__restrict__ xin,
+                                                       int nslice, Epi epi, const KStateT<T> *st) {
+  if (st && solve_done(st)) return;
+  spmv_sellb_body<T, T, BS, U, D>(rows, n, rmap, cb, nbase, off, vals, xin, nslice, epi);
+}
+
+template <typename T, int D, class Epi>
+static void launch_epi_sellb(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
+  const int g = sell_grid(s);
+  if (s->prof_start)
+    hipExtLaunchKernelGGL((k_spmv_epi_sellb<T, 256, kSellBU, D, Epi>), dim3(g), dim3(256), 0, s->stream,
+                          s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sellb_c16, s->sellb_base,
+                          s->sell_off, sv, xin, s->nslice, e, st);
+  else
+    launch_k((k_spmv_epi_sellb<T, 256, kSellBU, D, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n, s->sell_row,
+             s->sellb_c16, s->sellb_base, s->sell_off, sv, xin, s->nslice, e, st);
+}
+
 // Workgroups (and so published partials) of an epilogue SpMV launch
 template <typename T>
 static int epi_grid(const ehip_system *s, const SpmvPlan &p) {
@@ -1101,6 +1123,11 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
     const T *sv;
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
     const int g = sell_grid(s);
+    if (s->sellb_d) {  // node-blocked layout
+      if (s->sellb_d == 3) launch_epi_sellb<T, 3>(s, sv, xin, e, st);
+      else launch_epi_sellb<T, 2>(s, sv, xin, e, st);
+      return;
+    }
     if constexpr (!F32) {
       if (s->sell2_ok) {  // pair-packed FP64 layout
         if (s->prof_start)

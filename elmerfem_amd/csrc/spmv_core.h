@@ -358,4 +358,68 @@ __device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, in
   if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
 }
 
+// SELL-64 over node blocks, for patterns whose unknowns are interleaved per
+// node (D unknowns per node, every row made of whole D-column blocks
+// node*D .. node*D+D-1, as CRS patterns of D-DOF equations are): values lie
+// as in spmv_sell_body (entry k*D+d of slice row 64 s + l at
+// off[s] + 64 (k D + d) + l), and one 16-bit node offset from nbase[s] per
+// block, at off[s]/D + 64 k + l, replaces D column indices.  Summation order
+// is the row's ascending column order, as in the other layouts.
+template <typename V, typename A, int BS, int U, int D, class Epi>
+__device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
+                                                const unsigned short *__restrict__ cb,
+                                                const int *__restrict__ nbase, const int *__restrict__ off,
+                                                const V *__restrict__ vals, const A *__restrict__ x, int nslice,
+                                                Epi &epi) {
+  constexpr int WPB = BS / 64;
+  constexpr int NV = Epi::NV;
+  constexpr int NA = NV > 0 ? NV : 1;
+  A acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = (A)0;
+  epi.init();
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const XcdWalk w((nslice + WPB - 1) / WPB);
+  for (int gidx = w.first; gidx < w.last; gidx += w.step) {
+    const int sl = gidx * WPB + wv;  // wave-uniform
+    if (sl >= nslice) continue;
+    const int r = rmap ? rmap[sl * 64 + lane] : sl * 64 + lane;  // >= n: a padding lane
+    const int rl = min(r, n - 1);
+    const typename Epi::Pre pre = epi.load(rl);
+    const int nb = r < n ? (rows[rl + 1] - rows[rl]) / D : 0;  // blocks of this row
+    const int p0 = off[sl];
+    const int wb = (off[sl + 1] - p0) / (64 * D);  // slice width in blocks
+    const int q0 = p0 / D;
+    const int b = nbase[sl];
+    A sum = (A)0;
+    for (int k0 = 0; k0 < wb; k0 += U) {
+      V vv[U][D];
+      int cc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = min(k0 + u, wb - 1);  // clamped into the slice
+        cc[u] = D * (b + (int)ntload(cb + q0 + kk * 64 + lane));
+#pragma unroll
+        for (int d = 0; d < D; ++d) vv[u][d] = ntload(vals + p0 + (kk * D + d) * 64 + lane);
+      }
+      A xv[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int d = 0; d < D; ++d) xv[u][d] = x[cc[u] + d];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+#pragma clang fp contract(off)
+          const A t = sum + (A)vv[u][d] * xv[u][d];
+          sum = (k0 + u < nb) ? t : sum;
+        }
+    }
+    if (r < n) epi.row(r, sum, pre, acc);
+  }
+  if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
+}
+
 }  // namespace ehip

@@ -147,6 +147,16 @@ __global__ __launch_bounds__(BS) void k_spmv_sell2(const int *__restrict__ rows,
   spmv_sell2_body<A, BS, U>(rows, n, rmap, c16, sbase, off2, vals, x, nslice, epi);
 }
 
+template <typename V, typename A, int BS, int U, int D>
+__global__ __launch_bounds__(BS) void k_spmv_sellb(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
+                                                   const unsigned short *__restrict__ cb,
+                                                   const int *__restrict__ nbase, const int *__restrict__ off,
+                                                   const V *__restrict__ vals, const A *__restrict__ x, int nslice,
+                                                   A *y) {
+  EpiStore<A> epi{y};
+  spmv_sellb_body<V, A, BS, U, D>(rows, n, rmap, cb, nbase, off, vals, x, nslice, epi);
+}
+
 template <typename V, typename A>
 static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A *x, A *y) {
   if (s->nnz == 0) {
@@ -159,6 +169,15 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
     if (!sv) return EHIP_ENOMEM;
     const int g = sell_grid(s);
+    if (s->sellb_d) {
+      if (s->sellb_d == 3)
+        hipLaunchKernelGGL((k_spmv_sellb<V, A, 256, kSellBU, 3>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                           s->sell_row, s->sellb_c16, s->sellb_base, s->sell_off, sv, x, s->nslice, y);
+      else
+        hipLaunchKernelGGL((k_spmv_sellb<V, A, 256, kSellBU, 2>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                           s->sell_row, s->sellb_c16, s->sellb_base, s->sell_off, sv, x, s->nslice, y);
+      return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
+    }
     if constexpr (!F32) {
       if (s->sell2_ok) {
         hipLaunchKernelGGL((k_spmv_sell2<A, 256, kSell2U>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
@@ -308,6 +327,7 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   int64_t tot = sell_slices(n, rows, nullptr, off);
   for (int sigma : {256, 1024, 4096}) {
     if (tot <= limit) break;
+    s->sell_sigma = sigma;
     ord.resize((size_t)n);
     for (int i = 0; i < n; ++i) ord[i] = i;
     for (int w0 = 0; w0 < n; w0 += sigma)
@@ -315,8 +335,29 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
                        [rows](int a, int b) { return rows[a + 1] - rows[a] > rows[b + 1] - rows[b]; });
     tot = sell_slices(n, rows, &ord, off);
   }
-  if (tot > limit) return EHIP_OK;  // too much padding: CSR stream only
+  if (tot > limit) {  // too much padding: CSR stream only
+    s->sell_sigma = 0;
+    return EHIP_OK;
+  }
   const bool mapped = !ord.empty();
+  // node blocks: every row made of whole D-column blocks starting at node*D
+  // (the pattern of D unknowns per node, interleaved; read off the pattern
+  // itself, so CRS handed over without the DOF count gets it too)
+  auto blocked = [&](int D) {
+    if (n % D) return false;
+    for (int r = 0; r < n; ++r) {
+      const int j0 = rows[r] - 1, len = rows[r + 1] - rows[r];
+      if (len % D || len == 0) return false;
+      for (int m = 0; m < len; m += D) {
+        if (c0[j0 + m] % D) return false;
+        for (int d = 1; d < D; ++d)
+          if (c0[j0 + m + d] != c0[j0 + m] + d) return false;
+      }
+    }
+    return true;
+  };
+  const int D = blocked(3) ? 3 : blocked(2) ? 2 : 1;
+  bool blk = D > 1;
   std::vector<int> pos((size_t)nnz), col((size_t)tot, 0);
   std::vector<int> base((size_t)S, 0);
   bool c16 = true;
@@ -340,6 +381,7 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     }
     base[sl] = lo;
     if ((int64_t)hi - lo > 65535) c16 = false;
+    if (blk && hi / D - lo / D > 65535) blk = false;
   }
   int rc;
   if (mapped) {
@@ -353,7 +395,24 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   EHIP_HIP_TRY(hipMemcpy(s->sell_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->sell_pos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
   int64_t tot64 = tot;  // FP64 values: unpacked, or pair-packed below
-  if (c16) {
+  if (blk) {
+    // one node offset per block: block k of slice row 64 sl + l at off[sl]/D + 64 k + l
+    std::vector<unsigned short> cb((size_t)(tot / D));
+    std::vector<int> nb((size_t)S);
+    for (int sl = 0; sl < S; ++sl) {
+      nb[sl] = base[sl] / D;
+      const int wb = (off[sl + 1] - off[sl]) / (64 * D);
+      const size_t q0 = (size_t)off[sl] / D;
+      for (int k = 0; k < wb; ++k)
+        for (int l = 0; l < 64; ++l)
+          cb[q0 + (size_t)k * 64 + l] = (unsigned short)(col[(size_t)off[sl] + (size_t)k * D * 64 + l] / D - nb[sl]);
+    }
+    if ((rc = dmalloc(s, (void **)&s->sellb_c16, sizeof(unsigned short) * cb.size()))) return rc;
+    if ((rc = dmalloc(s, (void **)&s->sellb_base, sizeof(int) * S))) return rc;
+    EHIP_HIP_TRY(hipMemcpy(s->sellb_c16, cb.data(), sizeof(unsigned short) * cb.size(), hipMemcpyHostToDevice));
+    EHIP_HIP_TRY(hipMemcpy(s->sellb_base, nb.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+    s->sellb_d = D;
+  } else if (c16) {
     std::vector<unsigned short> c((size_t)tot);
     for (int sl = 0; sl < S; ++sl)
       for (int p = off[sl]; p < off[sl + 1]; ++p) c[p] = (unsigned short)(col[p] - base[sl]);
@@ -591,7 +650,8 @@ int ehip_system_free(ehip_system *s) {
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
                   s->cols16,     s->seg_base,    s->sell_off,  s->sell_pos,   s->sell_c32,
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
-                  s->sell2_c16,  s->sell_row,    s->bw64,      s->bw32};
+                  s->sell2_c16,  s->sell_row,    s->sellb_c16, s->sellb_base, s->bw64,
+                  s->bw32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
@@ -755,13 +815,24 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
     if (block) *block = 256;
     if (nnz_per_block) *nnz_per_block = 0;
     if (chunks) *chunks = s->nslice;
-    if (index_bits) *index_bits = s->sell_c16 ? 16 : 32;
+    if (index_bits) *index_bits = s->sell_c16 || s->sellb_d ? 16 : 32;
   } else {
     if (block) *block = p.block;
     if (nnz_per_block) *nnz_per_block = p.nb;
     if (chunks) *chunks = p.nchunks;
     if (index_bits) *index_bits = s->c16 ? 16 : 32;
   }
+  return EHIP_OK;
+}
+
+int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block) {
+  if (!s) return EHIP_EARG;
+  if (!s->sell_ok) {
+    set_error("ehip_spmv_sell_layout: this pattern has no SELL-64 layout (padding above the limit)");
+    return EHIP_EUNSUP;
+  }
+  if (sort_window) *sort_window = s->sell_sigma;
+  if (node_block) *node_block = s->sellb_d;
   return EHIP_OK;
 }
 

@@ -196,6 +196,16 @@ int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
  * columns; 32: int32 indices).  Any pointer may be NULL. */
 int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int *nnz_per_block, int *chunks,
                      int *index_bits);
+/* Shape of the SELL-64 layout (EHIP_EUNSUP when the pattern has none):
+ * sort_window = 0 when slices take rows in order, else the window (256,
+ * 1024 or 4096 rows) inside which rows were sorted by length, longest
+ * first, to keep the padding under the limit; node_block = D when the
+ * pattern is made of whole D x D blocks of node-interleaved unknowns
+ * (D = DOFs per node, 2 or 3) and one 16-bit node offset per block
+ * replaces D column indices, else 0.  Either pointer may be NULL.
+ * No counterpart in the reference (CRS_MatrixVectorProd, CRSMatrix.src:
+ * 3515-3640, reads CRS directly). */
+int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block);
 /* Selects the format for one precision (EHIP_EUNSUP: no SELL layout). */
 int ehip_spmv_set_format(ehip_system *s, int precision, int format);
 /* Autotuner: times every compiled CSR schedule and the SELL format on this

@@ -172,45 +172,59 @@ def test_spmv_long_rows(ehip):
     assert np.allclose(s.spmv(u), dense(rows, cols, vals) @ u, rtol=1e-13, atol=1e-12)
 
 
-def _ragged_matrix(n, far, rng):
+def _ragged_matrix(n, far, rng, dofs=1):
     """Rows of 3-5 nonzeros and rows of 40-48 nonzeros in blocks of 16, so
     every slice of 64 consecutive rows mixes both: in row order the slices
     pad far beyond the limit, sorted by length inside windows they do not.
-    Columns lie within `far` of the row (far >= 65536 forces int32 columns)."""
-    rows, cols = [1], []
-    for i in range(n):
+    Columns lie within `far` of the row (far >= 65536 forces int32 columns).
+    With dofs > 1 this is the node pattern of n // dofs nodes, each entry a
+    dense dofs x dofs block of node-interleaved unknowns."""
+    nn = n // dofs
+    nrows, ncols = [0], []
+    for i in range(nn):
         k = int(rng.integers(40, 49)) if (i // 16) % 2 else int(rng.integers(3, 6))
-        lo, hi = max(0, i - far), min(n - 1, i + far)
+        lo, hi = max(0, i - far), min(nn - 1, i + far)
         c = np.unique(np.concatenate([[i], rng.integers(lo, hi + 1, k)]))
-        cols.extend((c + 1).tolist())
-        rows.append(rows[-1] + len(c))
-    rows = np.array(rows, np.int32)
-    cols = np.array(cols, np.int32)
+        ncols.append(c)
+        nrows.append(nrows[-1] + len(c))
+    if dofs == 1:
+        rows = np.array(nrows, np.int32) + 1
+        cols = np.concatenate(ncols).astype(np.int32) + 1
+    else:
+        rl = np.repeat(dofs * np.diff(nrows), dofs)
+        rows = np.concatenate([[1], 1 + np.cumsum(rl)]).astype(np.int32)
+        blocks = [(dofs * c[:, None] + np.arange(dofs)).ravel() for c in ncols]
+        cols = np.concatenate([b for b in blocks for _ in range(dofs)]).astype(np.int32) + 1
     diag = np.array([rows[i] + np.nonzero(cols[rows[i] - 1:rows[i + 1] - 1] == i + 1)[0][0] for i in range(n)],
                     np.int32)
     return rows, cols, diag
 
 
-@pytest.mark.parametrize("n,far,bits", [(50003, 2000, 16), (140001, 70000, 32)])
-def test_spmv_sorted_sell_layout(ehip, n, far, bits):
+@pytest.mark.parametrize("n,far,bits,dofs,block", [(50003, 2000, 16, 1, 0), (140001, 70000, 32, 1, 0),
+                                                   (50001, 700, 16, 3, 3), (50000, 1000, 16, 2, 2),
+                                                   (210003, 70000, 32, 3, 0)])
+def test_spmv_sorted_sell_layout(ehip, n, far, bits, dofs, block):
     """Ragged row lengths: the SELL-64 layout exists only with rows sorted by
     length inside windows; row sums keep the reference's bits in FP64 (pair-
-    packed values with 16-bit offsets, plain values with int32 columns), the
-    FP32 copy stays within FP32 rounding, and CG converges the same in both
-    formats."""
+    packed values with 16-bit offsets, plain values with int32 columns, node
+    blocks with one 16-bit node offset per block -- unless the node span of a
+    slice needs more than 16 bits), the FP32 copy stays within FP32
+    rounding, and BiCGStab converges the same in both formats."""
     rng = np.random.default_rng(7)
-    rows, cols, diag = _ragged_matrix(n, far, rng)
+    rows, cols, diag = _ragged_matrix(n, far, rng, dofs)
     r = np.repeat(np.arange(n), np.diff(rows))
     vals = -rng.random(len(cols))
     vals[diag - 1] = 0.0
     rowsum = np.zeros(n)
     np.add.at(rowsum, r, np.abs(vals))
     vals[diag - 1] = rowsum + 1.0 + rng.random(n)  # diagonally dominant, not symmetric
-    s = System.from_csr(rows, cols, diag)
+    s = System.from_csr(rows, cols, diag)  # the block size is read off the pattern
     s.set_values(vals, np.ones(n))
     s.spmv_set_format(_lib.SPMV_SELL)
     cfg = s.spmv_config(0)
     assert cfg["format"] == "sell" and cfg["index_bits"] == bits
+    lay = s.spmv_sell_layout()
+    assert lay["sort_window"] > 0 and lay["node_block"] == block, lay
     u = rng.standard_normal(n)
     ref = np.zeros(n)
     O.lib().or_spmv(n, O._p(rows), O._p(cols), O._p(vals), O._p(u), O._p(ref))
