@@ -244,12 +244,17 @@ struct SellCols {
 };
 
 // U: nonzeros per lane loaded before any of them is used
+// FP64 values stop at the row's own length, so the sum is the reference's to
+// the bit whatever x holds.  FP32 copies (no bit-exact contract) sum the
+// padding too: a padding entry is +0 times x at a real column, which leaves
+// a finite sum unchanged, and the row-pointer loads are saved.
 template <typename V, typename A, int BS, int U, class SC, class Epi>
 __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                const SC &sc,
                                                const int *__restrict__ off, const V *__restrict__ vals,
                                                const A *__restrict__ x, int nslice, Epi &epi) {
   constexpr int WPB = BS / 64;  // slices per workgroup step
+  constexpr bool kF32 = sizeof(V) == 4;
   constexpr int NV = Epi::NV;
   constexpr int NA = NV > 0 ? NV : 1;
   A acc[NA];
@@ -265,7 +270,7 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
     const int r = rmap ? rmap[sl * 64 + lane] : sl * 64 + lane;  // >= n: a padding lane
     const int rl = min(r, n - 1);
     const typename Epi::Pre pre = epi.load(rl);
-    const int len = r < n ? rows[rl + 1] - rows[rl] : 0;
+    const int len = kF32 || r >= n ? 0 : rows[rl + 1] - rows[rl];
     const int p0 = off[sl];
     const int wd = (off[sl + 1] - p0) >> 6;  // slice width
     A sum = (A)0;
@@ -287,7 +292,8 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
         // loop (no fused multiply-add)
 #pragma clang fp contract(off)
         const A t = sum + (A)vv[u] * xv[u];
-        sum = (k0 + u < len) ? t : sum;
+        if constexpr (kF32) sum = (k0 + u < wd) ? t : sum;  // only the clamped repeats are skipped
+        else sum = (k0 + u < len) ? t : sum;
       }
     }
     if (r < n) epi.row(r, sum, pre, acc);
@@ -372,6 +378,7 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
                                                 const V *__restrict__ vals, const A *__restrict__ x, int nslice,
                                                 Epi &epi) {
   constexpr int WPB = BS / 64;
+  constexpr bool kF32 = sizeof(V) == 4;  // as in spmv_sell_body
   constexpr int NV = Epi::NV;
   constexpr int NA = NV > 0 ? NV : 1;
   A acc[NA];
@@ -387,7 +394,7 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
     const int r = rmap ? rmap[sl * 64 + lane] : sl * 64 + lane;  // >= n: a padding lane
     const int rl = min(r, n - 1);
     const typename Epi::Pre pre = epi.load(rl);
-    const int nb = r < n ? (rows[rl + 1] - rows[rl]) / D : 0;  // blocks of this row
+    const int nb = kF32 || r >= n ? 0 : (rows[rl + 1] - rows[rl]) / D;  // blocks of this row
     const int p0 = off[sl];
     const int wb = (off[sl + 1] - p0) / (64 * D);  // slice width in blocks
     const int q0 = p0 / D;
@@ -414,7 +421,8 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
         for (int d = 0; d < D; ++d) {
 #pragma clang fp contract(off)
           const A t = sum + (A)vv[u][d] * xv[u][d];
-          sum = (k0 + u < nb) ? t : sum;
+          if constexpr (kF32) sum = (k0 + u < wb) ? t : sum;
+          else sum = (k0 + u < nb) ? t : sum;
         }
     }
     if (r < n) epi.row(r, sum, pre, acc);
