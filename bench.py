@@ -42,17 +42,23 @@ def index_bytes(nnz: int, bits: int, fmt: str = "csr", units: int = 0) -> int:
     return 4 * nnz if bits == 32 else 2 * nnz + 4 * ((nnz + 2047) // 2048 + 2)
 
 
+def row_ptr_bytes(n: int, fp32: bool, fmt: str) -> int:
+    """Row pointers, where the kernel reads them: the FP32 SELL-64 kernels
+    sum the zero padding instead of reading row lengths."""
+    return 0 if (fp32 and fmt == "sell") else 4 * (n + 1)
+
+
 def spmv_bytes(nnz: int, n: int, fp32: bool = False, bits: int = 32, fmt: str = "csr", units: int = 0) -> int:
-    """Algorithmic bytes of one SpMV: values, column indices, row pointers,
-    x read once, y written once."""
+    """Algorithmic bytes of one SpMV: values, column indices, row pointers
+    (where read), x read once, y written once."""
     vb = 4 if fp32 else 8
-    return vb * nnz + index_bytes(nnz, bits, fmt, units) + 4 * (n + 1) + 2 * vb * n
+    return vb * nnz + index_bytes(nnz, bits, fmt, units) + row_ptr_bytes(n, fp32, fmt) + 2 * vb * n
 
 
 def fused_cg32_bytes(nnz: int, n: int, bits: int = 32, fmt: str = "csr", units: int = 0) -> int:
     """Fused FP32 CG step 1 (q = A z + beta q, p = z + beta p, p.q):
-    vals32 + column indices + rows + z + q (r/w) + p (r/w)."""
-    return 4 * nnz + index_bytes(nnz, bits, fmt, units) + 4 * (n + 1) + 4 * n + 16 * n
+    vals32 + column indices + row pointers (where read) + z + q (r/w) + p (r/w)."""
+    return 4 * nnz + index_bytes(nnz, bits, fmt, units) + row_ptr_bytes(n, True, fmt) + 4 * n + 16 * n
 
 
 def measured_traffic(kernel: str):
@@ -463,7 +469,12 @@ def main():
     c32 = sys_.spmv_config(1)
     bits = c32["index_bits"]
     if c32["format"] == "sell":
-        kname = "k_spmv_epi_sell<float,256,8,%s,EpiCgFast1<float>>" % ("true" if bits == 16 else "false")
+        lay = sys_.spmv_sell_layout()
+        if lay["node_block"]:
+            kname = "k_spmv_epi_sellb<float,256,%d,%d,EpiCgFast1<float>>" % (lay["lane_unroll"], lay["node_block"])
+        else:
+            kname = "k_spmv_epi_sell<float,256,%d,%s,EpiCgFast1<float>>" % (lay["lane_unroll"],
+                                                                           "true" if bits == 16 else "false")
     else:
         kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
                                                                   "ColsU16" if bits == 16 else "ColsI32")

@@ -115,6 +115,7 @@ struct ehip_system {
   unsigned short *sellb_c16 = nullptr;
   int *sellb_base = nullptr;
   int sell_sigma = 0;                 // window of the length sort (0: row order)
+  int sell_u = 8;                     // nonzeros per lane in flight: kSellU or 9
   double *sell64 = nullptr;
   float *sell32 = nullptr;
   bool sell_ok = false;
@@ -187,7 +188,8 @@ inline void touch_vals(ehip_system *s, bool v64, bool v32) {
 const double *sell_values64(ehip_system *s);
 const float *sell_values32(ehip_system *s);
 
-// SELL-64 launch shape: nonzeros per lane in flight, and the grid (slice
+// SELL-64 launch shape: nonzeros per lane in flight (kSellU, or 9 when that
+// wastes fewer loads on the pattern's slice widths), and the grid (slice
 // groups of four per workgroup step, at most 4096 workgroups -- measured
 // 2-3% faster than 2048 for FP32 and FP64 at 128^3 -- a multiple of 8)
 constexpr int kSellU = 8;

@@ -185,12 +185,15 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
         return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
       }
     }
-    if (s->sell_c16)
-      hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, true>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
-                         s->sell_row, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
-    else
-      hipLaunchKernelGGL((k_spmv_sell<V, A, 256, kSellU, false>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
-                         s->sell_row, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y);
+#define EHIP_SELL_LAUNCH(U, C16)                                                                          \
+  hipLaunchKernelGGL((k_spmv_sell<V, A, 256, U, C16>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,      \
+                     s->sell_row, s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, x, s->nslice, y)
+    if (s->sell_u == 9) {
+      if (s->sell_c16) EHIP_SELL_LAUNCH(9, true); else EHIP_SELL_LAUNCH(9, false);
+    } else {
+      if (s->sell_c16) EHIP_SELL_LAUNCH(kSellU, true); else EHIP_SELL_LAUNCH(kSellU, false);
+    }
+#undef EHIP_SELL_LAUNCH
     return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
   }
   if (s->c16) return launch_spmv_cs<V, A, ColsU16>(s, p, vals, x, y);
@@ -340,6 +343,17 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     return EHIP_OK;
   }
   const bool mapped = !ord.empty();
+  // nonzeros per lane in flight: 8 or 9, whichever loads fewer clamped
+  // repeats past the slice widths (9 fits the 27-wide slices of Hex8 meshes)
+  {
+    int64_t w8 = 0, w9 = 0;
+    for (int sl = 0; sl < S; ++sl) {
+      const int64_t wd = (off[sl + 1] - off[sl]) / 64;
+      w8 += (wd + 7) / 8 * 8 - wd;
+      w9 += (wd + 8) / 9 * 9 - wd;
+    }
+    s->sell_u = w9 < w8 ? 9 : kSellU;
+  }
   // node blocks: every row made of whole D-column blocks starting at node*D
   // (the pattern of D unknowns per node, interleaved; read off the pattern
   // itself, so CRS handed over without the DOF count gets it too)
@@ -825,7 +839,7 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
   return EHIP_OK;
 }
 
-int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block) {
+int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll) {
   if (!s) return EHIP_EARG;
   if (!s->sell_ok) {
     set_error("ehip_spmv_sell_layout: this pattern has no SELL-64 layout (padding above the limit)");
@@ -833,6 +847,7 @@ int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block) {
   }
   if (sort_window) *sort_window = s->sell_sigma;
   if (node_block) *node_block = s->sellb_d;
+  if (lane_unroll) *lane_unroll = s->sellb_d ? kSellBU : s->sell_u;
   return EHIP_OK;
 }
 

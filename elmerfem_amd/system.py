@@ -190,10 +190,12 @@ class System:
         _lib.check(_lib.load().ehip_spmv_set_format(self._h, precision, fmt), "ehip_spmv_set_format")
 
     def spmv_sell_layout(self):
-        """{"sort_window": 0 or the length-sort window, "node_block": 0 or DOFs per node}"""
-        w, b = C.c_int(), C.c_int()
-        _lib.check(_lib.load().ehip_spmv_sell_layout(self._h, C.byref(w), C.byref(b)), "ehip_spmv_sell_layout")
-        return {"sort_window": w.value, "node_block": b.value}
+        """{"sort_window": 0 or the length-sort window, "node_block": 0 or DOFs per node,
+        "lane_unroll": nonzeros (blocks, with node_block) each lane loads at once}"""
+        w, b, u = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.load().ehip_spmv_sell_layout(self._h, C.byref(w), C.byref(b), C.byref(u)),
+                   "ehip_spmv_sell_layout")
+        return {"sort_window": w.value, "node_block": b.value, "lane_unroll": u.value}
 
     def spmv_config(self, precision=0):
         f, b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -257,6 +257,51 @@ def test_spmv_sorted_sell_layout(ehip, n, far, bits, dofs, block):
     s.free()
 
 
+@pytest.mark.parametrize("far,bits", [(70000, 32), (2000, 16)])
+def test_spmv_lane_unroll_nine(ehip, far, bits):
+    """27 nonzeros per row (three runs of 9 around columns i - far, i,
+    i + far): the SELL-64 kernels load 9 nonzeros per lane at a time; the
+    FP64 row sums keep the reference's bits (plain values with int32
+    columns, pair-packed values with 16-bit offsets) and the FP32 copy stays
+    within FP32 rounding."""
+    n = 3 * far + 64 * 50 + 17
+    rows, cols = [1], []
+    for i in range(n):
+        c = set()
+        for cen in (i - far, i, i + far):
+            lo = min(max(cen - 4, 0), n - 9)
+            c.update(range(lo, lo + 9))
+        c = sorted(c)
+        while len(c) < 27:  # runs that overlap near the ends: fill up to 27 columns
+            c = sorted(set(c) | {min(set(range(n)) - set(c))})
+        cols.extend(j + 1 for j in c)
+        rows.append(len(cols) + 1)
+    rows = np.array(rows, np.int32)
+    cols = np.array(cols, np.int32)
+    assert np.all(np.diff(rows) == 27)
+    diag = np.array([rows[i] + np.nonzero(cols[rows[i] - 1:rows[i + 1] - 1] == i + 1)[0][0] for i in range(n)],
+                    np.int32)
+    rng = np.random.default_rng(9)
+    vals = rng.standard_normal(len(cols))
+    vals[diag - 1] = 30.0 + rng.random(n)
+    s = System.from_csr(rows, cols, diag)
+    s.set_values(vals, np.ones(n))
+    s.spmv_set_format(_lib.SPMV_SELL)
+    assert s.spmv_config(0)["index_bits"] == bits
+    lay = s.spmv_sell_layout()
+    assert lay == {"sort_window": 0, "node_block": 0, "lane_unroll": 9}, lay
+    u = rng.standard_normal(n)
+    ref = np.zeros(n)
+    O.lib().or_spmv(n, O._p(rows), O._p(cols), O._p(vals), O._p(u), O._p(ref))
+    assert np.array_equal(s.spmv(u), ref)
+    s.solve(_lib.Params(1, 1, 1e-8, 30, 0, _lib.MIXED_IR, _lib.FAST, 0, 0, 0, 0, 0, 0))
+    s.spmv_set_format(_lib.SPMV_SELL, 1)
+    ref32 = dense_free_spmv(rows, cols, vals.astype(np.float32).astype(np.float64), u.astype(np.float32))
+    got32 = s.spmv(u, precision=1)
+    assert np.abs(got32 - ref32).max() <= 1e-5 * np.abs(ref32).max()
+    s.free()
+
+
 def test_matvec_abi(ehip):
     """The exact Matrix Vector Proc signature (Load.c:734-747) with host arrays."""
     w, pat, vals, rhs, rows = assembled_case(CASES["cg_poisson8"])
