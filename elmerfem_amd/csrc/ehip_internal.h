@@ -116,6 +116,7 @@ struct ehip_system {
   int *sellb_base = nullptr;
   int sell_sigma = 0;                 // window of the length sort (0: row order)
   int sell_u = 8;                     // nonzeros per lane in flight: kSellU or 9
+  int sell2_u = 4;                    // pairs per lane in flight (pair-packed FP64): kSell2U or 7
   double *sell64 = nullptr;
   float *sell32 = nullptr;
   bool sell_ok = false;

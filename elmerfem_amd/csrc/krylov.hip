@@ -1105,6 +1105,18 @@ static void launch_epi_sellb(ehip_system *s, const T *sv, const T *xin, const Ep
              s->sellb_c16, s->sellb_base, s->sell_off, sv, xin, s->nslice, e, st);
 }
 
+template <int U, class Epi>
+static void launch_epi_sell2(ehip_system *s, const double *sv, const double *xin, const Epi &e, const KState *st) {
+  const int g = sell_grid(s);
+  if (s->prof_start)
+    hipExtLaunchKernelGGL((k_spmv_epi_sell2<256, U, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
+                          s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sell2_c16, s->sell_base, s->sell2_off, sv,
+                          xin, s->nslice, e, st);
+  else
+    launch_k((k_spmv_epi_sell2<256, U, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n, s->sell_row,
+             s->sell2_c16, s->sell_base, s->sell2_off, sv, xin, s->nslice, e, st);
+}
+
 template <typename T, int U, bool C16, class Epi>
 static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
   const int g = sell_grid(s);
@@ -1142,13 +1154,8 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
     }
     if constexpr (!F32) {
       if (s->sell2_ok) {  // pair-packed FP64 layout
-        if (s->prof_start)
-          hipExtLaunchKernelGGL((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), 0, s->stream,
-                                s->prof_start, s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sell2_c16, s->sell_base,
-                                s->sell2_off, sv, xin, s->nslice, e, st);
-        else
-          launch_k((k_spmv_epi_sell2<256, kSell2U, Epi>), dim3(g), dim3(256), s->stream, s->rows, s->n, s->sell_row,
-                   s->sell2_c16, s->sell_base, s->sell2_off, sv, xin, s->nslice, e, st);
+        if (s->sell2_u == 7) launch_epi_sell2<7>(s, sv, xin, e, st);
+        else launch_epi_sell2<kSell2U>(s, sv, xin, e, st);
         return;
       }
     }

@@ -180,8 +180,12 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
     }
     if constexpr (!F32) {
       if (s->sell2_ok) {
-        hipLaunchKernelGGL((k_spmv_sell2<A, 256, kSell2U>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
-                           s->sell_row, s->sell2_c16, s->sell_base, s->sell2_off, sv, x, s->nslice, y);
+        if (s->sell2_u == 7)
+          hipLaunchKernelGGL((k_spmv_sell2<A, 256, 7>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                             s->sell_row, s->sell2_c16, s->sell_base, s->sell2_off, sv, x, s->nslice, y);
+        else
+          hipLaunchKernelGGL((k_spmv_sell2<A, 256, kSell2U>), dim3(g), dim3(256), 0, s->stream, s->rows, s->n,
+                             s->sell_row, s->sell2_c16, s->sell_base, s->sell2_off, sv, x, s->nslice, y);
         return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
       }
     }
@@ -461,6 +465,15 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
       EHIP_HIP_TRY(hipMemcpy(s->sell2_c16, c2.data(), sizeof(unsigned short) * tot2, hipMemcpyHostToDevice));
       s->sell2_nnz = tot2;
       s->sell2_ok = true;
+      // pairs per lane in flight: 4 or 7, whichever loads fewer clamped
+      // repeats (7 fits the 14-pair slices of Hex8 meshes)
+      int64_t w4 = 0, w7 = 0;
+      for (int sl = 0; sl < S; ++sl) {
+        const int64_t np = (off2[sl + 1] - off2[sl]) / 128;
+        w4 += (np + 3) / 4 * 4 - np;
+        w7 += (np + 6) / 7 * 7 - np;
+      }
+      s->sell2_u = w7 < w4 ? 7 : kSell2U;
       tot64 = tot2;
     }
   } else {
@@ -839,7 +852,8 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
   return EHIP_OK;
 }
 
-int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll) {
+int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll,
+                          int *pair_unroll) {
   if (!s) return EHIP_EARG;
   if (!s->sell_ok) {
     set_error("ehip_spmv_sell_layout: this pattern has no SELL-64 layout (padding above the limit)");
@@ -848,6 +862,7 @@ int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int
   if (sort_window) *sort_window = s->sell_sigma;
   if (node_block) *node_block = s->sellb_d;
   if (lane_unroll) *lane_unroll = s->sellb_d ? kSellBU : s->sell_u;
+  if (pair_unroll) *pair_unroll = s->sell2_ok ? s->sell2_u : 0;
   return EHIP_OK;
 }
 

@@ -191,11 +191,12 @@ class System:
 
     def spmv_sell_layout(self):
         """{"sort_window": 0 or the length-sort window, "node_block": 0 or DOFs per node,
-        "lane_unroll": nonzeros (blocks, with node_block) each lane loads at once}"""
-        w, b, u = C.c_int(), C.c_int(), C.c_int()
-        _lib.check(_lib.load().ehip_spmv_sell_layout(self._h, C.byref(w), C.byref(b), C.byref(u)),
+        "lane_unroll": nonzeros (blocks, with node_block) each lane loads at once,
+        "pair_unroll": FP64 value pairs each lane loads at once (0: not pair-packed)}"""
+        w, b, u, pu = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _lib.check(_lib.load().ehip_spmv_sell_layout(self._h, C.byref(w), C.byref(b), C.byref(u), C.byref(pu)),
                    "ehip_spmv_sell_layout")
-        return {"sort_window": w.value, "node_block": b.value, "lane_unroll": u.value}
+        return {"sort_window": w.value, "node_block": b.value, "lane_unroll": u.value, "pair_unroll": pu.value}
 
     def spmv_config(self, precision=0):
         f, b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()

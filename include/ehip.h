@@ -202,12 +202,16 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
  * first, to keep the padding under the limit; node_block = D when the
  * pattern is made of whole D x D blocks of node-interleaved unknowns
  * (D = DOFs per node, 2 or 3) and one 16-bit node offset per block
- * replaces D column indices, else 0; lane_unroll = nonzeros each lane
- * loads before using them (8, or 9 when that wastes fewer loads on the
- * slice widths).  Any pointer may be NULL.
+ * replaces D column indices, else 0; lane_unroll = nonzeros (node blocks
+ * with node_block) each lane loads before using them (8, or 9 when that
+ * wastes fewer loads on the slice widths); pair_unroll = value pairs each
+ * lane loads at once in the pair-packed FP64 layout (4 or 7, chosen the
+ * same way; 0 when FP64 values are not pair-packed).  Any pointer may be
+ * NULL.
  * No counterpart in the reference (CRS_MatrixVectorProd, CRSMatrix.src:
  * 3515-3640, reads CRS directly). */
-int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll);
+int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll,
+                          int *pair_unroll);
 /* Selects the format for one precision (EHIP_EUNSUP: no SELL layout). */
 int ehip_spmv_set_format(ehip_system *s, int precision, int format);
 /* Autotuner: times every compiled CSR schedule and the SELL format on this
