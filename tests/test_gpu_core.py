@@ -460,6 +460,40 @@ def test_fast_and_mixed_solve(ehip, case, precision, mode, fmt):
     s.free()
 
 
+@pytest.mark.parametrize("precision,mode", [(_lib.FP64, _lib.FAITHFUL), (_lib.FP64, _lib.FAST),
+                                            (_lib.MIXED_IR, _lib.FAST)])
+def test_hex27_elasticity_solve(ehip, precision, mode):
+    """Hex27 elasticity (rows of 81 to 375 nonzeros in 3 x 3 node blocks):
+    the GPU solves on the length-sorted, node-blocked SELL-64 layout and
+    matches the oracle's FP64 solution."""
+    w = W.elasticity_hex(4, quadratic=True)
+    m = w["mesh"]
+    pat = O.Pattern(m.elements, m.nnodes, 3)
+    vals, rhs = O.assemble(1, m, pat, w["coef"], poisson=0.3)
+    O.assemble_traction(m, pat, m.bnd_ids == w["traction_bnd"], w["traction"], rhs)
+    k = pat.perm[w["bc_nodes"] - 1].astype(np.int64)
+    bc = np.concatenate([3 * (k - 1) + d for d in (1, 2, 3)]).astype(np.int32)
+    O.dirichlet(pat, vals, rhs, bc)
+    s = System.from_csr(pat.rows, pat.cols, pat.diag)
+    s.set_values(vals, rhs)
+    set_format(s, _lib.SPMV_SELL)
+    assert s.spmv_sell_layout()["node_block"] == 3
+    u = np.random.default_rng(3).standard_normal(pat.n)
+    ref = np.zeros(pat.n)
+    O.lib().or_spmv(pat.n, O._p(pat.rows), O._p(pat.cols), O._p(vals), O._p(u), O._p(ref))
+    assert np.array_equal(s.spmv(u), ref)
+    kw = w["kw"]
+    kw.method, kw.precond, kw.tol, kw.maxit = 0, 1, 1e-10, 20000
+    x, rep = s.solve(kw, precision=precision, mode=mode)
+    info, it, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, method=0, precond=1, tol=1e-12,
+                                                    maxit=20000)
+    assert rep["info"] == 1, rep
+    assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
+    A = dense(pat.rows, pat.cols, vals)
+    assert np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs) <= 1e-9
+    s.free()
+
+
 def test_status_codes(ehip):
     spec = CASES["cg_lognormal8"]
     w, pat, vals, rhs, rows = assembled_case(spec)
