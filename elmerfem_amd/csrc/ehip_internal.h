@@ -96,7 +96,6 @@ struct ehip_system {
   int *sell_row = nullptr;            // nslice * 64, or null
   int64_t sell_nnz = 0;
   int *sell_off = nullptr;            // nslice + 1
-  int *sell_pos = nullptr;            // nnz: SELL position of each CRS nonzero
   int *sell_c32 = nullptr;            // sell_nnz column indices, or
   unsigned short *sell_c16 = nullptr; // 16-bit offsets from sell_base[slice]
   int *sell_base = nullptr;           // nslice

@@ -376,7 +376,7 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   };
   const int D = blocked(3) ? 3 : blocked(2) ? 2 : 1;
   bool blk = D > 1;
-  std::vector<int> pos((size_t)nnz), col((size_t)tot, 0);
+  std::vector<int> col((size_t)tot, 0);
   std::vector<int> base((size_t)S, 0);
   bool c16 = true;
   for (int sl = 0; sl < S; ++sl) {
@@ -391,7 +391,6 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
       for (int k = 0; k < wd; ++k) {
         const int p = off[sl] + k * 64 + l;
         const int c = k < len ? c0[j0 + k] : pad;
-        if (k < len) pos[j0 + k] = p;
         col[p] = c;
         lo = std::min(lo, c);
         hi = std::max(hi, c);
@@ -409,9 +408,7 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     EHIP_HIP_TRY(hipMemcpy(s->sell_row, rm.data(), sizeof(int) * rm.size(), hipMemcpyHostToDevice));
   }
   if ((rc = dmalloc(s, (void **)&s->sell_off, sizeof(int) * off.size()))) return rc;
-  if ((rc = dmalloc(s, (void **)&s->sell_pos, sizeof(int) * pos.size()))) return rc;
   EHIP_HIP_TRY(hipMemcpy(s->sell_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-  EHIP_HIP_TRY(hipMemcpy(s->sell_pos, pos.data(), sizeof(int) * pos.size(), hipMemcpyHostToDevice));
   int64_t tot64 = tot;  // FP64 values: unpacked, or pair-packed below
   if (blk) {
     // one node offset per block: block k of slice row 64 sl + l at off[sl]/D + 64 k + l
@@ -489,32 +486,43 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   return EHIP_OK;
 }
 
-template <typename TI, typename TO>
-__global__ void k_sell_fill(int64_t nnz, const int *__restrict__ pos, const TI *__restrict__ in, TO *__restrict__ out) {
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x)
-    out[pos[j]] = (TO)in[j];
-}
-
-// FP64 values into the pair-packed layout: one thread per slice row (the
-// layout position follows from slice, lane and k; padding entries stay zero)
-__global__ void k_sell2_fill(int n, const int *__restrict__ rows, const int *__restrict__ rmap,
-                             const int *__restrict__ off2, const double *__restrict__ in, double *__restrict__ out) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int r = rmap ? rmap[i] : i;
-    const int j0 = rows[r], len = rows[r + 1] - j0;
-    const int64_t p0 = off2[i >> 6] + 2 * (i & 63);
-    for (int k = 0; k < len; ++k) out[p0 + (int64_t)(k >> 1) * 128 + (k & 1)] = in[j0 + k];
+// Values into a SELL-64 layout, one wave per slice: lane l gathers its
+// row's values in order and every position of the slice is written, the
+// padding as 0, so consecutive lanes store to consecutive addresses.  Plain
+// layout: entry k of lane l at off[sl] + 64 k + l; pair-packed (Pair): at
+// off[sl] + 128 (k/2) + 2 l + k%2.
+template <typename T, bool Pair>
+__global__ __launch_bounds__(256) void k_sell_gather(int n, int nslice, const int *__restrict__ rows,
+                                                     const int *__restrict__ rmap, const int *__restrict__ off,
+                                                     const T *__restrict__ in, T *__restrict__ out) {
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = (int)threadIdx.x >> 6;
+  for (int sl = blockIdx.x * 4 + wv; sl < nslice; sl += gridDim.x * 4) {
+    const int i = sl * 64 + lane;
+    const int r = rmap ? rmap[i] : i;  // >= n: a padding lane
+    const int j0 = r < n ? rows[r] : 0;
+    const int len = r < n ? rows[r + 1] - j0 : 0;
+    const int64_t p0 = off[sl];
+    const int wd = (off[sl + 1] - off[sl]) >> 6;  // entries per lane (pairs x 2 when Pair)
+#pragma unroll 4
+    for (int k = 0; k < wd; ++k) {
+      const T v = k < len ? in[j0 + k] : (T)0;
+      const int64_t p = Pair ? p0 + (int64_t)(k >> 1) * 128 + 2 * lane + (k & 1) : p0 + (int64_t)k * 64 + lane;
+      out[p] = v;
+    }
   }
 }
+
+static int gather_grid(const ehip_system *s) { return std::max(1, std::min(4096, (s->nslice + 3) / 4)); }
 
 const double *sell_values64(ehip_system *s) {
   if (s->sell64_ver != s->vals_ver) {
     if (s->sell2_ok)
-      hipLaunchKernelGGL(k_sell2_fill, dim3(std::min(4096, (s->n + 255) / 256)), dim3(256), 0, s->stream, s->n,
-                         s->rows, s->sell_row, s->sell2_off, (const double *)s->vals, s->sell64);
+      hipLaunchKernelGGL((k_sell_gather<double, true>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
+                         s->nslice, s->rows, s->sell_row, s->sell2_off, (const double *)s->vals, s->sell64);
     else
-      hipLaunchKernelGGL((k_sell_fill<double, double>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
-                         (const double *)s->vals, s->sell64);
+      hipLaunchKernelGGL((k_sell_gather<double, false>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
+                         s->nslice, s->rows, s->sell_row, s->sell_off, (const double *)s->vals, s->sell64);
     s->sell64_ver = s->vals_ver;
   }
   return s->sell64;
@@ -523,12 +531,11 @@ const double *sell_values64(ehip_system *s) {
 const float *sell_values32(ehip_system *s) {
   if (!s->sell32) {
     if (dmalloc(s, (void **)&s->sell32, sizeof(float) * s->sell_nnz)) return nullptr;
-    (void)hipMemsetAsync(s->sell32, 0, sizeof(float) * s->sell_nnz, s->stream);
     s->sell32_ver = 0;
   }
   if (s->sell32_ver != s->vals32_ver) {
-    hipLaunchKernelGGL((k_sell_fill<float, float>), dim3(4096), dim3(256), 0, s->stream, s->nnz, s->sell_pos,
-                       (const float *)s->vals32, s->sell32);
+    hipLaunchKernelGGL((k_sell_gather<float, false>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
+                       s->nslice, s->rows, s->sell_row, s->sell_off, (const float *)s->vals32, s->sell32);
     s->sell32_ver = s->vals32_ver;
   }
   return s->sell32;
@@ -675,7 +682,7 @@ int ehip_system_free(ehip_system *s) {
                   s->partials,   s->st,          s->elements,  s->st32,       s->partials32,
                   s->hist,       s->slot_start,  s->slot_contrib, s->rhs_start, s->rhs_contrib,
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
-                  s->cols16,     s->seg_base,    s->sell_off,  s->sell_pos,   s->sell_c32,
+                  s->cols16,     s->seg_base,    s->sell_off,  s->sell_c32,
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
                   s->sell2_c16,  s->sell_row,    s->sellb_c16, s->sellb_base, s->bw64,
                   s->bw32};
