@@ -513,16 +513,75 @@ __global__ __launch_bounds__(256) void k_sell_gather(int n, int nslice, const in
   }
 }
 
+// The same for slices that take rows in order: a slice's rows are one
+// contiguous CRS range, which the wave first copies into LDS with coalesced
+// loads (when it fits), so the per-lane row gathers read LDS instead of
+// touching 64 cache lines per load.  The four waves of a workgroup take
+// four consecutive slices per step, so every wave runs the same number of
+// steps and the workgroup barriers are uniform.
+template <typename T, bool Pair>
+__global__ __launch_bounds__(256) void k_sell_gather_lds(int n, int nslice, const int *__restrict__ rows,
+                                                         const int *__restrict__ off, const T *__restrict__ in,
+                                                         T *__restrict__ out) {
+  constexpr int kCap = 2048;  // entries staged per slice
+  __shared__ T buf[4][kCap];
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = (int)threadIdx.x >> 6;
+  const int ngroups = (nslice + 3) / 4;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int sl = grp * 4 + wv;
+    const bool live = sl < nslice;
+    const int a = live ? rows[sl * 64] : 0;
+    const int b = live ? rows[min(sl * 64 + 64, n)] : 0;
+    const bool staged = live && b - a <= kCap;
+    if (staged)
+      for (int j = lane; j < b - a; j += 64) buf[wv][j] = in[a + j];
+    __syncthreads();
+    if (live) {
+      const int r = sl * 64 + lane;
+      const int j0 = r < n ? rows[r] : 0;
+      const int len = r < n ? rows[r + 1] - j0 : 0;
+      const int64_t p0 = off[sl];
+      const int wd = (off[sl + 1] - off[sl]) >> 6;
+      auto val = [&](int k) -> T { return k < len ? (staged ? buf[wv][j0 - a + k] : in[j0 + k]) : (T)0; };
+      if constexpr (Pair) {
+        // both entries of a pair in one 16-byte store: a wave writes 1 KB contiguously
+        static_assert(sizeof(T) == 8, "pair-packed values are FP64");
+#pragma unroll 2
+        for (int kp = 0; kp < (wd >> 1); ++kp) {
+          EhipDouble2 v;
+          v[0] = val(2 * kp);
+          v[1] = val(2 * kp + 1);
+          *reinterpret_cast<EhipDouble2 *>(out + p0 + (int64_t)kp * 128 + 2 * lane) = v;
+        }
+      } else {
+#pragma unroll 4
+        for (int k = 0; k < wd; ++k) out[p0 + (int64_t)k * 64 + lane] = val(k);
+      }
+    }
+    __syncthreads();  // the next step's staging overwrites buf
+  }
+}
+
 static int gather_grid(const ehip_system *s) { return std::max(1, std::min(4096, (s->nslice + 3) / 4)); }
+
+// values of one precision into a SELL layout (plain or pair-packed)
+template <typename T, bool Pair>
+static void sell_gather(ehip_system *s, const int *off, const T *in, T *out) {
+  if (s->sell_row)
+    hipLaunchKernelGGL((k_sell_gather<T, Pair>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n, s->nslice,
+                       s->rows, s->sell_row, off, in, out);
+  else
+    hipLaunchKernelGGL((k_sell_gather_lds<T, Pair>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
+                       s->nslice, s->rows, off, in, out);
+}
 
 const double *sell_values64(ehip_system *s) {
   if (s->sell64_ver != s->vals_ver) {
     if (s->sell2_ok)
-      hipLaunchKernelGGL((k_sell_gather<double, true>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
-                         s->nslice, s->rows, s->sell_row, s->sell2_off, (const double *)s->vals, s->sell64);
+      sell_gather<double, true>(s, s->sell2_off, s->vals, s->sell64);
     else
-      hipLaunchKernelGGL((k_sell_gather<double, false>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
-                         s->nslice, s->rows, s->sell_row, s->sell_off, (const double *)s->vals, s->sell64);
+      sell_gather<double, false>(s, s->sell_off, s->vals, s->sell64);
     s->sell64_ver = s->vals_ver;
   }
   return s->sell64;
@@ -534,8 +593,7 @@ const float *sell_values32(ehip_system *s) {
     s->sell32_ver = 0;
   }
   if (s->sell32_ver != s->vals32_ver) {
-    hipLaunchKernelGGL((k_sell_gather<float, false>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n,
-                       s->nslice, s->rows, s->sell_row, s->sell_off, (const float *)s->vals32, s->sell32);
+    sell_gather<float, false>(s, s->sell_off, s->vals32, s->sell32);
     s->sell32_ver = s->vals32_ver;
   }
   return s->sell32;
