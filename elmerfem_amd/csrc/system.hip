@@ -305,7 +305,8 @@ int ensure_state(ehip_system *s) {
 // stable) inside windows of sigma rows, the smallest window of 256, 1024 or
 // 4096 rows that meets the limit: the slice-row-to-row map is then kept on
 // the device.  c0 holds the 0-based columns.  Padding entries take the row's
-// last column (value 0; they are never summed).
+// last column with value 0 (the FP64 kernels stop at the row length; the
+// FP32 kernels add them, +0 times a finite x, which changes no sum).
 static int64_t sell_slices(int n, const int *rows, const std::vector<int> *ord, std::vector<int> &off) {
   const int S = (n + 63) / 64;
   off.assign((size_t)S + 1, 0);
