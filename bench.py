@@ -126,8 +126,9 @@ def form_batch(args):
     nnz = int(sum(s.nnz for s in systems))
     kw = W._kw("bicgstab", "diagonal", 1e-10, 20000)
     out = {}
+    # (the multi-system launches run FP64 in HUTI's loop order whatever the
+    # mode, so there is no separate FP64 fast batched line)
     for name, prec, mode in (("fp64_faithful_batched", _lib.FP64, _lib.FAITHFUL),
-                             ("fp64_fast_batched", _lib.FP64, _lib.FAST),
                              ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
         solve_batch(systems[:2], kw, precision=prec, mode=mode)  # warm-up
         t = time.perf_counter()
@@ -163,7 +164,7 @@ def form_batch(args):
                    "systems, as the reference HUTI does on the same matrices (checked against the compiled "
                    "reference); the fast forms confirm convergence on the true residual and restart",
            "speedup_vs_fp64_fast_one_at_a_time": {n: out["fp64_fast_one_at_a_time"]["wall_s"] / out[n]["wall_s"]
-                                                  for n in ("fp64_fast_batched", "mixed_batched")}}
+                                                  for n in ("fp64_faithful_batched", "mixed_batched")}}
     print(json.dumps(rec), flush=True)
     for s in systems:
         s.free()

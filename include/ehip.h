@@ -238,7 +238,10 @@ int ehip_batch_free(ehip_batch *b);
 int ehip_batch_set_lane(ehip_batch *b, int64_t lane_min_dofs, int lane_workers);
 /* reports[nsys]; solutions stay on the device (ehip_batch_get_solution).
  * t_ms: device time of the multi-system launches, or the wall time of the
- * whole batch when some systems took the concurrent lane. */
+ * whole batch when some systems took the concurrent lane.  In the
+ * multi-system launches p->precision selects FP64 (HUTI's loop order,
+ * whatever p->mode says) or mixed precision; p->mode applies to systems on
+ * the concurrent lane, which run ehip_solve. */
 int ehip_batch_solve(ehip_batch *b, const ehip_params *p, ehip_report *reports, double *t_ms);
 int ehip_batch_get_solution(ehip_batch *b, int i, double *x);
 
