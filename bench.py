@@ -130,7 +130,7 @@ def form_batch(args):
     # mode, so there is no separate FP64 fast batched line)
     for name, prec, mode in (("fp64_faithful_batched", _lib.FP64, _lib.FAITHFUL),
                              ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
-        solve_batch(systems[:2], kw, precision=prec, mode=mode)  # warm-up
+        solve_batch(systems, kw, precision=prec, mode=mode)  # warm-up: first-use allocations and copies
         t = time.perf_counter()
         xs, reps, ms = solve_batch(systems, kw, precision=prec, mode=mode)
         wall = time.perf_counter() - t
