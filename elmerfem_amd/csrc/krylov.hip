@@ -1105,6 +1105,17 @@ static void launch_epi_sellb(ehip_system *s, const T *sv, const T *xin, const Ep
              s->sellb_c16, s->sellb_base, s->sell_off, sv, xin, s->nslice, e, st);
 }
 
+// Workgroups of an epilogue SpMV on the SELL-64 layout: sell_grid, but at
+// most 1024 for the FP32 plain-SELL kernels, whose waves then take ~8 slices
+// each and spread their per-slice epilogue set-up (and the vector step
+// after them reduces 4x fewer partials): the fused FP32 CG step on 128^3
+// Hex8 Poisson measured 73.5 against 75.2 us.  The FP64 and node-blocked
+// kernels measured slower that way and keep sell_grid.
+template <typename T>
+static int epi_sell_grid(const ehip_system *s) {
+  return sizeof(T) == 4 && !s->sellb_d ? std::min(sell_grid(s), 1024) : sell_grid(s);
+}
+
 template <int U, class Epi>
 static void launch_epi_sell2(ehip_system *s, const double *sv, const double *xin, const Epi &e, const KState *st) {
   const int g = sell_grid(s);
@@ -1119,7 +1130,7 @@ static void launch_epi_sell2(ehip_system *s, const double *sv, const double *xin
 
 template <typename T, int U, bool C16, class Epi>
 static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
-  const int g = sell_grid(s);
+  const int g = epi_sell_grid<T>(s);
   if (s->prof_start)
     hipExtLaunchKernelGGL((k_spmv_epi_sell<T, 256, U, C16, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
                           s->prof_stop, 0, s->rows, s->n, s->sell_row, s->sell_c32, s->sell_c16, s->sell_base,
@@ -1133,7 +1144,7 @@ static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi
 template <typename T>
 static int epi_grid(const ehip_system *s, const SpmvPlan &p) {
   constexpr bool F32 = sizeof(T) == 4;
-  return ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) ? sell_grid(s) : p.grid;
+  return ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) ? epi_sell_grid<T>(s) : p.grid;
 }
 
 // SpMV with epilogue on the system's active format (SELL-64 or CSR stream);
