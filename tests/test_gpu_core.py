@@ -168,6 +168,8 @@ def test_spmv_long_rows(ehip):
     assert s.spmv_config(0)["format"] == "csr"
     with pytest.raises(_lib.EhipError):
         s.spmv_set_format(_lib.SPMV_SELL)
+    with pytest.raises(_lib.EhipError):
+        s.spmv_sell_layout()
     u = rng.standard_normal(n)
     assert np.allclose(s.spmv(u), dense(rows, cols, vals) @ u, rtol=1e-13, atol=1e-12)
 
