@@ -130,9 +130,9 @@ def form_batch(args):
     # mode, so there is no separate FP64 fast batched line)
     for name, prec, mode in (("fp64_faithful_batched", _lib.FP64, _lib.FAITHFUL),
                              ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
-        solve_batch(systems, kw, precision=prec, mode=mode)  # warm-up: first-use allocations and copies
+        solve_batch(systems, kw, precision=prec, mode=mode, true_residual_stop=True)  # warm-up: first-use work
         t = time.perf_counter()
-        xs, reps, ms = solve_batch(systems, kw, precision=prec, mode=mode)
+        xs, reps, ms = solve_batch(systems, kw, precision=prec, mode=mode, true_residual_stop=True)
         wall = time.perf_counter() - t
         it = np.array([r["iters"] for r in reps], dtype=np.float64)
         conv = sum(1 for r in reps if r["info"] == 1)
@@ -144,7 +144,7 @@ def form_batch(args):
         t = time.perf_counter()
         it_seq, conv = [], 0
         for s in systems:
-            _, r = s.solve(kw, precision=_lib.FP64, mode=mode, return_x=False)
+            _, r = s.solve(kw, precision=_lib.FP64, mode=mode, return_x=False, true_residual_stop=True)
             it_seq.append(r["iters"])
             conv += r["info"] == 1
         wall = time.perf_counter() - t
@@ -197,19 +197,20 @@ def form_serve(args):
     # timing (first-use work -- FP32 and SELL value copies, workspace, the
     # recorded iteration graphs -- happens once per system, not per request)
     for s in pool:
-        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False)
-        s.solve(kw, precision=_lib.MIXED_IR, mode=_lib.FAST, return_x=False)
+        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False, true_residual_stop=True)
+        s.solve(kw, precision=_lib.MIXED_IR, mode=_lib.FAST, return_x=False, true_residual_stop=True)
     log("pool warm (%.1fs)" % (time.perf_counter() - t0))
     # one at a time, plain FP64 CG
     t = time.perf_counter()
     seq_lat = []
     for s in pool:
         t1 = time.perf_counter()
-        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False)
+        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False, true_residual_stop=True)
         seq_lat.append(time.perf_counter() - t1)
     seq_rate = P / (time.perf_counter() - t)
     log("one-at-a-time FP64: %.1f solves/s" % seq_rate)
-    srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)))
+    srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)),
+                 true_residual_stop=True)
     if args.lane_workers > 0:
         srv.set_lane(args.lane_dofs, args.lane_workers)
     R = args.requests
@@ -328,7 +329,8 @@ def form_elastic(args):
                              ("mixed", _lib.MIXED_IR, _lib.FAST)):
         sys_.set_values(vals, rhs)
         t = time.perf_counter()
-        x, r = sys_.solve(kw, precision=prec, mode=mode, inner_tol=args.inner_tol if prec == _lib.MIXED_IR else 0.0)
+        x, r = sys_.solve(kw, precision=prec, mode=mode, inner_tol=args.inner_tol if prec == _lib.MIXED_IR else 0.0,
+                          true_residual_stop=True)
         wall = time.perf_counter() - t
         xs[name] = x
         out[name] = {"iters": r["iters"], "ms": 1000.0 * wall, "info": r["info"], "res_true": r["res_true"],
@@ -421,14 +423,22 @@ def main():
     copy_gbs, _ = copy_bench(1 << 30, reps=20)
     forms = {}
 
-    def run(name, precision, mode, reps, profile_every=0):
+    sols = {}
+
+    def run(name, precision, mode, reps, profile_every=0, keep_x=False):
+        """Solves to 1e-10 on both the scaled (HUTI) and the unscaled FP64
+        residual; keep_x keeps the last solution (its copy to the host is
+        outside the timed region)."""
         out = []
         for _ in range(reps):
             t = time.perf_counter()
             x, rep = sys_.solve(kw, precision=precision, mode=mode, return_x=False, profile_every=profile_every,
-                                inner_tol=args.inner_tol if precision == _lib.MIXED_IR else 0.0)
+                                inner_tol=args.inner_tol if precision == _lib.MIXED_IR else 0.0,
+                                true_residual_stop=True)
             rep["wall_ms"] = 1000.0 * (time.perf_counter() - t)
             out.append(rep)
+            if keep_x:
+                sols[name] = sys_.solution()
             log("%-14s info=%d iters=%d outer=%d res_scaled=%.3e res_true=%.3e wall=%.1f ms krylov=%.1f ms" % (
                 name, rep["info"], rep["iters"], rep["outer_iters"], rep["res_scaled"], rep["res_true"],
                 rep["wall_ms"], rep["t_krylov_ms"]))
@@ -439,7 +449,7 @@ def main():
     spmv64_us = sys_.spmv_bench(0, reps=50)
     if not args.skip_forms:
         for name, mode in (("fp64_faithful", _lib.FAITHFUL), ("fp64_fast", _lib.FAST)):
-            reps = run(name, _lib.FP64, mode, 2)
+            reps = run(name, _lib.FP64, mode, 2, keep_x=True)
             r = reps[-1]
             forms[name] = {"iters": r["iters"], "ms": r["wall_ms"], "krylov_ms": r["t_krylov_ms"],
                            "res_scaled": r["res_scaled"], "res_true": r["res_true"], "info": r["info"],
@@ -451,7 +461,7 @@ def main():
     run("mixed_ir(warm)", _lib.MIXED_IR, _lib.FAST, max(0, args.warmup - 1))
     # timed steps run as a user would (graph-replayed iteration blocks); one
     # more solve samples every 4th fused SpMV launch with kernel-bound events
-    steps = run("mixed_ir", _lib.MIXED_IR, _lib.FAST, args.steps)
+    steps = run("mixed_ir", _lib.MIXED_IR, _lib.FAST, args.steps, keep_x=True)
     sampled = run("mixed_ir(sampled)", _lib.MIXED_IR, _lib.FAST, 1, profile_every=4)
     wall = np.array([r["wall_ms"] for r in steps])
     iters = np.array([r["iters"] for r in steps])
@@ -464,6 +474,10 @@ def main():
     if "fp64_faithful" in forms:
         forms["mixed_ir"]["speedup_vs_fp64_faithful"] = forms["fp64_faithful"]["ms"] / ms_per_step
         forms["mixed_ir"]["speedup_vs_fp64_fast"] = forms["fp64_fast"]["ms"] / ms_per_step
+        # error of each solution against the FP64 (HUTI loop order) solution
+        ref = sols["fp64_faithful"]
+        for name in ("fp64_fast", "mixed_ir"):
+            forms[name]["rel_err_vs_fp64"] = float(np.abs(sols[name] - ref).max() / np.abs(ref).max())
 
     # roofline of the dominant kernel: the fused FP32 SpMV of the inner CG,
     # its launches sampled with HIP events on the system's stream in the timed solves

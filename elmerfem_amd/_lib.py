@@ -33,7 +33,7 @@ class Params(C.Structure):
     _fields_ = [("method", C.c_int), ("precond", C.c_int), ("tol", C.c_double), ("maxit", C.c_int),
                 ("scaling", C.c_int), ("precision", C.c_int), ("mode", C.c_int), ("inner_tol", C.c_double),
                 ("inner_maxit", C.c_int), ("check_every", C.c_int), ("divtol", C.c_double),
-                ("record_history", C.c_int), ("profile_every", C.c_int)]
+                ("record_history", C.c_int), ("profile_every", C.c_int), ("true_residual_stop", C.c_int)]
 
 
 class Report(C.Structure):

@@ -16,7 +16,7 @@ from . import _lib
 
 
 def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=0.0, inner_maxit=0,
-                check_every=0, lane_dofs=None, lane_workers=None):
+                check_every=0, lane_dofs=None, lane_workers=None, true_residual_stop=False):
     """Returns (solutions, reports, ms).  With lane_workers > 0, systems of
     at least lane_dofs unknowns (default 40000) are solved whole on that many
     concurrent threads instead of in the multi-system launches."""
@@ -29,7 +29,8 @@ def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=
             _lib.check(L.ehip_batch_set_lane(h, 40000 if lane_dofs is None else int(lane_dofs),
                                              0 if lane_workers is None else int(lane_workers)),
                        "ehip_batch_set_lane")
-        p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every)
+        p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every,
+                                                                true_residual_stop=true_residual_stop)
         reps = (_lib.Report * len(systems))()
         ms = C.c_double()
         _lib.check(L.ehip_batch_solve(h, C.byref(p), reps, C.byref(ms)), "ehip_batch_solve")
@@ -47,10 +48,11 @@ class Server:
     """Continuous batching over device-resident systems."""
 
     def __init__(self, kw, precision=_lib.MIXED_IR, max_slots=256, mem_budget=0, inner_tol=0.0, inner_maxit=0,
-                 check_every=0):
+                 check_every=0, true_residual_stop=False):
         L = _lib.init()
         self.p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, _lib.FAST, inner_tol, inner_maxit,
-                                                                     check_every)
+                                                                     check_every,
+                                                                     true_residual_stop=true_residual_stop)
         self._h = C.c_void_p()
         _lib.check(L.ehip_server_create(C.byref(self.p), max_slots, int(mem_budget), C.byref(self._h)),
                    "ehip_server_create")

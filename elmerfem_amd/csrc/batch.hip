@@ -10,8 +10,8 @@
 // work list and the slot table between replays, so the graph never changes.
 //
 // Per-slot reductions are deterministic: each chunk publishes its partial
-// sums, an integer ticket per slot finds the chunk that finishes last, and
-// that workgroup adds the slot's chunk partials in chunk order.
+// sums with plain stores, and a per-slot finalize launch (bk_fin) adds the
+// slot's chunk partials in chunk order and runs the scalar update.
 //
 // FP64 mode runs HUTI's CG / BiCGStab loop order per slot (true residual
 // every iteration, hutiter/src/huti_cg.src:145-227, huti_bicgstab.src:
@@ -67,6 +67,7 @@ struct SlotPtrs {
   double *b, *x;      // scaled right-hand side and solution (the system's own buffers)
   double *w64;        // NV64 vectors of n doubles
   float *w32;         // NV32 vectors of n floats
+  const double *du;   // the scaling's D when the unscaled residual is tested, else null
   int n, ck0, nck, method;
 };
 
@@ -74,10 +75,19 @@ struct BSys {
   double rho, rho_new, alpha, omega, rhsnorm, res, scale, tol, snorm, scale_prev;
   float rho32, rho_new32, alpha32, omega32, beta32, rhsnorm32, res32, itol;
   int iter, info, done, phase, maxit, outer, inner_iter, inner_total, inner_maxit, snorm_exit;
-  unsigned ticket;
   int inner_bad;   // the inner solve ended on a non-finite residual: its correction is dropped
   int bad_count;   // consecutive such inner solves
+  // unscaled residual test (tol_u > 0): ||b - A x|| / ||b|| = ||r_s / D|| / ||b_s / D||
+  double tol_u, bb_u, res_u;
 };
+
+// Records the slot's unscaled residual from the sum of squares of r_s / D and
+// tells whether it meets tol_u (true when the test is off).
+__device__ __forceinline__ bool slot_unscaled_ok(BSys &st, double sum_u) {
+  if (!(st.tol_u > 0.0)) return true;
+  st.res_u = sqrt(sum_u / st.bb_u);
+  return st.res_u < st.tol_u;
+}
 
 struct BatchArgs {
   const Chunk *chunks;
@@ -267,27 +277,29 @@ struct OpBase {
 struct Ctx64 {
   Vecs<double> v;
   double alpha, beta, omega;
+  const double *du;
 };
 
 __device__ __forceinline__ Ctx64 ctx64(const BSys &st, const SlotPtrs &sp) {
   Ctx64 c;
   c.v = slot_vecs<double>(sp.w64, sp.n, sp.method, sp.x, sp.b);
+  c.du = sp.du;
   c.alpha = st.alpha;
   c.omega = st.omega;
   c.beta = 0.0;
   return c;
 }
 
-// r = b - A x ; z = r/dg ; r.z, r.r, b.b
+// r = b - A x ; z = r/dg ; r.z, r.r, b.b, (b/D).(b/D)
 struct OpInit : OpBase<double> {
   static constexpr bool SPMV = true;
-  static constexpr int NV = 3;
+  static constexpr int NV = 4;
   using Ctx = Ctx64;
   __device__ bool active(const BSys &st) const { return st.phase == PH_INIT; }
   __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
   __device__ const double *xin(const Ctx &c) const { return c.v.x; }
   __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
-  __device__ void row(Ctx &c, int i, double s, double (&acc)[3]) const {
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[4]) const {
     const double bi = c.v.b[i], ri = bi - s;
     c.v.r[i] = ri;
     const double zi = ri / c.v.dg[i];
@@ -295,9 +307,12 @@ struct OpInit : OpBase<double> {
     acc[0] += ri * zi;
     acc[1] += ri * ri;
     acc[2] += bi * bi;
+    const double ui = c.du ? bi / c.du[i] : bi;
+    acc[3] += ui * ui;
   }
-  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &sp, double (&tot)[3]) const {
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &sp, double (&tot)[4]) const {
     st.rhsnorm = sqrt(tot[2]);
+    st.bb_u = tot[3];
     st.rho = sp.method == EHIP_CG ? tot[0] : tot[1];
     st.phase = PH_START;
   }
@@ -368,7 +383,7 @@ struct OpCgDa : OpBase<double> {  // x += alpha p ; r -= alpha q ; z = r/dg ; rh
 
 struct OpCgRes : OpBase<double> {  // ||A x - b|| / ||b|| ; p = z + (rho_new/rho) p
   static constexpr bool SPMV = true;
-  static constexpr int NV = 1;
+  static constexpr int NV = 2;
   using Ctx = Ctx64;
   __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
   __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const {
@@ -378,15 +393,17 @@ struct OpCgRes : OpBase<double> {  // ||A x - b|| / ||b|| ; p = z + (rho_new/rho
   }
   __device__ const double *xin(const Ctx &c) const { return c.v.x; }
   __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
-  __device__ void row(Ctx &c, int i, double s, double (&acc)[1]) const {
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[2]) const {
     const double ti = s - c.v.b[i];
     acc[0] += ti * ti;
+    const double ui = c.du ? ti / c.du[i] : ti;
+    acc[1] += ui * ui;
     c.v.p[i] = c.v.z[i] + c.beta * c.v.p[i];
   }
-  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[1]) const {
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[2]) const {
     const double res = sqrt(tot[0]) / st.rhsnorm;
     st.res = res;
-    if (res < st.tol) { slot_done(a, st, 1); return; }
+    if (res < st.tol && slot_unscaled_ok(st, tot[1])) { slot_done(a, st, 1); return; }
     st.rho = st.rho_new;
     st.iter += 1;
     if (st.iter > st.maxit) slot_done(a, st, 2);
@@ -477,7 +494,7 @@ struct OpBiXr : OpBase<double> {  // x += alpha t1v + omega t2v ; r = s - omega 
 
 struct OpBiRes : OpBase<double> {  // ||A x - b||/||b|| ; p = r + beta (p - omega v) ; t1v = p/dg
   static constexpr bool SPMV = true;
-  static constexpr int NV = 1;
+  static constexpr int NV = 2;
   using Ctx = Ctx64;
   __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && !st.snorm_exit; }
   __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const {
@@ -487,17 +504,19 @@ struct OpBiRes : OpBase<double> {  // ||A x - b||/||b|| ; p = r + beta (p - omeg
   }
   __device__ const double *xin(const Ctx &c) const { return c.v.x; }
   __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
-  __device__ void row(Ctx &c, int i, double s, double (&acc)[1]) const {
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[2]) const {
     const double ti = s - c.v.b[i];
     acc[0] += ti * ti;
+    const double ui = c.du ? ti / c.du[i] : ti;
+    acc[1] += ui * ui;
     const double pi = c.v.r[i] + c.beta * (c.v.p[i] - c.omega * c.v.v[i]);
     c.v.p[i] = pi;
     c.v.t1v[i] = pi / c.v.dg[i];
   }
-  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[1]) const {
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[2]) const {
     const double res = sqrt(tot[0]) / st.rhsnorm;
     st.res = res;
-    if (res < st.tol) { slot_done(a, st, 1); return; }
+    if (res < st.tol && slot_unscaled_ok(st, tot[1])) { slot_done(a, st, 1); return; }
     if (st.omega == 0.0) { slot_done(a, st, 37); return; }
     st.iter += 1;
     if (st.iter > st.maxit) slot_done(a, st, 2);
@@ -511,6 +530,7 @@ struct OpBiRes : OpBase<double> {  // ||A x - b||/||b|| ; p = r + beta (p - omeg
 struct CtxM {
   Vecs<float> v;   // inner FP32 vectors; v.x = d (the correction), v.b = normalised residual
   double *t64, *x64, *b64, *dg64;
+  const double *du;
   double scale;
   double ratio;  // previous residual scale / current one (0 before the first replacement)
   float alpha, beta, omega;
@@ -526,6 +546,7 @@ __device__ __forceinline__ CtxM ctxm(const BSys &st, const SlotPtrs &sp) {
   c.dg64 = sp.w64 + (size_t)sp.n;
   c.x64 = sp.x;
   c.b64 = sp.b;
+  c.du = sp.du;
   c.scale = st.scale;
   c.ratio = st.scale_prev > 0.0 ? st.scale_prev / st.scale : 0.0;
   c.alpha = st.alpha32;
@@ -538,25 +559,32 @@ __device__ __forceinline__ CtxM ctxm(const BSys &st, const SlotPtrs &sp) {
 // t = b - A x (FP64), the refinement's true residual
 struct OpRes : OpBase<double> {
   static constexpr bool SPMV = true;
-  static constexpr int NV = 2;
+  static constexpr int NV = 4;
   using Ctx = CtxM;
   __device__ bool active(const BSys &st) const { return st.phase == PH_RES; }
   __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctxm(st, sp); }
   __device__ const double *xin(const Ctx &c) const { return c.x64; }
   __device__ const double *vals(const SlotPtrs &sp) const { return sp.vals; }
-  __device__ void row(Ctx &c, int i, double s, double (&acc)[2]) const {
+  __device__ void row(Ctx &c, int i, double s, double (&acc)[4]) const {
     const double bi = c.b64[i], ti = bi - s;
     c.t64[i] = ti;
     acc[0] += ti * ti;
     acc[1] += bi * bi;
+    const double d = c.du ? c.du[i] : 1.0;
+    const double ui = ti / d, vi = bi / d;
+    acc[2] += ui * ui;
+    acc[3] += vi * vi;
   }
-  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[2]) const {
-    if (st.outer == 0) st.rhsnorm = sqrt(tot[1]);
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[4]) const {
+    if (st.outer == 0) {
+      st.rhsnorm = sqrt(tot[1]);
+      st.bb_u = tot[3];
+    }
     const double nt = sqrt(tot[0]);
     const double res = nt / st.rhsnorm;
     st.res = res;
     st.outer += 1;
-    if (res < st.tol) slot_done(a, st, 1);
+    if (res < st.tol && slot_unscaled_ok(st, tot[2])) slot_done(a, st, 1);
     else if (!(res == res)) slot_done(a, st, 3);
     else if (st.inner_total >= st.maxit) slot_done(a, st, 2);
     else {
@@ -972,6 +1000,8 @@ static void enqueue_iteration(ehip_batch *b) {
 }
 
 static int build_graph(ehip_batch *b, int iters) {
+  // nothing recorded earlier may still be queued when its graph goes
+  EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
   if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
   if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
   // built node by node, not by stream capture, which would make the runtime
@@ -1000,6 +1030,7 @@ static int upload_worklist(ehip_batch *b) {
       ehip_system *s = sl.s;
       sp.rows = s->rows; sp.cols = s->cols; sp.vals = s->vals; sp.vals32 = s->vals32;
       sp.b = s->rhs; sp.x = s->x; sp.w64 = sl.w64; sp.w32 = sl.w32;
+      sp.du = (b->p.true_residual_stop && b->p.scaling) ? s->dscale : nullptr;
       sp.n = s->n; sp.method = b->method;
       sp.ck0 = (int)all.size();
       sp.nck = (int)sl.ck.size();
@@ -1121,6 +1152,8 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   BSys z;
   std::memset(&z, 0, sizeof(z));
   z.tol = b->p.tol;
+  z.tol_u = (b->p.true_residual_stop && b->p.scaling) ? b->p.tol : 0.0;
+  z.res_u = -1.0;
   z.maxit = b->p.maxit;
   // CG: residual drop between replacements; BiCGStab: restarted inner tolerance
   z.itol = (float)(b->p.inner_tol > 0 ? b->p.inner_tol : (b->p.method == EHIP_CG ? 1e-2 : 1e-4));
@@ -1165,13 +1198,20 @@ static int engine_retire(ehip_batch *b, int slot, ehip_report *r) {
   return EHIP_OK;
 }
 
-// Replays the iteration graph once (graph_iters iterations of every slot).
+// Replays the iteration graph once (graph_iters iterations of every slot),
+// or launches the same kernels one by one when graphs are off.
 static int engine_step(ehip_batch *b) {
-  if (!b->graph_ok) {
-    int rc = build_graph(b, b->p.check_every > 0 ? b->p.check_every : 8);
-    if (rc) return rc;
+  const int iters = b->p.check_every > 0 ? b->p.check_every : 8;
+  if (graphs_enabled()) {
+    if (!b->graph_ok) {
+      int rc = build_graph(b, iters);
+      if (rc) return rc;
+    }
+    EHIP_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    for (int k = 0; k < iters; ++k) enqueue_iteration(b);
+    EHIP_HIP_TRY(take_launch_error());
   }
-  EHIP_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   EHIP_HIP_TRY(hipMemcpyAsync(b->h_active, b->active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   EHIP_HIP_TRY(hipMemcpyAsync(b->h_st, b->st, sizeof(BSys) * b->capacity, hipMemcpyDeviceToHost, b->stream));
   EHIP_HIP_TRY(hipStreamSynchronize(b->stream));

@@ -54,11 +54,14 @@ static_assert(4 * kEpiStride <= kVecPartOff, "epilogue partials overlap the vect
 // ru: reliable-update threshold of the mixed-precision CG (0 = off): the
 // FP32 iteration stops for an FP64 residual replacement once its recurrence
 // residual falls below it.
+// tol_u: tolerance of the unscaled residual ||b - A x|| / ||b|| (0 = not
+// tested), evaluated on the scaled system as ||r_s / D|| / ||b_s / D||;
+// bb_u = ||b_s / D||^2; res_u: the last such residual.
 template <typename T>
 struct KStateT {
   T rho, rho_new, alpha, beta, omega, pq, rr, rhsnorm, res, ts, tt, snorm, res_true, scale, ru;
   int iter, info, done, maxit, snorm_exit, rec_conv, pad0, pad1;
-  T tol, divtol;
+  T tol, divtol, tol_u, bb_u, res_u;
   unsigned int ticket[4];
 };
 using KState = KStateT<double>;

@@ -21,7 +21,9 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "ehip_internal.h"
@@ -32,6 +34,16 @@
 namespace ehip {
 
 thread_local GraphRec *tl_graph_rec = nullptr;
+thread_local hipError_t tl_launch_err = hipSuccess;
+
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char *g = std::getenv("EHIP_GRAPHS");
+    if (g && *g) return g[0] != '0';
+    return std::getenv("ROCPROF_OUTPUT_PATH") == nullptr;
+  }();
+  return on;
+}
 
 // ---------------------------------------------------------------- scaling
 
@@ -260,6 +272,18 @@ __device__ __forceinline__ bool solve_done(const KStateT<T> *st) {
   return __builtin_amdgcn_readfirstlane(st->done) != 0;
 }
 
+// The unscaled relative residual ||b - A x|| / ||b|| from the sum of squares
+// of r_s / D on the scaled system (r_s = D0 (b - A x) / beta and D = D0 beta,
+// so r_s / D = (b - A x) / beta^2, and likewise b_s / D = b / beta^2).
+// Records it and tells whether it meets tol_u; true when the test is off.
+template <typename T>
+__device__ __forceinline__ bool unscaled_ok(KStateT<T> *st, T sum_u) {
+  if (!(st->tol_u > (T)0)) return true;
+  const T ru = sqrt(sum_u / st->bb_u);
+  st->res_u = ru;
+  return ru < st->tol_u;
+}
+
 #define EHIP_EPI_MEMBERS(T) \
   Vecs<T> v;                \
   KStateT<T> *st;           \
@@ -461,36 +485,48 @@ __global__ __launch_bounds__(256) void k_fin(Fin f, int g, const KStateT<T> *che
 }
 
 // true residual t = b - A x ; restart r = t, z = t/dg ; partials t.t, t.z
+// and (unscaled test on) (t/D).(t/D)
 template <typename T>
 struct EpiTrue {
-  static constexpr int NV = 2;
+  static constexpr int NV = 3;
   EHIP_EPI_MEMBERS(T)
+  const T *du;  // the scaling's D, or null when the unscaled test is off
   __device__ void init() {}
-  struct Pre { T b, dg; };
-  __device__ Pre load(int i) const { return Pre{v.b[i], v.dg[i]}; }
-  __device__ void row(int i, T s, const Pre &pr, T (&acc)[2]) {
+  struct Pre { T b, dg, d; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.dg[i], du ? du[i] : (T)1}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[3]) {
     const T ti = pr.b - s;
     v.r[i] = ti;
     const T zi = ti / pr.dg;
     v.z[i] = zi;
     acc[0] += ti * ti;
     acc[1] += ti * zi;
+    const T ui = ti / pr.d;
+    acc[2] += ui * ui;
   }
-  __device__ void finish(T (&tot)[2]) {
+  __device__ void finish(T (&tot)[3]) {
     if (threadIdx.x == 0) {
       const T rt = sqrt(tot[0]) / st->rhsnorm;
+      const T rho_prev = st->rho_new;  // r.z before the last step (FinCgFast2)
       st->res_true = rt;
       st->rho = tot[1];
       st->rr = tot[0];
       st->beta = (T)0;
       st->done = 0;
       st->rec_conv = 0;
-      if (rt < st->tol) {
+      const bool ok_s = rt < st->tol;
+      if (ok_s && unscaled_ok(st, tot[2])) {
         st->info = 1;
         st->done = 1;
       } else if (st->iter >= st->maxit) {
         st->info = 2;
         st->done = 1;
+      } else if (ok_s) {
+        // the scaled test holds but the unscaled residual still misses tol_u:
+        // the recurrence test is tightened by the observed ratio of the two
+        // and the CG direction continues through this residual replacement
+        st->tol = rt * (st->tol_u / st->res_u);
+        st->beta = rho_prev != (T)0 ? tot[1] / rho_prev : (T)0;
       }
     }
   }
@@ -530,12 +566,13 @@ __global__ __launch_bounds__(kVecBlock) void k_cg_da(int n, Vecs<T> v, KStateT<T
 
 template <typename T>
 struct EpiCgResidual {
-  static constexpr int NV = 1;
+  static constexpr int NV = 2;
   EHIP_EPI_MEMBERS(T)
   double *hist;
   const T *rn_part;  // Kda's partials of rho_new = r.z
   int rn_g;
   T beta;
+  const T *du;  // the scaling's D, or null when the unscaled test is off
   // rho_new summed from Kda's partials by every workgroup in the same order
   // (workgroup 0 records it for the finalize), in place of a finalize launch
   __device__ void init() {
@@ -544,20 +581,22 @@ struct EpiCgResidual {
     if (blockIdx.x == 0 && threadIdx.x == 0) st->rho_new = rn[0];
     beta = rn[0] / st->rho;
   }
-  struct Pre { T b, z, p; };
-  __device__ Pre load(int i) const { return Pre{v.b[i], v.z[i], v.p[i]}; }
-  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+  struct Pre { T b, z, p, d; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.z[i], v.p[i], du ? du[i] : (T)1}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[2]) {
     const T ti = s - pr.b;
     acc[0] += ti * ti;
+    const T ui = ti / pr.d;
+    acc[1] += ui * ui;
     v.p[i] = pr.z + beta * pr.p;
   }
-  __device__ void finish(T (&tot)[1]) {
+  __device__ void finish(T (&tot)[2]) {
     if (threadIdx.x != 0) return;
     const T res = sqrt(tot[0]) / st->rhsnorm;
     st->res = res;
     const int it = st->iter;  // HUTI iter_count, 1-based
     if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
-    if (res < st->tol) {
+    if (res < st->tol && unscaled_ok(st, tot[1])) {
       st->info = 1;
       st->done = 1;
       return;
@@ -731,30 +770,33 @@ struct FinBiXr {
 
 template <typename T>
 struct EpiBiResidual {
-  static constexpr int NV = 1;
+  static constexpr int NV = 2;
   EHIP_EPI_MEMBERS(T)
   double *hist;
   T beta, omega;
+  const T *du;  // the scaling's D, or null when the unscaled test is off
   __device__ void init() {
     omega = st->omega;
     beta = (st->rho_new * st->alpha) / (st->rho * omega);
   }
-  struct Pre { T b, r, p, v, dg; };
-  __device__ Pre load(int i) const { return Pre{v.b[i], v.r[i], v.p[i], v.v[i], v.dg[i]}; }
-  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+  struct Pre { T b, r, p, v, dg, d; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], v.r[i], v.p[i], v.v[i], v.dg[i], du ? du[i] : (T)1}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[2]) {
     const T ti = s - pr.b;
     acc[0] += ti * ti;
+    const T ui = ti / pr.d;
+    acc[1] += ui * ui;
     const T pi = pr.r + beta * (pr.p - omega * pr.v);
     v.p[i] = pi;
     v.t1v[i] = pi / pr.dg;
   }
-  __device__ void finish(T (&tot)[1]) {
+  __device__ void finish(T (&tot)[2]) {
     if (threadIdx.x != 0) return;
     const T res = sqrt(tot[0]) / st->rhsnorm;
     st->res = res;
     const int it = st->iter;
     if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
-    if (res < st->tol) {
+    if (res < st->tol && unscaled_ok(st, tot[1])) {
       st->info = 1;
       st->done = 1;
       return;
@@ -847,8 +889,10 @@ __global__ void k_zero2(int n, T *a, T *b) {
 }
 
 template <typename T>
-__global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int iter0) {
+__global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int iter0, T tol_u) {
   st->tol = tol;
+  st->tol_u = tol_u;
+  st->res_u = (T)-1;
   st->maxit = maxit;
   st->divtol = divtol;
   st->iter = iter0;
@@ -908,23 +952,27 @@ __global__ void k_correct(int n, double *x, const float *__restrict__ d, double 
 }
 
 // FP64 outer residual: t = b - A x ; partial t.t -> res_true, scale = ||t||
+// (and (t/D).(t/D) for the unscaled test)
 struct EpiOuter {
-  static constexpr int NV = 1;
+  static constexpr int NV = 2;
   double *b, *t;
   KState *st;
   double *partials;
   int stride;
   unsigned *ticket;
   double *hist;
+  const double *du;  // the scaling's D, or null when the unscaled test is off
   __device__ void init() {}
-  struct Pre { double b; };
-  __device__ Pre load(int i) const { return Pre{b[i]}; }
-  __device__ void row(int i, double s, const Pre &pr, double (&acc)[1]) {
+  struct Pre { double b, d; };
+  __device__ Pre load(int i) const { return Pre{b[i], du ? du[i] : 1.0}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[2]) {
     const double ti = pr.b - s;
     t[i] = ti;
     acc[0] += ti * ti;
+    const double ui = ti / pr.d;
+    acc[1] += ui * ui;
   }
-  __device__ void finish(double (&tot)[1]) {
+  __device__ void finish(double (&tot)[2]) {
     if (threadIdx.x != 0) return;
     const double nt = sqrt(tot[0]);
     st->scale = nt;
@@ -934,7 +982,7 @@ struct EpiOuter {
     const int k = st->pad0;  // outer step
     if (hist && k < st->maxit) hist[k] = res;
     st->pad0 = k + 1;
-    if (res < st->tol) {
+    if (res < st->tol && unscaled_ok(st, tot[1])) {
       st->info = 1;
       st->done = 1;
     } else if (!(res == res)) {
@@ -951,7 +999,7 @@ struct EpiOuter {
 // search direction kept: rho = r32.z32 and beta = rho / (r.z of the step
 // before), so the next FP32 step continues the same CG sequence.
 struct EpiRU {
-  static constexpr int NV = 2;
+  static constexpr int NV = 3;
   const double *b;
   Vecs<float> v;
   KState *st;
@@ -961,10 +1009,11 @@ struct EpiRU {
   unsigned *ticket;
   double *hist;
   double delta;  // next replacement when the FP32 residual has dropped by this factor
+  const double *du;  // the scaling's D, or null when the unscaled test is off
   __device__ void init() {}
-  struct Pre { double b; float dg; };
-  __device__ Pre load(int i) const { return Pre{b[i], v.dg[i]}; }
-  __device__ void row(int i, double s, const Pre &pr, double (&acc)[2]) {
+  struct Pre { double b, d; float dg; };
+  __device__ Pre load(int i) const { return Pre{b[i], du ? du[i] : 1.0, v.dg[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[3]) {
     const double t = pr.b - s;
     const float r = (float)t;
     const float z = r / pr.dg;
@@ -973,8 +1022,10 @@ struct EpiRU {
     v.x[i] = 0.f;
     acc[0] += t * t;
     acc[1] += (double)(r * z);
+    const double u = t / pr.d;
+    acc[2] += u * u;
   }
-  __device__ void finish(double (&tot)[2]) {
+  __device__ void finish(double (&tot)[3]) {
     if (threadIdx.x != 0) return;
     const double res = sqrt(tot[0]) / st->rhsnorm;
     st->res_true = res;
@@ -987,13 +1038,17 @@ struct EpiRU {
     st32->rho = rz;
     st32->beta = rho_prev != 0.f ? rz / rho_prev : 0.f;
     st32->res = (float)res;
-    if (res < st->tol) {
+    const bool ok_s = res < st->tol;
+    if (ok_s && unscaled_ok(st, tot[2])) {
       st->info = 1;
       st->done = 1;
     } else if (!(res == res)) {
       st->info = 3;
       st->done = 1;
     } else {
+      // scaled test met, unscaled not yet: the FP32 sequence's own stop
+      // threshold is tightened by the observed ratio of the two residuals
+      if (ok_s) st32->tol = fminf(st32->tol, (float)(res * (st->tol_u / st->res_u)));
       st32->ru = (float)(res * delta);
       st32->rec_conv = 0;
       st32->done = 0;
@@ -1032,10 +1087,14 @@ struct EpiResid {
   }
 };
 
-__global__ __launch_bounds__(kVecBlock) void k_sumsq(int n, const double *x, double *partials, unsigned *ticket,
-                                                     double *out) {
+// partials of sum_i (x_i / d_i)^2 (d null: sum_i x_i^2)
+__global__ __launch_bounds__(kVecBlock) void k_sumsq(int n, const double *x, const double *d, double *partials,
+                                                     unsigned *ticket, double *out) {
   double acc[1] = {0.0};
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc[0] += x[i] * x[i];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double u = d ? x[i] / d[i] : x[i];
+    acc[0] += u * u;
+  }
   (void)ticket;
   (void)out;
   block_publish<double, 1>(acc, partials, gridDim.x);
@@ -1201,10 +1260,11 @@ static void fin(ehip_system *s, const Fin &f, int g, const KStateT<T> *st) {
   launch_k((k_fin<T, Fin>), dim3(1), dim3(256), s->stream, f, g, st);
 }
 
-// *out = sum of x_i^2 (two launches, no in-kernel fence)
-static void sumsq(ehip_system *s, int n, const double *x, double *out) {
+// *out = sum of (x_i / d_i)^2, or of x_i^2 without d (two launches, no
+// in-kernel fence)
+static void sumsq(ehip_system *s, int n, const double *x, double *out, const double *d = nullptr) {
   const int g = vec_grid(n);
-  hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(kVecBlock), 0, s->stream, n, x, s->partials, (unsigned *)nullptr,
+  hipLaunchKernelGGL(k_sumsq, dim3(g), dim3(kVecBlock), 0, s->stream, n, x, d, s->partials, (unsigned *)nullptr,
                      (double *)nullptr);
   hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s->stream, s->partials, g, out);
 }
@@ -1220,6 +1280,8 @@ struct SolveCtx {
   int method, precond, maxit, check, mode;
   bool check_set;  // check_every given by the caller (else the loops choose)
   double tol, divtol;
+  double tol_u;        // unscaled-residual tolerance (0: HUTI's scaled test alone)
+  const double *du;    // the scaling's D when tol_u > 0 (FP64 loops only)
   double *hist;  // device, may be null
   int spmv;
   bool prof;     // sample the SpMV launches of this loop
@@ -1253,14 +1315,19 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
   const int pg = kEpiStride;
   int rc;
   const int iter0 = (c.mode == EHIP_FAITHFUL || c.method == EHIP_BICGSTAB) ? 1 : 0;
+  // the unscaled test runs on FP64 residuals only
+  const T *du = std::is_same<T, double>::value ? reinterpret_cast<const T *>(c.du) : nullptr;
+  const T tol_u = du ? (T)c.tol_u : (T)0;
   if (init_residual) {
-    hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0);
+    hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0,
+                       tol_u);
     EpiInit<T> e{v, st, part, pg, &st->ticket[0]};
     launch_epi_fin<T>(s, plan, vals, v.x, e, (const KStateT<T> *)nullptr);
     c.spmv++;
   } else {
     // mixed inner solve: k_to32 prepared r, z, rho; x (= d) is zero
-    hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0);
+    hipLaunchKernelGGL(k_init_state<T>, dim3(1), dim3(1), 0, s->stream, st, (T)c.tol, c.maxit, (T)c.divtol, iter0,
+                       tol_u);
   }
   if (c.method == EHIP_CG) {
     if (c.mode == EHIP_FAITHFUL) {
@@ -1273,7 +1340,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
           if (c.prof) prof_end(s);
           hipLaunchKernelGGL(k_cg_da<T>, dim3(gf), dim3(kVecBlock), 0, s->stream, n, v, st, (const T *)part,
                              epi_grid<T>(s, plan), part + kVecPartOff);
-          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, part + kVecPartOff, gf, (T)0};
+          EpiCgResidual<T> e2{v, st, part, pg, &st->ticket[2], c.hist, part + kVecPartOff, gf, (T)0, du};
           launch_epi_fin<T>(s, plan, vals, v.x, e2, st);
           c.spmv += 2;
         }
@@ -1297,7 +1364,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
         if (!hst->done) continue;
         if (!hst->rec_conv || !init_residual) break;  // inner solves stop on the recurrence
         // recurrence converged: confirm with the true residual, restart from it otherwise
-        EpiTrue<T> et{v, st, part, pg, &st->ticket[0]};
+        EpiTrue<T> et{v, st, part, pg, &st->ticket[0], du};
         launch_epi_fin<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
         c.spmv++;
         if ((rc = read_state(s, st, hst))) return rc;
@@ -1325,7 +1392,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
         fin<T>(s, FinBiXr<T>{st, part, g, c.mode == EHIP_FAST ? 1 : 0, c.hist}, g, st);
         c.spmv += 2;
         if (c.mode == EHIP_FAITHFUL) {
-          EpiBiResidual<T> e8{v, st, part, pg, &st->ticket[2], c.hist, (T)0, (T)0};
+          EpiBiResidual<T> e8{v, st, part, pg, &st->ticket[2], c.hist, (T)0, (T)0, du};
           launch_epi_fin<T>(s, plan, vals, v.x, e8, st);
           c.spmv++;
         } else {
@@ -1335,7 +1402,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
       if ((rc = read_state(s, st, hst))) return rc;
       if (!hst->done) continue;
       if (c.mode == EHIP_FAITHFUL || !hst->rec_conv || !init_residual) break;
-      EpiTrue<T> et{v, st, part, pg, &st->ticket[0]};
+      EpiTrue<T> et{v, st, part, pg, &st->ticket[0], du};
       launch_epi_fin<T>(s, plan, vals, v.x, et, (const KStateT<T> *)nullptr);
       c.spmv++;
       if ((rc = read_state(s, st, hst))) return rc;
@@ -1345,6 +1412,7 @@ static int krylov(ehip_system *s, SolveCtx &c, const SpmvPlan &plan, const T *va
     }
   }
   EHIP_HIP_TRY(hipGetLastError());
+  EHIP_HIP_TRY(take_launch_error());
   return EHIP_OK;
 }
 
@@ -1366,7 +1434,8 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
                      c.precond);
   KState *st = s->st;
   // ||b|| of the scaled system and the outer state
-  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0);
+  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0,
+                     c.du ? c.tol_u : 0.0);
   sumsq(s, n, s->rhs, &st->rr);
   if ((rc = read_state(s, st, s->h_st))) return rc;
   const double bn = sqrt(s->h_st->rr);
@@ -1381,9 +1450,11 @@ static int mixed_ir(ehip_system *s, SolveCtx &c, double inner_tol, int inner_max
   ci.hist = nullptr;
   ci.tol = inner_tol;
   ci.divtol = 0;
+  ci.tol_u = 0;
+  ci.du = nullptr;
   ci.prof = c.prof;
   for (;;) {
-    EpiOuter eo{s->rhs, t64, st, s->partials, pg, &st->ticket[0], c.hist};
+    EpiOuter eo{s->rhs, t64, st, s->partials, pg, &st->ticket[0], c.hist, c.du};
     launch_epi_fin<double>(s, s->plan64, s->vals, s->x, eo, (const KState *)nullptr);
     c.spmv++;
     if ((rc = read_state(s, st, s->h_st))) return rc;
@@ -1487,6 +1558,8 @@ static uint64_t cg32_key(const ehip_system *s) {
 static int cg32_iterations_graph(ehip_system *s, const Vecs<float> &v, int count) {
   const uint64_t key = cg32_key(s);
   if (key != s->cg32_key) {
+    // nothing recorded earlier may still be queued when its graph goes
+    EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
     for (hipGraphExec_t &g : s->cg32_graph)
       if (g) {
         (void)hipGraphExecDestroy(g);
@@ -1534,13 +1607,14 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   KState32 *st32 = s->st32;
   hipLaunchKernelGGL(k_make_dg<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, v.dg,
                      c.precond);
-  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0);
+  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 0,
+                     c.du ? c.tol_u : 0.0);
   hipLaunchKernelGGL(k_init_state<float>, dim3(1), dim3(1), 0, s->stream, st32, (float)c.tol, c.maxit,
-                     (float)c.divtol, 0);
+                     (float)c.divtol, 0, 0.f);
   sumsq(s, n, s->rhs, &st->rr);
   hipLaunchKernelGGL(k_set_rhsnorm, dim3(1), dim3(1), 0, s->stream, st, st32);
   hipLaunchKernelGGL(k_zero2<float>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v.p, v.q);
-  EpiRU eru{s->rhs, v, st, st32, s->partials, pg, &st->ticket[0], c.hist, delta};
+  EpiRU eru{s->rhs, v, st, st32, s->partials, pg, &st->ticket[0], c.hist, delta, c.du};
   launch_epi_fin<double>(s, s->plan64, s->vals, s->x, eru, (const KState *)nullptr);
   c.spmv++;
   int nupd = 0;
@@ -1560,8 +1634,8 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   // the SELL copy of the FP32 values is refreshed here, outside any recording
   if (s->fmt32 == 1 && s->sell_ok && !sell_values32(s)) return EHIP_ENOMEM;
   while (!s->h_st->done) {
-    if (c.prof) {
-      cg32_iterations(s, v, batch, true);  // sampled launches need their own events
+    if (c.prof || !graphs_enabled()) {
+      cg32_iterations(s, v, batch, c.prof);  // sampled launches need their own events
     } else if ((rc = cg32_iterations_graph(s, v, batch))) {
       return rc;
     }
@@ -1601,6 +1675,7 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
     batch = std::max(1, std::min(batch, h.maxit - h.iter));
   }
   if ((rc = read_state(s, st32, s->h_st32))) return rc;
+  EHIP_HIP_TRY(take_launch_error());
   *inner_total = s->h_st32->iter;
   *updates = nupd;
   s->h_st->iter = *inner_total;
@@ -1698,6 +1773,10 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   c.check = p->check_every > 0 ? p->check_every : (n > 200000 ? 25 : 10);
   c.check_set = p->check_every > 0;
   c.hist = dhist;
+  // the unscaled test: without scaling the scaled residual is the unscaled one
+  c.tol_u = (p->true_residual_stop && scaling) ? p->tol : 0.0;
+  c.du = c.tol_u > 0 ? s->dscale : nullptr;
+  if (c.du) sumsq(s, n, s->rhs, &s->st->bb_u, s->dscale);  // ||b_s / D||^2
   c.prof = p->profile_every > 0;
   s->prof_every = p->profile_every;
   s->prof_n = 0;

@@ -23,6 +23,21 @@ struct GraphRec {
 };
 extern thread_local GraphRec *tl_graph_rec;
 
+// First failure of a direct launch_k on this thread since the last
+// take_launch_error() (HIP's own last-error slot is overwritten by every
+// later call, so launch failures are kept here until a driver checks them).
+extern thread_local hipError_t tl_launch_err;
+inline hipError_t take_launch_error() {
+  const hipError_t e = tl_launch_err;
+  tl_launch_err = hipSuccess;
+  return e;
+}
+
+// Whether iteration blocks may be replayed as graphs: off when EHIP_GRAPHS=0,
+// and off under rocprofv3 (ROCPROF_OUTPUT_PATH in the environment), where
+// the same kernels are launched one by one instead.
+bool graphs_enabled();
+
 template <typename Tuple, std::size_t... I>
 inline void kernel_arg_ptrs(Tuple &t, void **out, std::index_sequence<I...>) {
   ((out[I] = (void *)&std::get<I>(t)), ...);
@@ -36,7 +51,8 @@ inline void launch_k(void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_
   kernel_arg_ptrs(a, ptrs, std::index_sequence_for<KArgs...>{});
   GraphRec *r = tl_graph_rec;
   if (!r) {
-    (void)hipLaunchKernel(reinterpret_cast<const void *>(kernel), grid, block, ptrs, 0, stream);
+    const hipError_t e = hipLaunchKernel(reinterpret_cast<const void *>(kernel), grid, block, ptrs, 0, stream);
+    if (e != hipSuccess && tl_launch_err == hipSuccess) tl_launch_err = e;
     return;
   }
   if (r->err != hipSuccess) return;
@@ -45,7 +61,7 @@ inline void launch_k(void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_
   p.gridDim = grid;
   p.blockDim = block;
   p.sharedMemBytes = 0;
-  p.kernelParams = ptrs;  // copied into the node
+  p.kernelParams = ptrs;  // the node keeps its own copy of the argument values
   p.extra = nullptr;
   hipGraphNode_t node = nullptr;
   r->err = hipGraphAddKernelNode(&node, r->graph, r->last ? &r->last : nullptr, r->last ? 1 : 0, &p);

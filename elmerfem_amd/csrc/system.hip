@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 
 #include "ehip_internal.h"
@@ -632,8 +633,13 @@ static int system_alloc(int n, int64_t nnz, ehip_system **out) {
     return EHIP_EARG;
   }
   ehip_system *s = new ehip_system();
-  EHIP_HIP_TRY(hipGetDevice(&s->dev));
-  EHIP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  hipError_t e = hipGetDevice(&s->dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error("system_alloc: %s", hipGetErrorString(e));
+    delete s;
+    return EHIP_EHIP;
+  }
   s->n = n;
   s->nnz = nnz;
   *out = s;
@@ -1006,8 +1012,23 @@ static uint64_t values_checksum(const double *v, int64_t nnz) {
   return h;
 }
 
+// Elmer's call site (CRS_MatrixVectorProd, CRSMatrix.src:3549-3551) has no
+// error channel, so a failure must not leave v holding a stale product: v
+// is filled with NaN, which makes HUTI's residual test fail (the solve ends
+// unconverged and "Linear System Abort Not Converged" stops the run), and
+// the reason is printed to stderr and kept for ehip_last_error().
+static void matvec_fail(int n, double *v) {
+  std::fprintf(stderr, "ehip_matvec: %s\n", g_err[0] ? g_err : "failed");
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  for (int i = 0; i < n; ++i) v[i] = nan;
+}
+
 void ehip_matvec(void **spmv, int *n, int *rows, int *cols, double *vals, double *u, double *v, int *reinit) {
-  if (!spmv || !n || !rows || !cols || !vals || !u || !v) return;
+  if (!spmv || !n || !rows || !cols || !vals || !u || !v) {
+    set_error("ehip_matvec: null argument");
+    if (n && v) matvec_fail(*n, v);
+    return;
+  }
   MatvecCache *c = static_cast<MatvecCache *>(*spmv);
   const int nn = *n;
   const int64_t nnz = (int64_t)rows[nn] - 1;
@@ -1030,21 +1051,25 @@ void ehip_matvec(void **spmv, int *n, int *rows, int *cols, double *vals, double
     }
     // ehip_system_create insists on a stored diagonal; bypass its check for SpMV-only use
     ehip_system *s = nullptr;
-    if (system_alloc(nn, nnz, &s)) return;
+    if (system_alloc(nn, nnz, &s)) { matvec_fail(nn, v); return; }
     s->h_rows.assign(rows, rows + nn + 1);
     s->h_cols.assign(cols, cols + nnz);
     s->h_diag = dg;
     s->nnodes = nn;
-    if (ehip_internal_upload_pattern(s)) { ehip_system_free(s); return; }
+    if (ehip_internal_upload_pattern(s)) {
+      ehip_system_free(s);
+      matvec_fail(nn, v);
+      return;
+    }
     c->s = s;
     c->vsum = ~values_checksum(vals, nnz);
   }
   uint64_t h = values_checksum(vals, nnz);
   if (h != c->vsum) {
-    if (ehip_system_set_values(c->s, vals, nullptr)) return;
+    if (ehip_system_set_values(c->s, vals, nullptr)) { matvec_fail(nn, v); return; }
     c->vsum = h;
   }
-  ehip_spmv(c->s, 0, u, v);
+  if (ehip_spmv(c->s, 0, u, v) != EHIP_OK) matvec_fail(nn, v);
 }
 
 // The name ElmerSolver's loader asks dlsym for when it was built with the

@@ -40,6 +40,7 @@ module ehip
      real(c_double) :: divtol = 0.0d0
      integer(c_int) :: record_history = 0
      integer(c_int) :: profile_every = 0
+     integer(c_int) :: true_residual_stop = 0
   end type ehip_params
 
   type, bind(C) :: ehip_report

@@ -43,9 +43,10 @@ class LinearSystemKeywords:
 
     def to_params(self, precision: int = _lib.FP64, mode: int = _lib.FAITHFUL, inner_tol: float = 0.0,
                   inner_maxit: int = 0, check_every: int = 0, record_history: bool = False,
-                  profile_every: int = 0) -> _lib.Params:
+                  profile_every: int = 0, true_residual_stop: bool = False) -> _lib.Params:
         return _lib.Params(self.method, self.precond, self.tol, self.maxit, int(self.scaling), precision, mode,
-                           inner_tol, inner_maxit, check_every, self.divtol, int(record_history), profile_every)
+                           inner_tol, inner_maxit, check_every, self.divtol, int(record_history), profile_every,
+                           int(true_residual_stop))
 
 
 def _method(name: str | None) -> tuple[int, str]:

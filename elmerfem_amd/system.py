@@ -150,11 +150,14 @@ class System:
 
     # ------------------------------------------------------------ solve
     def solve(self, kw: LinearSystemKeywords | _lib.Params, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL,
-              inner_tol=0.0, inner_maxit=0, check_every=0, history=False, return_x=True, profile_every=0):
+              inner_tol=0.0, inner_maxit=0, check_every=0, history=False, return_x=True, profile_every=0,
+              true_residual_stop=False):
         p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every,
-                                                              history, profile_every)
+                                                              history, profile_every, true_residual_stop)
         if isinstance(kw, _lib.Params):
             p.record_history = int(history)
+            if true_residual_stop:
+                p.true_residual_stop = 1
         x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
         rep = _lib.Report()
         hist = np.zeros(p.maxit + 2) if history else None
@@ -165,6 +168,12 @@ class System:
             k = rep.outer_iters + 1 if p.precision == _lib.MIXED_IR else min(rep.iters, p.maxit)
             out["history"] = hist[:k].copy()
         return (x if return_x else None), out
+
+    def solution(self):
+        """The solution currently on the device (the last solve's, back-scaled)."""
+        x = np.zeros(self.n)
+        _lib.check(_lib.load().ehip_get_solution(self._h, _ptr(x)), "ehip_get_solution")
+        return x
 
     # ------------------------------------------------------------ SpMV
     def spmv(self, u, precision=0):
@@ -214,7 +223,8 @@ def copy_bench(nbytes: int, reps: int = 20):
     return g.value, us.value
 
 
-def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL):
+def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL,
+              true_residual_stop=False):
     """One-shot drop-in (the Before-Linsolve hook path): host CRS in, x out."""
     L = _lib.init()
     rows = np.ascontiguousarray(rows, np.int32)
@@ -223,7 +233,8 @@ def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mod
     v = _f64(values)
     bb = _f64(b)
     x = np.zeros(len(rows) - 1) if x0 is None else np.array(x0, dtype=np.float64)
-    p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode)
+    p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode,
+                                                          true_residual_stop=true_residual_stop)
     rep = _lib.Report()
     _lib.check(L.ehip_solve_csr(len(rows) - 1, _ptr(rows), _ptr(cols), _ptr(diag), _ptr(v), _ptr(bb), _ptr(x),
                                 C.byref(p), C.byref(rep)), "ehip_solve_csr")

@@ -74,6 +74,12 @@ typedef struct {
   double divtol;     /* "Linear System Divergence Tolerance" (0 = none) */
   int record_history;/* 1: keep the residual of every iteration */
   int profile_every; /* > 0: time every k-th SpMV launch of the Krylov loop with HIP events */
+  int true_residual_stop; /* 0: HUTI's stop test alone, ||D(b - A x)|| / ||D b|| < tol on the
+                        scaled system (TRESID_SCALED_BYB, IterSolve.src:318-340).  1: convergence
+                        also needs the unscaled FP64 residual ||b - A x|| / ||b|| < tol, evaluated
+                        on the device from the scaled residual as ||r_s / D|| / ||b_s / D||.  The
+                        loop order stays HUTI's in FAITHFUL mode: iterations continue until both
+                        hold.  No counterpart in the reference. */
 } ehip_params;
 
 typedef struct {

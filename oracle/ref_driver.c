@@ -7,10 +7,13 @@
  * hutiter/src/huti_fdefs.h, user-supplied initial x, TRESID_SCALED_BYB,
  * matvec = CRS_MatrixVectorProd, pcondl = CRS_DiagPrecondition or the
  * dummy copy, pcondr = dummy copy (huti_sfe_c.src:117-124), and the
- * reference BLAS ddot/dnrm2 for dot and norm.  The matvec/pcond callbacks are
- * this file's restatement (the Fortran CRS module needs Elmer's generated
- * build configuration and is not built here); the Krylov loops, dot and norm
- * that run are the reference's compiled sources.
+ * reference BLAS ddot/dnrm2 for dot and norm.  The dummy copy is the
+ * reference's own huti_ddummy_pcondfun (hutiter/src/huti_aux_D.F90, built
+ * beside the solvers, which also supplies huti_drandvec).  The matvec and
+ * diagonal-preconditioner callbacks are this file's restatement (the Fortran
+ * CRS module needs Elmer's generated build configuration and is not built
+ * here); the Krylov loops, dot, norm and dummy copy that run are the
+ * reference's compiled sources.
  *
  * The residual history is read off the norm callback: with stopc = 1 HUTI
  * calls normfun once for ||b|| and once per iteration for ||Ax - b||.
@@ -27,14 +30,7 @@ extern void huti_dbicgstabsolv_(int *, int *, double *, double *, int *, double 
                                 double (*)());
 extern double ddot_(int *, double *, int *, double *, int *);
 extern double dnrm2_(int *, double *, int *);
-
-/* HUTI only calls this for a random initial guess, which IterSolver never
- * requests (HUTI_INITIALX = USERSUPPLIEDX). */
-void huti_drandvec_(double *x, int *ipar) {
-  (void)x;
-  (void)ipar;
-  abort();
-}
+extern void huti_ddummy_pcondfun_(double *u, double *v, int *ipar);
 
 static struct {
   int n;
@@ -55,17 +51,15 @@ static void cb_matvec(double *u, double *v, int *ipar) {
 }
 
 static void cb_pcond(double *u, double *v, int *ipar) {
-  (void)ipar;
+  if (!G.precond) {
+    huti_ddummy_pcondfun_(u, v, ipar);
+    return;
+  }
   const double AEPS = 10.0 * DBL_EPSILON;
   for (int i = 0; i < G.n; i++) {
     double d = G.values[G.diag[i] - 1];
     u[i] = (G.precond && fabs(d) > AEPS) ? v[i] / d : v[i];
   }
-}
-
-static void cb_copy(double *u, double *v, int *ipar) {
-  (void)ipar;
-  memcpy(u, v, sizeof(double) * G.n);
 }
 
 static double cb_norm(int *n, double *x, int *inc) {
@@ -104,11 +98,11 @@ int ref_huti_solve(int method, int precond, int n, const int *rows, const int *c
   int nd = n, wd = wrk;
   if (method == 0)
     huti_dcgsolv_(&nd, &wd, x, bb, ipar, dpar, work, (void (*)())cb_matvec, (void (*)())cb_pcond,
-                  (void (*)())cb_copy, (double (*)())ddot_, (double (*)())cb_norm,
+                  (void (*)())huti_ddummy_pcondfun_, (double (*)())ddot_, (double (*)())cb_norm,
                   (double (*)())cb_stopc);
   else
     huti_dbicgstabsolv_(&nd, &wd, x, bb, ipar, dpar, work, (void (*)())cb_matvec,
-                        (void (*)())cb_pcond, (void (*)())cb_copy, (double (*)())ddot_,
+                        (void (*)())cb_pcond, (void (*)())huti_ddummy_pcondfun_, (double (*)())ddot_,
                         (double (*)())cb_norm, (double (*)())cb_stopc);
   *iters = ipar[30];
   if (G.nnorm > G.cap) G.nnorm = G.cap;

@@ -3,7 +3,8 @@
 Bars: integer topology bit-exact; FP64 SpMV bit-exact (same summation order
 as CRS_MatrixVectorProd); assembled Values within 1e-12 of the row's largest
 entry; Dirichlet rows bit-exact on identical input; solutions within 1e-8
-relative of the oracle; true FP64 residual <= the requested tolerance.
+relative of the oracle; with the unscaled stop test on, the relative
+residual ||b - A x|| / ||b|| recomputed in FP64 on the host <= 1e-10.
 """
 import ctypes as C
 
@@ -52,6 +53,17 @@ def dense(rows, cols, vals):
     for i in range(n):
         A[i, cols[rows[i] - 1:rows[i + 1] - 1] - 1] = vals[rows[i] - 1:rows[i + 1] - 1]
     return A
+
+
+def host_residual(pat, vals, rhs, x):
+    """||b - A x|| / ||b|| in FP64 on the host, A from the unscaled CRS."""
+    ax = np.zeros(pat.n)
+    O.lib().or_spmv(pat.n, O._p(pat.rows), O._p(pat.cols), O._p(vals), O._p(np.ascontiguousarray(x)), O._p(ax))
+    return np.linalg.norm(rhs - ax) / np.linalg.norm(rhs)
+
+
+# the north-star bar on the unscaled FP64 residual
+RES_BAR = 1e-10
 
 
 # ------------------------------------------------------------------ SpMV
@@ -349,6 +361,24 @@ def test_pattern_bit_exact(ehip, kind, dofs):
         s.free()
 
 
+@pytest.mark.parametrize("name", ["quad4_2x2_scrambled", "two_chains_ties"])
+@pytest.mark.parametrize("dofs", [1, 2])
+def test_pattern_hand_traced_reordering(ehip, name, dofs):
+    """The library's pattern build reproduces the accepted OptimizeBandwidth
+    permutation worked out by hand from the reference routine."""
+    from types import SimpleNamespace
+    from golden import bandwidth_cases as BC
+    el, nnodes, perm, bw_before, bw_after, accepted = BC.case(name)
+    m = SimpleNamespace(elements=el, bnd_elements=np.zeros((0, el.shape[1]), np.int32), nnodes=nnodes)
+    s = System.from_mesh(m, dofs=dofs, optimize_bandwidth=True)
+    p, rows, cols, diag = s.pattern()
+    assert np.array_equal(p, perm)
+    assert s.bandwidth == (bw_before, bw_after, accepted)
+    pat = O.Pattern(el, nnodes, dofs, optimize=True)
+    assert np.array_equal(rows, pat.rows) and np.array_equal(cols, pat.cols) and np.array_equal(diag, pat.diag)
+    s.free()
+
+
 # ------------------------------------------------------------------ assembly + Dirichlet
 
 @pytest.mark.parametrize("wl", ["poisson", "lognormal", "convdiff", "tet4", "elastic8", "elastic27", "poisson27"])
@@ -432,10 +462,14 @@ def test_faithful_solve_matches_reference(ehip, case, fmt):
     assert np.allclose(rep["history"][:n_common], hist[:n_common], rtol=1e-6)
     assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
     assert abs(rep["norm"] - nrm) <= 1e-8 * nrm
-    # FP64 true residual recomputed on the host
-    A = dense(pat.rows, pat.cols, vals)
-    assert np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs) <= 1e-9
-    assert rep["res_true"] <= 1e-9
+    # the same HUTI loop with the unscaled test added: iterations continue
+    # until the FP64 residual recomputed on the host is <= 1e-10 as well
+    s.set_values(vals, rhs)
+    x2, rep2 = s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, true_residual_stop=True)
+    assert rep2["info"] == 1 and rep2["iters"] >= rep["iters"]
+    assert host_residual(pat, vals, rhs, x2) <= RES_BAR
+    assert rep2["res_true"] <= RES_BAR and rep2["res_scaled"] < spec["tol"]
+    assert np.abs(x2 - xo).max() <= 1e-8 * np.abs(xo).max()
     s.free()
 
 
@@ -451,14 +485,14 @@ def test_fast_and_mixed_solve(ehip, case, precision, mode, fmt):
     s = System.from_csr(pat.rows, pat.cols, pat.diag)
     s.set_values(vals, rhs)
     set_format(s, fmt)
-    x, rep = s.solve(kw, precision=precision, mode=mode)
+    x, rep = s.solve(kw, precision=precision, mode=mode, true_residual_stop=True)
     info, k, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, method=spec["method"],
                                                    precond=spec["precond"], tol=1e-12, maxit=5000)
     assert rep["info"] == 1, rep
     assert rep["res_scaled"] < 1e-10
     assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
-    A = dense(pat.rows, pat.cols, vals)
-    assert np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs) <= 1e-9
+    assert host_residual(pat, vals, rhs, x) <= RES_BAR
+    assert rep["res_true"] <= RES_BAR
     s.free()
 
 
@@ -486,13 +520,12 @@ def test_hex27_elasticity_solve(ehip, precision, mode):
     assert np.array_equal(s.spmv(u), ref)
     kw = w["kw"]
     kw.method, kw.precond, kw.tol, kw.maxit = 0, 1, 1e-10, 20000
-    x, rep = s.solve(kw, precision=precision, mode=mode)
+    x, rep = s.solve(kw, precision=precision, mode=mode, true_residual_stop=True)
     info, it, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, method=0, precond=1, tol=1e-12,
                                                     maxit=20000)
     assert rep["info"] == 1, rep
     assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
-    A = dense(pat.rows, pat.cols, vals)
-    assert np.linalg.norm(rhs - A @ x) / np.linalg.norm(rhs) <= 1e-9
+    assert host_residual(pat, vals, rhs, x) <= RES_BAR
     s.free()
 
 
@@ -518,9 +551,10 @@ def test_status_codes(ehip):
 
 def test_solve_csr_dropin(ehip):
     w, pat, vals, rhs, rows = assembled_case(CASES["cg_lognormal8"])
-    x, rep = solve_csr(pat.rows, pat.cols, pat.diag, vals, rhs, w["kw"])
+    x, rep = solve_csr(pat.rows, pat.cols, pat.diag, vals, rhs, w["kw"], true_residual_stop=True)
     info, k, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, tol=1e-10)
     assert rep["info"] == 1 and np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
+    assert host_residual(pat, vals, rhs, x) <= RES_BAR
 
 
 def test_solve_deterministic(ehip):

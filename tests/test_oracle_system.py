@@ -77,6 +77,64 @@ def test_optimize_bandwidth_invariants():
     assert not pat2.accepted and np.array_equal(pat2.perm, np.arange(1, m.nnodes + 1))
 
 
+@pytest.mark.parametrize("name", ["quad4_2x2_scrambled", "two_chains_ties"])
+def test_optimize_bandwidth_hand_traced(name):
+    """The accepted reordering against Perm worked out by hand from
+    BandwidthOptimize.src:108-355 (steps in tests/golden/bandwidth_cases.py)."""
+    from golden import bandwidth_cases as BC
+    el, nnodes, perm, bw_before, bw_after, accepted = BC.case(name)
+    pat = O.Pattern(el, nnodes, 1, optimize=True)
+    assert np.array_equal(pat.perm, perm)
+    assert (pat.bw_before, pat.bw_after, pat.accepted) == (bw_before, bw_after, accepted)
+    # two unknowns per node interleave behind the same node order
+    pat2 = O.Pattern(el, nnodes, 2, optimize=True)
+    assert np.array_equal(pat2.perm, perm)
+
+
+def test_optimize_bandwidth_start_is_first_min_degree():
+    """The NewRoot loop of OptimizeBandwidth (BandwidthOptimize.src:172-200)
+    can never move the start node: it needs a node of strictly smaller
+    Degree than the start node, which already has the smallest.  So the
+    accepted permutation is the reversed breadth-first numbering from the
+    first node of smallest Degree, components taken from the smallest
+    unnumbered node -- checked on random scrambled meshes, where a wrong
+    start node, queue order or reversal shows."""
+    rng = np.random.default_rng(11)
+    accepted = 0
+    for trial in range(6):
+        m = M.quad4_square(3 + trial % 3) if trial % 2 else M.hex8_cube(2 + trial % 2)
+        relabel = rng.permutation(m.nnodes) + 1
+        el = relabel[m.elements - 1].astype(np.int32)
+        pat = O.Pattern(el, m.nnodes, 1, optimize=True)
+        adj = [set() for _ in range(m.nnodes + 1)]
+        for e in el:
+            for a in e:
+                adj[a].update(int(b) for b in e)
+        deg = [len(a) for a in adj]
+        start = min(range(1, m.nnodes + 1), key=lambda i: (deg[i], i))
+        order, placed = [start], {start}
+        i = 0
+        while len(order) < m.nnodes:
+            if i == len(order):
+                nxt = min(set(range(1, m.nnodes + 1)) - placed)
+                order.append(nxt)
+                placed.add(nxt)
+            for b in sorted(adj[order[i]]):
+                if b not in placed:
+                    order.append(b)
+                    placed.add(b)
+            i += 1
+        expect = np.zeros(m.nnodes, np.int32)
+        for k, node in enumerate(order):
+            expect[node - 1] = m.nnodes - k
+        if pat.accepted:
+            accepted += 1
+            assert np.array_equal(pat.perm, expect)
+        else:
+            assert np.array_equal(pat.perm, np.arange(1, m.nnodes + 1))
+    assert accepted >= 4
+
+
 @pytest.mark.parametrize("wl", ["poisson", "lognormal", "convdiff", "tet4", "elastic"])
 def test_glue_matches_dense_assembly(wl):
     w = {"poisson": lambda: W.poisson_hex8(3), "lognormal": lambda: W.poisson_hex8(3, sigma=1.0),
