@@ -19,6 +19,9 @@ FP64, MIXED_IR = 0, 1
 FAITHFUL, FAST = 0, 1
 PHYS_MODELPDE, PHYS_ELASTIC = 0, 1
 SPMV_CSR, SPMV_SELL = 0, 1
+EARG, EHIP, ENOMEM, EUNSUP, ECAP, ESTATE = -1, -2, -3, -4, -5, -6
+NOALLOC_SCRATCH_BYTES = 96 << 20
+NOALLOC_MAX_N, NOALLOC_MAX_NNZ = 262144, 4194304
 
 HUTI_CONVERGENCE = 1
 HUTI_MAXITER = 2
@@ -75,6 +78,8 @@ SIGNATURES = {
     "ehip_solve": (C.c_int, [_vp, C.POINTER(Params), _vp, C.POINTER(Report), _vp]),
     "ehip_get_solution": (C.c_int, [_vp, _vp]),
     "ehip_solve_csr": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params), C.POINTER(Report)]),
+    "ehip_solve_csr_noalloc": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params),
+                                         C.POINTER(Report)]),
     "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),

@@ -206,6 +206,14 @@ inline int sell_grid(const ehip_system *s) {
 }
 
 // SpMV
+// Chunk cut of the CSR stream schedule: the first row of every chunk of at
+// most nb nonzeros (a longer row alone), then n.  rows is 1-based (any
+// common offset works: only differences are used).  Writes entries
+// [first, first + cap) of the list to out (may be null) and returns the
+// length of the whole list.
+int chunk_rows(const int *rows, int n, int nb, int *out, int first, int cap);
+// Fills a plan's shape (block, nb, chunk count, XCD-aware grid).
+void plan_shape(SpmvPlan &plan, int block, int nb, int nchunks);
 int plan_spmv(ehip_system *s, SpmvPlan &plan, int block, int nb);
 int spmv_f64(ehip_system *s, const double *x, double *y);
 int spmv_f32(ehip_system *s, const float *x, float *y);
@@ -216,6 +224,10 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
 int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale);
 
 int max_grid();
+
+// ehip_solve's body.  events = false times the solve with host clock
+// readings instead of HIP events (for callers that create no runtime objects).
+int solve_impl(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist, bool events);
 
 // SpMV launch sampling (ehip_params.profile_every)
 void prof_begin(ehip_system *s);

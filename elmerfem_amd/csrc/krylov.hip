@@ -1683,11 +1683,43 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   return EHIP_OK;
 }
 
-}  // namespace ehip
+// Time marks of a solve: HIP events on the system's stream, or, for callers
+// that create no runtime objects, host clock readings taken after the
+// stream has drained.
+struct SolveClock {
+  bool ev = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t e[4] = {};
+  std::chrono::steady_clock::time_point h[4];
+  SolveClock(bool use_events, hipStream_t st) : ev(use_events), stream(st) {}
+  ~SolveClock() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+  hipError_t create() {
+    if (!ev) return hipSuccess;
+    for (hipEvent_t &x : e) {
+      const hipError_t err = hipEventCreate(&x);
+      if (err != hipSuccess) return err;
+    }
+    return hipSuccess;
+  }
+  hipError_t mark(int i) {
+    if (ev) return hipEventRecord(e[i], stream);
+    const hipError_t err = hipStreamSynchronize(stream);
+    h[i] = std::chrono::steady_clock::now();
+    return err;
+  }
+  // after the last mark has completed
+  float ms(int a, int b) const {
+    if (!ev) return (float)std::chrono::duration<double, std::milli>(h[b] - h[a]).count();
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e[a], e[b]);
+    return t;
+  }
+};
 
-// ================================================================ C ABI
-
-extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist) {
+int solve_impl(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist, bool events) {
   if (!s || !p) return EHIP_EARG;
   if (!s->has_values) {
     set_error("ehip_solve: the system has no values (assemble or set them first)");
@@ -1701,13 +1733,10 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   const int n = s->n;
   int rc;
   if ((rc = ensure_state(s))) return rc;
-  hipEvent_t e0, e1, e2, e3;
-  EHIP_HIP_TRY(hipEventCreate(&e0));
-  EHIP_HIP_TRY(hipEventCreate(&e1));
-  EHIP_HIP_TRY(hipEventCreate(&e2));
-  EHIP_HIP_TRY(hipEventCreate(&e3));
+  SolveClock clk(events, s->stream);
+  EHIP_HIP_TRY(clk.create());
   ehip_report rep{};
-  EHIP_HIP_TRY(hipEventRecord(e0, s->stream));
+  EHIP_HIP_TRY(clk.mark(0));
   // initial guess
   bool xzero = true;
   if (x) {
@@ -1781,7 +1810,7 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
   s->prof_every = p->profile_every;
   s->prof_n = 0;
   s->prof_count = 0;
-  EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
+  EHIP_HIP_TRY(clk.mark(1));
   int inner_total = 0;
   if (mixed && p->method == EHIP_CG) {
     // reliable-update CG: inner_tol is the residual drop between replacements
@@ -1801,7 +1830,7 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
                        v.dg, p->precond);
     if ((rc = krylov<double>(s, c, s->plan64, s->vals, v, s->st, s->h_st, s->partials, true))) return rc;
   }
-  EHIP_HIP_TRY(hipEventRecord(e2, s->stream));
+  EHIP_HIP_TRY(clk.mark(2));
   const KState hs = *s->h_st;
   rep.info = hs.info;
   rep.iters = hs.iter;
@@ -1822,25 +1851,26 @@ extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_
     rep.res_true = sqrt(s->h_st->ts) / sqrt(s->h_st->tt);
     rep.norm = sqrt(s->h_st->rr / (double)n);
   }
-  EHIP_HIP_TRY(hipEventRecord(e3, s->stream));
+  EHIP_HIP_TRY(clk.mark(3));
   if (x) EHIP_HIP_TRY(hipMemcpyAsync(x, s->x, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
   EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
-  float tk = 0, tt = 0;
-  (void)hipEventElapsedTime(&tk, e1, e2);
-  (void)hipEventElapsedTime(&tt, e0, e3);
-  rep.t_krylov_ms = tk;
-  rep.t_total_ms = tt;
+  rep.t_krylov_ms = clk.ms(1, 2);
+  rep.t_total_ms = clk.ms(0, 3);
   rep.spmv_count = c.spmv + 1;
   prof_collect(s, &rep.spmv_us, &rep.spmv_samples);
   rep.spmv_precision = mixed ? 1 : 0;
   s->prof_every = 0;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
-  (void)hipEventDestroy(e3);
   s->has_x = true;
   if (r) *r = rep;
   return EHIP_OK;
+}
+
+}  // namespace ehip
+
+// ================================================================ C ABI
+
+extern "C" int ehip_solve(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist) {
+  return ehip::solve_impl(s, p, x, r, hist, true);
 }
 
 extern "C" int ehip_get_solution(ehip_system *s, double *x) {

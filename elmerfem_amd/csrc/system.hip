@@ -47,36 +47,48 @@ void dfree(ehip_system *s, void *p, size_t bytes) {
 
 // ---------------------------------------------------------------- SpMV plan
 
-int plan_spmv(ehip_system *s, SpmvPlan &plan, int block, int nb) {
-  std::vector<int> cr;
-  cr.reserve(s->n / 8 + 16);
-  const int *rows = s->h_rows.data();  // 1-based
+int chunk_rows(const int *rows, int n, int nb, int *out, int first, int cap) {
+  int m = 0;
+  auto put = [&](int v) {
+    if (out && m >= first && m - first < cap) out[m - first] = v;
+    ++m;
+  };
   int r = 0;
-  cr.push_back(0);
-  while (r < s->n) {
+  put(0);
+  while (r < n) {
     const int start = rows[r];
     int e = r + 1;
     if (rows[e] - start > nb) {  // long row: its own chunk
-      cr.push_back(e);
+      put(e);
       r = e;
       continue;
     }
     // extend while the chunk fits (binary search on the row pointer)
-    int lo = e, hi = s->n;
+    int lo = e, hi = n;
     while (lo < hi) {
       int mid = lo + (hi - lo + 1) / 2;
       if (rows[mid] - start <= nb) lo = mid; else hi = mid - 1;
     }
-    cr.push_back(lo);
+    put(lo);
     r = lo;
   }
-  if (plan.chunk_row) dfree(s, plan.chunk_row, sizeof(int) * (plan.nchunks + 1));
+  return m;
+}
+
+void plan_shape(SpmvPlan &plan, int block, int nb, int nchunks) {
   plan.block = block;
   plan.nb = nb;
-  plan.nchunks = (int)cr.size() - 1;
+  plan.nchunks = nchunks;
   int g = std::min(max_grid(), std::max(plan.nchunks, 1));
-  g = (g + 7) / 8 * 8;
-  plan.grid = g;
+  plan.grid = (g + 7) / 8 * 8;
+}
+
+int plan_spmv(ehip_system *s, SpmvPlan &plan, int block, int nb) {
+  const int *rows = s->h_rows.data();  // 1-based
+  std::vector<int> cr((size_t)chunk_rows(rows, s->n, nb, nullptr, 0, 0));
+  chunk_rows(rows, s->n, nb, cr.data(), 0, (int)cr.size());
+  if (plan.chunk_row) dfree(s, plan.chunk_row, sizeof(int) * (plan.nchunks + 1));
+  plan_shape(plan, block, nb, (int)cr.size() - 1);
   int rc = dmalloc(s, (void **)&plan.chunk_row, sizeof(int) * cr.size());
   if (rc) return rc;
   EHIP_HIP_TRY(hipMemcpyAsync(plan.chunk_row, cr.data(), sizeof(int) * cr.size(),

@@ -169,6 +169,18 @@ module ehip
        type(ehip_report), intent(out) :: report
      end function ehip_solve_csr
 
+     ! The same solve with no device allocation (FP64; limits in ehip.h).
+     integer(c_int) function ehip_solve_csr_noalloc(n, rows, cols, diag, values, b, x, params, report) &
+          bind(C, name="ehip_solve_csr_noalloc")
+       import :: c_int, c_double, ehip_params, ehip_report
+       integer(c_int), value :: n
+       integer(c_int), intent(in) :: rows(*), cols(*), diag(*)
+       real(c_double), intent(in) :: values(*), b(*)
+       real(c_double), intent(inout) :: x(*)
+       type(ehip_params), intent(in) :: params
+       type(ehip_report), intent(out) :: report
+     end function ehip_solve_csr_noalloc
+
      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s

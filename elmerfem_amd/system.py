@@ -224,8 +224,10 @@ def copy_bench(nbytes: int, reps: int = 20):
 
 
 def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mode=_lib.FAITHFUL,
-              true_residual_stop=False):
-    """One-shot drop-in (the Before-Linsolve hook path): host CRS in, x out."""
+              true_residual_stop=False, noalloc=False):
+    """One-shot drop-in (the Before-Linsolve hook path): host CRS in, x out.
+    noalloc=True calls ehip_solve_csr_noalloc (no device allocation; FP64 only,
+    within EHIP_NOALLOC_MAX_N / EHIP_NOALLOC_MAX_NNZ)."""
     L = _lib.init()
     rows = np.ascontiguousarray(rows, np.int32)
     cols = np.ascontiguousarray(cols, np.int32)
@@ -236,6 +238,8 @@ def solve_csr(rows, cols, diag, values, b, kw, x0=None, precision=_lib.FP64, mod
     p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode,
                                                           true_residual_stop=true_residual_stop)
     rep = _lib.Report()
-    _lib.check(L.ehip_solve_csr(len(rows) - 1, _ptr(rows), _ptr(cols), _ptr(diag), _ptr(v), _ptr(bb), _ptr(x),
-                                C.byref(p), C.byref(rep)), "ehip_solve_csr")
+    fn, name = (L.ehip_solve_csr_noalloc, "ehip_solve_csr_noalloc") if noalloc else (L.ehip_solve_csr,
+                                                                                     "ehip_solve_csr")
+    _lib.check(fn(len(rows) - 1, _ptr(rows), _ptr(cols), _ptr(diag), _ptr(v), _ptr(bb), _ptr(x), C.byref(p),
+                  C.byref(rep)), name)
     return x, rep.as_dict()

@@ -177,6 +177,29 @@ int ehip_get_solution(ehip_system *s, double *x);
 int ehip_solve_csr(int n, const int *rows, const int *cols, const int *diag, const double *values,
                    const double *b, double *x, const ehip_params *p, ehip_report *r);
 
+/* ---- no-allocation solve ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * solve as ehip_solve_csr (same arguments and meaning), with every device
+ * buffer -- the staged CRS and RHS, the solution, D, the Krylov workspace,
+ * partial sums and scalars -- carved from one static device array of
+ * EHIP_NOALLOC_SCRATCH_BYTES that the library owns on each GPU it runs on
+ * and holds until the process ends.  It calls no hipMalloc/hipFree and
+ * creates no stream or event; it runs on the null stream of the current
+ * device and times the solve with the host clock.
+ * Limits: n <= EHIP_NOALLOC_MAX_N and nnz <= EHIP_NOALLOC_MAX_NNZ (else
+ * EHIP_ECAP); precision EHIP_FP64 with record_history = 0 and
+ * profile_every = 0 (else EHIP_EUNSUP).  The SpMV is the CSR stream layout,
+ * whose FP64 row sums equal every other layout's.
+ * All calls on one GPU share its array and leave their data in it: calls
+ * that overlap on one GPU (several host threads) race, and the caller must
+ * serialise them. */
+#define EHIP_NOALLOC_SCRATCH_BYTES (96ll << 20)
+#define EHIP_NOALLOC_MAX_N 262144
+#define EHIP_NOALLOC_MAX_NNZ 4194304
+int ehip_solve_csr_noalloc(int n, const int *rows, const int *cols, const int *diag, const double *values,
+                           const double *b, double *x, const ehip_params *p, ehip_report *r);
+
 /* ---- measurement helpers ---- */
 /* v = A u on the device-resident system (host vectors), precision 0 = FP64
  * values, 1 = FP32 copy of the values. */
