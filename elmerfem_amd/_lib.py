@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libehip.so")
 # names and codes mirror include/ehip.h
 EHIP_OK = 0
 CG, BICGSTAB = 0, 1
-PRECOND_NONE, PRECOND_DIAGONAL = 0, 1
+PRECOND_NONE, PRECOND_DIAGONAL, PRECOND_ILU0 = 0, 1, 2  # ILU(k) = PRECOND_ILU0 + k
 FP64, MIXED_IR = 0, 1
 FAITHFUL, FAST = 0, 1
 PHYS_MODELPDE, PHYS_ELASTIC = 0, 1
@@ -80,6 +80,8 @@ SIGNATURES = {
     "ehip_solve_csr": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params), C.POINTER(Report)]),
     "ehip_solve_csr_noalloc": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(Params),
                                          C.POINTER(Report)]),
+    "ehip_ilu_factor": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_lu_solve": (C.c_int, [_vp, _vp, _vp]),
     "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),

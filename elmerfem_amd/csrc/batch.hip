@@ -1309,6 +1309,10 @@ extern "C" int ehip_batch_free(ehip_batch *b) {
 // result is exactly the system's ehip_solve result.
 extern "C" int ehip_batch_solve(ehip_batch *bh, const ehip_params *p, ehip_report *reports, double *t_ms) {
   if (!bh || !p) return EHIP_EARG;
+  if (p->precond >= EHIP_PRECOND_ILU0) {
+    set_error("ehip_batch_solve: the multi-system engine takes no ILU preconditioning");
+    return EHIP_EUNSUP;
+  }
   const int nsys = bh->capacity;
   std::vector<ehip_system *> sys(nsys);
   for (int i = 0; i < nsys; ++i) sys[i] = bh->slot[i].s;
@@ -1412,6 +1416,10 @@ extern "C" int ehip_batch_get_solution(ehip_batch *b, int i, double *x) {
 
 extern "C" int ehip_server_create(const ehip_params *p, int max_slots, int64_t mem_budget, ehip_batch **out) {
   if (!p || max_slots <= 0 || !out) return EHIP_EARG;
+  if (p->precond >= EHIP_PRECOND_ILU0) {
+    set_error("ehip_server_create: the multi-system engine takes no ILU preconditioning");
+    return EHIP_EUNSUP;
+  }
   ehip_batch *b = nullptr;
   int rc = engine_create(max_slots, p, &b);
   if (rc) return rc;

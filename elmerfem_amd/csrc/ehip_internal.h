@@ -170,6 +170,20 @@ struct ehip_system {
   // valid while cg32_key (the buffers and SpMV schedule they bake in) holds
   hipGraphExec_t cg32_graph[6] = {};
   uint64_t cg32_key = 0;
+  // ILU(k) preconditioner (ilu.hip): pattern order (-1: none), the 0-based
+  // device pattern (the matrix's own arrays when k = 0) with the position
+  // of every matrix nonzero in it (k > 0), the factors, rows grouped by
+  // forward / backward dependency level (host offsets per level), and the
+  // recorded sweeps of up to two (source, destination) vector pairs
+  int ilu_order = -1;
+  int64_t ilu_nnz = 0;
+  int *ilu_rows = nullptr, *ilu_cols = nullptr, *ilu_diag = nullptr, *ilu_amap = nullptr;
+  double *ilu = nullptr;
+  int *ilu_lord = nullptr, *ilu_uord = nullptr;
+  std::vector<int> ilu_loff, ilu_uoff;
+  hipGraphExec_t ilu_graph[2] = {};
+  const double *ilu_graph_src[2] = {};
+  double *ilu_graph_dst[2] = {};
 };
 
 namespace ehip {
@@ -224,6 +238,14 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
 int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale);
 
 int max_grid();
+
+// ILU(k) (ilu.hip): pattern and level schedule of order k (kept while the
+// order stays), factorization of the current Values, and the LU solve
+// dst = (LU)^-1 src enqueued on the system's stream.
+int ilu_setup(ehip_system *s, int order);
+int ilu_factor(ehip_system *s);
+int ilu_apply(ehip_system *s, const double *src, double *dst);
+void ilu_release(ehip_system *s);
 
 // ehip_solve's body.  events = false times the solve with host clock
 // readings instead of HIP events (for callers that create no runtime objects).

@@ -1683,6 +1683,314 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   return EHIP_OK;
 }
 
+// ================================================================ ILU(k)-preconditioned HUTI loops
+//
+// CG and BiCGStab in huti_cg.src / huti_bicgstab.src order with
+// pcondl = CRS_LUPrecondition (u = v, then the LU solve of ilu.hip) and
+// pcondr the dummy copy.  The preconditioner dominates each iteration (one
+// launch per dependency level, replayed from a recorded graph), so the
+// vector steps stay plain kernels and the host checks the state after every
+// iteration.  State: rho (this iteration), rho_new (BiCGStab's rho of the
+// previous iteration), alpha, beta, omega.
+
+// r = b - A x (and rtld = r for BiCGStab) ; partial b.b
+struct EpiIluInit {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(double)
+  double *rtld;
+  __device__ void init() {}
+  struct Pre { double b; };
+  __device__ Pre load(int i) const { return Pre{v.b[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[1]) {
+    const double ri = pr.b - s;
+    v.r[i] = ri;
+    if (rtld) rtld[i] = ri;
+    acc[0] += pr.b * pr.b;
+  }
+  __device__ void finish(double (&tot)[1]) {
+    if (threadIdx.x == 0) st->rhsnorm = sqrt(tot[0]);
+  }
+};
+
+// partials of a0.b0 (and a1.b1)
+template <int NV>
+__global__ __launch_bounds__(kVecBlock) void k_ilu_dots(int n, const double *__restrict__ a0,
+                                                        const double *__restrict__ b0, const double *__restrict__ a1,
+                                                        const double *__restrict__ b1, double *partials,
+                                                        const KState *st) {
+  if (solve_done(st)) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    acc[0] += a0[i] * b0[i];
+    if constexpr (NV > 1) acc[1] += a1[i] * b1[i];
+  }
+  block_publish<double, NV>(acc, partials, gridDim.x);
+}
+
+// CG: rho = r.z (HUTI_CG_RHO on zero); beta = rho / rho of the last iteration
+struct FinIluCgRho {
+  static constexpr int NV = 1;
+  KState *st;
+  double *partials;
+  int stride;
+  __device__ void finish(double (&tot)[1]) {
+    if (threadIdx.x) return;
+    if (tot[0] == 0.0) {
+      st->info = 20;
+      st->done = 1;
+      return;
+    }
+    st->beta = st->iter > 1 ? tot[0] / st->rho : 0.0;
+    st->rho = tot[0];
+  }
+};
+
+// p = z (first iteration) or z + beta p
+__global__ void k_ilu_cg_p(int n, Vecs<double> v, const KState *st) {
+  if (solve_done(st)) return;
+  const bool first = st->iter == 1;
+  const double beta = st->beta;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v.p[i] = first ? v.z[i] : v.z[i] + beta * v.p[i];
+}
+
+// x += alpha p ; r -= alpha q
+__global__ void k_ilu_cg_xr(int n, Vecs<double> v, const KState *st) {
+  if (solve_done(st)) return;
+  const double alpha = st->alpha;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.x[i] = v.x[i] + alpha * v.p[i];
+    v.r[i] = v.r[i] - alpha * v.q[i];
+  }
+}
+
+// The stop test: ||A x - b|| / ||b|| (TRESID_SCALED_BYB) and, when on, the
+// unscaled residual; then BiCGStab's omega = 0 breakdown, the iteration
+// count and the iteration limit in HUTI's order.
+struct EpiIluRes {
+  static constexpr int NV = 2;
+  EHIP_EPI_MEMBERS(double)
+  double *hist;
+  const double *du;
+  int bicg;
+  __device__ void init() {}
+  struct Pre { double b, d; };
+  __device__ Pre load(int i) const { return Pre{v.b[i], du ? du[i] : 1.0}; }
+  __device__ void row(int, double s, const Pre &pr, double (&acc)[2]) {
+    const double ti = s - pr.b;
+    acc[0] += ti * ti;
+    const double ui = ti / pr.d;
+    acc[1] += ui * ui;
+  }
+  __device__ void finish(double (&tot)[2]) {
+    if (threadIdx.x != 0) return;
+    const double res = sqrt(tot[0]) / st->rhsnorm;
+    st->res = res;
+    const int it = st->iter;
+    if (hist && it - 1 < st->maxit) hist[it - 1] = res;
+    if (res < st->tol && unscaled_ok(st, tot[1])) {
+      st->info = 1;
+      st->done = 1;
+      return;
+    }
+    if (bicg) {
+      if (st->omega == 0.0) {
+        st->info = 37;
+        st->done = 1;
+        return;
+      }
+      st->rho_new = st->rho;
+    }
+    st->iter = it + 1;
+    if (it + 1 > st->maxit) {
+      st->info = 2;
+      st->done = 1;
+    }
+  }
+};
+
+// BiCGStab start: rho of the last iteration 1, omega 1, alpha 0; p = v = 0
+__global__ void k_ilu_bi_start(int n, Vecs<double> v, KState *st) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rho_new = 1.0;
+    st->omega = 1.0;
+    st->alpha = 0.0;
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.p[i] = 0.0;
+    v.v[i] = 0.0;
+  }
+}
+
+// rho = rtld.r (HUTI_BICGSTAB_RHO on zero) ; beta = (rho alpha) / (rho_old omega)
+struct FinIluBiRho {
+  static constexpr int NV = 1;
+  KState *st;
+  double *partials;
+  int stride;
+  __device__ void finish(double (&tot)[1]) {
+    if (threadIdx.x) return;
+    if (tot[0] == 0.0) {
+      st->info = 35;
+      st->done = 1;
+      return;
+    }
+    st->beta = (tot[0] * st->alpha) / (st->rho_new * st->omega);
+    st->rho = tot[0];
+  }
+};
+
+// p = r + beta (p - omega v)
+__global__ void k_ilu_bi_p(int n, Vecs<double> v, const KState *st) {
+  if (solve_done(st)) return;
+  const double beta = st->beta, omega = st->omega;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v.p[i] = v.r[i] + beta * (v.p[i] - omega * v.v[i]);
+}
+
+// v = A t1v ; alpha = rho / (rtld.v)
+struct EpiIluBiV {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(double)
+  __device__ void init() {}
+  struct Pre { double rt; };
+  __device__ Pre load(int i) const { return Pre{v.rtld[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[1]) {
+    v.v[i] = s;
+    acc[0] += pr.rt * s;
+  }
+  __device__ void finish(double (&tot)[1]) {
+    if (threadIdx.x == 0) st->alpha = st->rho / tot[0];
+  }
+};
+
+// s = r - alpha v ; partial s.s
+__global__ __launch_bounds__(kVecBlock) void k_ilu_bi_s(int n, Vecs<double> v, const KState *st, double *partials) {
+  if (solve_done(st)) return;
+  const double alpha = st->alpha;
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double si = v.r[i] - alpha * v.v[i];
+    v.sv[i] = si;
+    acc[0] += si * si;
+  }
+  block_publish<double, 1>(acc, partials, gridDim.x);
+}
+
+// ||s|| below HUTI_EPSILON: HUTI_BICGSTAB_SNORM after x += alpha t1v
+struct FinIluSnorm {
+  static constexpr int NV = 1;
+  KState *st;
+  double *partials;
+  int stride;
+  __device__ void finish(double (&tot)[1]) {
+    if (threadIdx.x) return;
+    if (sqrt(tot[0]) < (double)1.17549435E-38f) {  // HUTI_EPSILON (huti_fdefs.h)
+      st->snorm_exit = 1;
+      st->info = 36;
+    }
+  }
+};
+
+__global__ void k_ilu_bi_snorm_x(int n, Vecs<double> v, const KState *st) {
+  if (solve_done(st) || !st->snorm_exit) return;
+  const double alpha = st->alpha;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v.x[i] = v.x[i] + alpha * v.t1v[i];
+}
+
+__global__ void k_ilu_bi_snorm_mark(KState *st) {
+  if (st->snorm_exit) st->done = 1;
+}
+
+// t = A t2v ; omega = (t.s) / (t.t)
+struct EpiIluBiT {
+  static constexpr int NV = 2;
+  EHIP_EPI_MEMBERS(double)
+  __device__ void init() {}
+  struct Pre { double sv; };
+  __device__ Pre load(int i) const { return Pre{v.sv[i]}; }
+  __device__ void row(int i, double s, const Pre &pr, double (&acc)[2]) {
+    v.t[i] = s;
+    acc[0] += s * pr.sv;
+    acc[1] += s * s;
+  }
+  __device__ void finish(double (&tot)[2]) {
+    if (threadIdx.x == 0) st->omega = tot[0] / tot[1];
+  }
+};
+
+// x += alpha t1v + omega t2v ; r = s - omega t
+__global__ void k_ilu_bi_xr(int n, Vecs<double> v, const KState *st) {
+  if (solve_done(st)) return;
+  const double alpha = st->alpha, omega = st->omega;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    v.x[i] = v.x[i] + alpha * v.t1v[i] + omega * v.t2v[i];
+    v.r[i] = v.sv[i] - omega * v.t[i];
+  }
+}
+
+// HUTI CG / BiCGStab preconditioned with ILU(order) of the system's (scaled)
+// values, factorized first as IterSolver does (IterSolve.src:418-520).
+static int ilu_krylov(ehip_system *s, SolveCtx &c, int order) {
+  const int n = s->n;
+  int rc;
+  if ((rc = ilu_setup(s, order))) return rc;
+  if ((rc = ilu_factor(s))) return rc;
+  const bool bicg = c.method == EHIP_BICGSTAB;
+  Vecs<double> v = make_vecs<double>(s->work, n, c.method, s->x, s->rhs);
+  KState *st = s->st;
+  const int g = vec_grid(n), pg = kEpiStride;
+  double *part = s->partials;
+  hipLaunchKernelGGL(k_init_state<double>, dim3(1), dim3(1), 0, s->stream, st, c.tol, c.maxit, c.divtol, 1,
+                     c.du ? c.tol_u : 0.0);
+  if (bicg) hipLaunchKernelGGL(k_ilu_bi_start, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, st);
+  EpiIluInit ei{v, st, part, pg, &st->ticket[0], bicg ? v.rtld : nullptr};
+  launch_epi_fin<double>(s, s->plan64, s->vals, v.x, ei, (const KState *)nullptr);
+  c.spmv++;
+  const KState *chk = st;
+  for (;;) {
+    if (!bicg) {
+      if ((rc = ilu_apply(s, v.r, v.z))) return rc;
+      hipLaunchKernelGGL(k_ilu_dots<1>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v.r, v.z, (const double *)nullptr,
+                         (const double *)nullptr, part, chk);
+      fin<double>(s, FinIluCgRho{st, part, g}, g, chk);
+      hipLaunchKernelGGL(k_ilu_cg_p, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, chk);
+      EpiCgQ<double> eq{v, st, part, pg, &st->ticket[0]};
+      launch_epi_fin<double>(s, s->plan64, s->vals, v.p, eq, chk);
+      hipLaunchKernelGGL(k_ilu_cg_xr, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, chk);
+      c.spmv++;
+    } else {
+      hipLaunchKernelGGL(k_ilu_dots<1>, dim3(g), dim3(kVecBlock), 0, s->stream, n, v.rtld, v.r,
+                         (const double *)nullptr, (const double *)nullptr, part, chk);
+      fin<double>(s, FinIluBiRho{st, part, g}, g, chk);
+      hipLaunchKernelGGL(k_ilu_bi_p, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, chk);
+      if ((rc = ilu_apply(s, v.p, v.t1v))) return rc;
+      EpiIluBiV ev{v, st, part, pg, &st->ticket[0]};
+      launch_epi_fin<double>(s, s->plan64, s->vals, v.t1v, ev, chk);
+      hipLaunchKernelGGL(k_ilu_bi_s, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, chk, part);
+      fin<double>(s, FinIluSnorm{st, part, g}, g, chk);
+      hipLaunchKernelGGL(k_ilu_bi_snorm_x, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, chk);
+      hipLaunchKernelGGL(k_ilu_bi_snorm_mark, dim3(1), dim3(1), 0, s->stream, st);
+      if ((rc = ilu_apply(s, v.sv, v.t2v))) return rc;
+      EpiIluBiT et{v, st, part, pg, &st->ticket[0]};
+      launch_epi_fin<double>(s, s->plan64, s->vals, v.t2v, et, chk);
+      hipLaunchKernelGGL(k_ilu_bi_xr, dim3(g), dim3(kVecBlock), 0, s->stream, n, v, chk);
+      c.spmv += 2;
+    }
+    EpiIluRes er{v, st, part, pg, &st->ticket[0], c.hist, c.du, bicg ? 1 : 0};
+    launch_epi_fin<double>(s, s->plan64, s->vals, v.x, er, chk);
+    c.spmv++;
+    if ((rc = read_state(s, st, s->h_st))) return rc;
+    if (s->h_st->done) break;
+  }
+  EHIP_HIP_TRY(hipGetLastError());
+  EHIP_HIP_TRY(take_launch_error());
+  return EHIP_OK;
+}
+
 // Time marks of a solve: HIP events on the system's stream, or, for callers
 // that create no runtime objects, host clock readings taken after the
 // stream has drained.
@@ -1729,6 +2037,14 @@ int solve_impl(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, 
   if (p->maxit < 1 || !(p->tol > 0)) {
     set_error("ehip_solve: maxit must be >= 1 and tol > 0");
     return EHIP_EARG;
+  }
+  if (p->precond < EHIP_PRECOND_NONE || p->precond > EHIP_PRECOND_ILU0 + 9) {
+    set_error("ehip_solve: unknown preconditioner %d", p->precond);
+    return EHIP_EARG;
+  }
+  if (p->precond >= EHIP_PRECOND_ILU0 && p->precision != EHIP_FP64) {
+    set_error("ehip_solve: ILU preconditioning runs in FP64 only");
+    return EHIP_EUNSUP;
   }
   const int n = s->n;
   int rc;
@@ -1824,6 +2140,8 @@ int solve_impl(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, 
     // FP32 vectors are the workspace; the FP64 workspace holds the residual
     if ((rc = mixed_ir(s, c, itol, imax, &inner_total))) return rc;
     rep.outer_iters = s->h_st->pad1;
+  } else if (p->precond >= EHIP_PRECOND_ILU0) {
+    if ((rc = ilu_krylov(s, c, p->precond - EHIP_PRECOND_ILU0))) return rc;
   } else {
     Vecs<double> v = make_vecs<double>(s->work, n, p->method, s->x, s->rhs);
     hipLaunchKernelGGL(k_make_dg<double>, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals,

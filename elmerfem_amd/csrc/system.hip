@@ -754,6 +754,7 @@ int ehip_system_create(int n, int64_t nnz, const int *rows, const int *cols, con
 int ehip_system_free(ehip_system *s) {
   if (!s) return EHIP_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  ilu_release(s);
   void *ptrs[] = {s->rows,       s->cols,        s->diag,      s->vals,       s->rhs,
                   s->x,          s->vals32,      s->dscale,    s->work,       s->work32,
                   s->partials,   s->st,          s->elements,  s->st32,       s->partials32,

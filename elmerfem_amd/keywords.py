@@ -66,7 +66,15 @@ def _method(name: str | None) -> tuple[int, str]:
     return _lib.BICGSTAB, "bicgstab"  # unknown names run BiCGStab in the reference
 
 
-def _precond(name: str | None) -> int:
+def _ilu_digit(s: str, pos: int) -> int:
+    """ICHAR(str(pos:pos)) - ICHAR('0') clamped to 0 outside 0..9 (IterSolve.src:387-395)."""
+    c = s[pos - 1] if len(s) >= pos else " "
+    k = ord(c) - ord("0")
+    return k if 0 <= k <= 9 else 0
+
+
+def _precond(name: str | None, ilu_order=None, dofs: int = 1, symmetric_ilu=False) -> int:
+    """Prefix decoding of IterSolve.src:372-411."""
     if name is None:
         return _lib.PRECOND_NONE
     s = name.strip().lower()
@@ -74,8 +82,24 @@ def _precond(name: str | None) -> int:
         return _lib.PRECOND_NONE
     if s.startswith("diagonal"):
         return _lib.PRECOND_DIAGONAL
-    raise KeywordError("Linear System Preconditioning = %s is outside the accelerated path "
-                       "(none, diagonal)" % name)
+    if s.startswith("ilut") or s.startswith("multigrid") or s.startswith("vanka"):
+        raise KeywordError("Linear System Preconditioning = %s is outside the accelerated path "
+                           "(none, diagonal, ILU0..ILU9)" % name)
+    if s.startswith("ilu") or s.startswith("bilu"):
+        if s.startswith("bilu"):
+            if dofs > 1:
+                raise KeywordError("Linear System Preconditioning = %s (block ILU for %d DOFs) is outside "
+                                   "the accelerated path" % (name, dofs))
+            k = _ilu_digit(s, 5)  # BILU for one DOF is ILU (IterSolve.src:396-405)
+        elif ilu_order is not None:
+            k = int(round(float(ilu_order)))
+            k = k if 0 <= k <= 9 else 0
+        else:
+            k = _ilu_digit(s, 4)
+        if symmetric_ilu:
+            raise KeywordError("Linear System Symmetric ILU (incomplete Cholesky) is outside the accelerated path")
+        return _lib.PRECOND_ILU0 + k
+    return _lib.PRECOND_NONE  # unknown names: no preconditioning, as the reference warns and does
 
 
 def _bool(v) -> bool:
@@ -84,15 +108,17 @@ def _bool(v) -> bool:
     return bool(v)
 
 
-def from_solver_section(sec: dict) -> LinearSystemKeywords:
-    """Reads a Solver section (keys matched case-insensitively)."""
+def from_solver_section(sec: dict, dofs: int = 1) -> LinearSystemKeywords:
+    """Reads a Solver section (keys matched case-insensitively); dofs is the
+    solver variable's DOFs per node (BILU decoding)."""
     d = {k.strip().lower(): v for k, v in sec.items()}
     k = LinearSystemKeywords()
     k.solver = str(d.get("linear system solver", "iterative")).strip().lower()
     if k.solver != "iterative":
         raise KeywordError("Linear System Solver = %s is outside the accelerated path (iterative)" % k.solver)
     k.method, k.method_name = _method(d.get("linear system iterative method"))
-    k.precond = _precond(d.get("linear system preconditioning"))
+    k.precond = _precond(d.get("linear system preconditioning"), d.get("linear system ilu order"), dofs,
+                         _bool(d.get("linear system symmetric ilu", False)))
     if "linear system convergence tolerance" not in d:
         raise KeywordError("Linear System Convergence Tolerance is required (ListGetCReal, IterSolve.src:361)")
     k.tol = float(d["linear system convergence tolerance"])

@@ -175,6 +175,27 @@ class System:
         _lib.check(_lib.load().ehip_get_solution(self._h, _ptr(x)), "ehip_get_solution")
         return x
 
+    # ------------------------------------------------------------ ILU(k)
+    def ilu_factor(self, order=0):
+        """CRS_IncompleteLU of the current Values: (irows, icols, idiag, factors), 1-based."""
+        L = _lib.load()
+        m = C.c_int64()
+        _lib.check(L.ehip_ilu_factor(self._h, order, C.byref(m), None, None, None, None), "ehip_ilu_factor")
+        ir = np.zeros(self.n + 1, np.int32)
+        ic = np.zeros(m.value, np.int32)
+        idg = np.zeros(self.n, np.int32)
+        f = np.zeros(m.value)
+        _lib.check(L.ehip_ilu_factor(self._h, order, C.byref(m), _ptr(ir), _ptr(ic), _ptr(idg), _ptr(f)),
+                   "ehip_ilu_factor")
+        return ir, ic, idg, f
+
+    def lu_solve(self, b):
+        """CRS_LUPrecondition with the last factors: (LU)^-1 b."""
+        b = _f64(b, self.n)
+        x = np.zeros(self.n)
+        _lib.check(_lib.load().ehip_lu_solve(self._h, _ptr(b), _ptr(x)), "ehip_lu_solve")
+        return x
+
     # ------------------------------------------------------------ SpMV
     def spmv(self, u, precision=0):
         u = _f64(u, self.n)

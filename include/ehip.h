@@ -38,6 +38,12 @@ extern "C" {
 /* Linear System Preconditioning (IterSolve.src:372-411) */
 #define EHIP_PRECOND_NONE 0
 #define EHIP_PRECOND_DIAGONAL 1
+/* ILU(k), k = 0..9 ("ILU0".."ILU9", or "ILU" with "Linear System ILU Order"):
+ * EHIP_PRECOND_ILU0 + k.  CRS_IncompleteLU's non-Cholesky factors of the
+ * scaled matrix (CRSMatrix.src:2157-2500), applied as CRS_LUPrecondition
+ * (:3259-3268, 3293-3397).  FP64 only; the Krylov loop runs in HUTI's order
+ * whatever ehip_params.mode says. */
+#define EHIP_PRECOND_ILU0 2
 /* precision forms */
 #define EHIP_FP64 0
 #define EHIP_MIXED_IR 1 /* FP32 inner Krylov inside FP64 iterative refinement */
@@ -199,6 +205,21 @@ int ehip_solve_csr(int n, const int *rows, const int *cols, const int *diag, con
 #define EHIP_NOALLOC_MAX_NNZ 4194304
 int ehip_solve_csr_noalloc(int n, const int *rows, const int *cols, const int *diag, const double *values,
                            const double *b, double *x, const ehip_params *p, ehip_report *r);
+
+/* ---- ILU(k) factors ----
+ * CRS_IncompleteLU (CRSMatrix.src:2157-2500), non-Cholesky, of the system's
+ * current Values with fill order k = 0..9 (the pattern of InitializeILU1's
+ * k rounds, :2387-2498; the matrix pattern for k = 0).  Copies out the ILU
+ * pattern (1-based, irows[n+1], icols and idiag) and the factors (unit lower
+ * part below idiag, U above it, U(i,i) stored inverted as the reference's
+ * LU solve expects); every pointer may be NULL, *ilu_nnz receives the
+ * pattern size.  ehip_solve factorizes the scaled matrix itself when
+ * params->precond selects ILU(k); this entry exposes what it uses. */
+int ehip_ilu_factor(ehip_system *s, int order, int64_t *ilu_nnz, int *irows, int *icols, int *idiag,
+                    double *ilu);
+/* CRS_LUPrecondition (CRSMatrix.src:3259-3268): x = (LU)^-1 b with the last
+ * factors of ehip_ilu_factor or of an ILU-preconditioned ehip_solve. */
+int ehip_lu_solve(ehip_system *s, const double *b, double *x);
 
 /* ---- measurement helpers ---- */
 /* v = A u on the device-resident system (host vectors), precision 0 = FP64

@@ -1105,6 +1105,137 @@ void or_spmv(int n, const int *rows, const int *cols, const double *values, cons
 }
 
 /* ------------------------------------------------------------------ */
+/* ILU(n) preconditioner                                                */
+/* ------------------------------------------------------------------ */
+
+/* One round of InitializeILU1 (CRSMatrix.src:2387-2498): the pattern of
+ * (rows, cols, diag) plus the fills that an original entry (k, j), k < i,
+ * j > k, of an original lower entry (i, k) of row i brings; only the input
+ * entries propagate within a round.  Returns the new 1-based arrays. */
+static int ilu_fill_round(int n, const int *rows, const int *cols, const int *diag, int **orows, int **ocols,
+                          int **odiag) {
+  unsigned char *C = calloc((size_t)n + 1, 1);
+  int *R = malloc(sizeof(int) * ((size_t)n + 1)), *D = malloc(sizeof(int) * (size_t)n);
+  size_t cap = (size_t)rows[n] - 1 + 16, m = 0;
+  int *K = malloc(sizeof(int) * cap);
+  if (!C || !R || !D || !K) return -1;
+  R[0] = 1;
+  for (int i = 1; i <= n; i++) {
+    for (int k = rows[i - 1]; k < rows[i]; k++) C[cols[k - 1]] = 1;
+    int rmin = cols[rows[i - 1] - 1], rmax = cols[rows[i] - 2];
+    for (int k = rmin; k <= i - 1; k++) {
+      if (C[k] != 1) continue;
+      for (int l = diag[k - 1] + 1; l < rows[k]; l++) {
+        int j = cols[l - 1];
+        if (C[j] == 0) {
+          C[j] = 2;
+          if (j > rmax) rmax = j;
+        }
+      }
+    }
+    for (int k = rmin; k <= rmax; k++) {
+      if (C[k] == 0) continue;
+      C[k] = 0;
+      if (m == cap) {
+        cap *= 2;
+        K = realloc(K, sizeof(int) * cap);
+        if (!K) return -1;
+      }
+      K[m++] = k;
+      if (k == i) D[i - 1] = (int)m;
+    }
+    R[i] = (int)m + 1;
+  }
+  free(C);
+  *orows = R;
+  *ocols = K;
+  *odiag = D;
+  return 0;
+}
+
+/* The ILU(order) pattern: the matrix pattern for order 0, else order rounds
+ * of InitializeILU1, each on the previous round's pattern (CRS_IncompleteLU,
+ * CRSMatrix.src:2194-2219). */
+int or_ilu_pattern(int n, const int *rows, const int *cols, const int *diag, int order, int **irows, int **icols,
+                   int **idiag) {
+  size_t nnz = (size_t)rows[n] - 1;
+  int *R = malloc(sizeof(int) * ((size_t)n + 1)), *K = malloc(sizeof(int) * nnz), *D = malloc(sizeof(int) * n);
+  if (!R || !K || !D) return -1;
+  memcpy(R, rows, sizeof(int) * ((size_t)n + 1));
+  memcpy(K, cols, sizeof(int) * nnz);
+  memcpy(D, diag, sizeof(int) * (size_t)n);
+  for (int r = 0; r < order; r++) {
+    int *R2, *K2, *D2;
+    if (ilu_fill_round(n, R, K, D, &R2, &K2, &D2)) return -1;
+    free(R); free(K); free(D);
+    R = R2; K = K2; D = D2;
+  }
+  *irows = R;
+  *icols = K;
+  *idiag = D;
+  return 0;
+}
+
+void or_free(void *p) { free(p); }
+
+/* CRS_IncompleteLU, real non-Cholesky branch (CRSMatrix.src:2312-2372):
+ * row by row, the row of A scattered into a full work row S (flags C mark
+ * the ILU pattern), eliminated with the finished rows k of its lower ILU
+ * part in column order (S(k) = 0 skipped, division only when
+ * |U(k,k)| > AEPS), stored back; then every U(i,i) is replaced by its
+ * inverse (1 when |U(i,i)| < AEPS).  AEPS = 10*EPSILON(1.0d0) (Types.src:61). */
+int or_ilu_factor(int n, const int *rows, const int *cols, const double *values, const int *irows,
+                  const int *icols, const int *idiag, double *ilu) {
+  const double AEPS = 10.0 * DBL_EPSILON;
+  double *S = calloc((size_t)n + 1, sizeof(double));
+  unsigned char *C = calloc((size_t)n + 1, 1);
+  if (!S || !C) return -1;
+  for (int i = 1; i <= n; i++) {
+    for (int k = rows[i - 1]; k < rows[i]; k++) S[cols[k - 1]] = values[k - 1];
+    for (int k = irows[i - 1]; k < irows[i]; k++) C[icols[k - 1]] = 1;
+    for (int m = irows[i - 1]; m < idiag[i - 1]; m++) {
+      int k = icols[m - 1];
+      if (S[k] == 0.0) continue;
+      if (fabs(ilu[idiag[k - 1] - 1]) > AEPS) S[k] = S[k] / ilu[idiag[k - 1] - 1];
+      for (int l = idiag[k - 1] + 1; l < irows[k]; l++) {
+        int j = icols[l - 1];
+        if (C[j]) S[j] = S[j] - S[k] * ilu[l - 1];
+      }
+    }
+    for (int k = irows[i - 1]; k < irows[i]; k++) {
+      int j = icols[k - 1];
+      if (C[j]) {
+        ilu[k - 1] = S[j];
+        S[j] = 0.0;
+        C[j] = 0;
+      }
+    }
+  }
+  for (int i = 1; i <= n; i++) {
+    double d = ilu[idiag[i - 1] - 1];
+    ilu[idiag[i - 1] - 1] = fabs(d) < AEPS ? 1.0 : 1.0 / d;
+  }
+  free(S);
+  free(C);
+  return 0;
+}
+
+/* CRS_LUSolve, non-Cholesky (CRSMatrix.src:3356-3374): forward with the unit
+ * lower factor, then backward with U and the inverted diagonal. */
+void or_lu_solve(int n, const int *irows, const int *icols, const int *idiag, const double *ilu, double *b) {
+  for (int i = 1; i <= n; i++) {
+    double s = b[i - 1];
+    for (int j = irows[i - 1]; j < idiag[i - 1]; j++) s = s - ilu[j - 1] * b[icols[j - 1] - 1];
+    b[i - 1] = s;
+  }
+  for (int i = n; i >= 1; i--) {
+    double s = b[i - 1];
+    for (int j = idiag[i - 1] + 1; j < irows[i]; j++) s = s - ilu[j - 1] * b[icols[j - 1] - 1];
+    b[i - 1] = ilu[idiag[i - 1] - 1] * s;
+  }
+}
+
+/* ------------------------------------------------------------------ */
 /* HUTI CG / BiCGStab                                                   */
 /* ------------------------------------------------------------------ */
 
@@ -1112,7 +1243,9 @@ typedef struct {
   int n;
   const int *rows, *cols, *diag;
   const double *values;
-  int precond;
+  int precond;  /* 0 none, 1 diagonal, 2 + k ILU(k) */
+  int *irows, *icols, *idiag;
+  double *ilu;
 } sysctx_t;
 
 static void mv(const sysctx_t *c, const double *u, double *v) {
@@ -1125,6 +1258,11 @@ static void pcond(const sysctx_t *c, double *u, const double *v) {
   const double AEPS = 10.0 * DBL_EPSILON;
   if (!c->precond) {
     memcpy(u, v, sizeof(double) * c->n);
+    return;
+  }
+  if (c->precond >= 2) {  /* CRS_LUPrecondition (CRSMatrix.src:3259-3268): u = v, LU solve */
+    memcpy(u, v, sizeof(double) * c->n);
+    or_lu_solve(c->n, c->irows, c->icols, c->idiag, c->ilu, u);
     return;
   }
   for (int i = 0; i < c->n; i++) {
@@ -1260,9 +1398,22 @@ static int huti_bicgstab(const sysctx_t *c, const double *B, double *X, double t
 int or_huti_solve(int method, int precond, int n, const int *rows, const int *cols, const int *diag,
                   const double *values, const double *b, double *x, double tol, int maxit, int stopc,
                   int *iters, double *hist) {
-  sysctx_t c = {n, rows, cols, diag, values, precond};
-  if (method == 0) return huti_cg(&c, b, x, tol, maxit, stopc, iters, hist);
-  return huti_bicgstab(&c, b, x, tol, maxit, stopc, iters, hist);
+  sysctx_t c = {n, rows, cols, diag, values, precond, NULL, NULL, NULL, NULL};
+  if (precond >= 2) {
+    /* IterSolver factorizes the (scaled) matrix it is given before HUTI
+     * starts (IterSolve.src:418-520, CRS_IncompleteLU) */
+    if (precond > 11) return -1;
+    if (or_ilu_pattern(n, rows, cols, diag, precond - 2, &c.irows, &c.icols, &c.idiag)) return -1;
+    c.ilu = malloc(sizeof(double) * ((size_t)c.irows[n] - 1));
+    if (!c.ilu || or_ilu_factor(n, rows, cols, values, c.irows, c.icols, c.idiag, c.ilu)) return -1;
+  }
+  int info = method == 0 ? huti_cg(&c, b, x, tol, maxit, stopc, iters, hist)
+                         : huti_bicgstab(&c, b, x, tol, maxit, stopc, iters, hist);
+  free(c.irows);
+  free(c.icols);
+  free(c.idiag);
+  free(c.ilu);
+  return info;
 }
 
 /* ComputeNorm NormDim 2 (SolverUtils.src:4887-5006). */

@@ -121,7 +121,25 @@ double or_dnrm2(int n, const double *x);                  /* blas/dnrm2.f */
 void or_spmv(int n, const int *rows, const int *cols, const double *values,
              const double *u, double *v);  /* CRS_MatrixVectorProd */
 
-/* method: 0 = CG, 1 = BiCGStab.  precond: 0 none, 1 diagonal.
+/* ---- ILU(n) preconditioner (CRS_IncompleteLU, CRSMatrix.src:2157-2500;
+ * CRS_LUSolve :3293-3397), real non-Cholesky branch ---- */
+/* Pattern of ILU(order): malloc'ed 1-based irows[n+1], icols, idiag[n]
+ * (release with or_free).  Returns 0 on success. */
+int or_ilu_pattern(int n, const int *rows, const int *cols, const int *diag,
+                   int order, int **irows, int **icols, int **idiag);
+void or_free(void *p);
+/* Factors of values (pattern rows/cols) on the ILU pattern: unit lower L
+ * strictly below idiag, U on and above it with U(i,i) stored inverted. */
+int or_ilu_factor(int n, const int *rows, const int *cols,
+                  const double *values, const int *irows, const int *icols,
+                  const int *idiag, double *ilu);
+/* b <- (LU)^-1 b */
+void or_lu_solve(int n, const int *irows, const int *icols, const int *idiag,
+                 const double *ilu, double *b);
+
+/* method: 0 = CG, 1 = BiCGStab.  precond: 0 none, 1 diagonal, 2 + k ILU(k)
+ * for k = 0..9 (the factors of the given values are computed first, as
+ * IterSolver does).
  * stopc: HUTI stopping criterion code (1 = TRESID_SCALED_BYB).
  * hist (length maxit, may be NULL) receives the residual of each
  * iteration.  Returns HUTI_INFO, *iters = HUTI_ITERS. */

@@ -9,11 +9,14 @@
  * dummy copy, pcondr = dummy copy (huti_sfe_c.src:117-124), and the
  * reference BLAS ddot/dnrm2 for dot and norm.  The dummy copy is the
  * reference's own huti_ddummy_pcondfun (hutiter/src/huti_aux_D.F90, built
- * beside the solvers, which also supplies huti_drandvec).  The matvec and
- * diagonal-preconditioner callbacks are this file's restatement (the Fortran
- * CRS module needs Elmer's generated build configuration and is not built
- * here); the Krylov loops, dot, norm and dummy copy that run are the
- * reference's compiled sources.
+ * beside the solvers, which also supplies huti_drandvec).  The matvec,
+ * diagonal-preconditioner and ILU-preconditioner callbacks are this file's
+ * restatement (the Fortran CRS module needs Elmer's generated build
+ * configuration and is not built here); the Krylov loops, dot, norm and
+ * dummy copy that run are the reference's compiled sources.  For ILU
+ * (precond >= 2) the caller hands over the factors (ref_set_ilu) and the
+ * preconditioner is CRS_LUPrecondition: u = v, then CRS_LUSolve
+ * (fem/src/CRSMatrix.src:3259-3268, 3356-3374).
  *
  * The residual history is read off the norm callback: with stopc = 1 HUTI
  * calls normfun once for ||b|| and once per iteration for ||Ax - b||.
@@ -39,7 +42,17 @@ static struct {
   int precond;
   double *norms;
   int nnorm, cap;
+  const int *irows, *icols, *idiag;
+  const double *ilu;
 } G;
+
+/* Factors for precond >= 2 (1-based ILU pattern and values, U(i,i) inverted). */
+void ref_set_ilu(const int *irows, const int *icols, const int *idiag, const double *ilu) {
+  G.irows = irows;
+  G.icols = icols;
+  G.idiag = idiag;
+  G.ilu = ilu;
+}
 
 static void cb_matvec(double *u, double *v, int *ipar) {
   (void)ipar;
@@ -53,6 +66,20 @@ static void cb_matvec(double *u, double *v, int *ipar) {
 static void cb_pcond(double *u, double *v, int *ipar) {
   if (!G.precond) {
     huti_ddummy_pcondfun_(u, v, ipar);
+    return;
+  }
+  if (G.precond >= 2) {
+    for (int i = 0; i < G.n; i++) u[i] = v[i];
+    for (int i = 1; i <= G.n; i++) {
+      double s = u[i - 1];
+      for (int j = G.irows[i - 1]; j < G.idiag[i - 1]; j++) s = s - G.ilu[j - 1] * u[G.icols[j - 1] - 1];
+      u[i - 1] = s;
+    }
+    for (int i = G.n; i >= 1; i--) {
+      double s = u[i - 1];
+      for (int j = G.idiag[i - 1] + 1; j < G.irows[i]; j++) s = s - G.ilu[j - 1] * u[G.icols[j - 1] - 1];
+      u[i - 1] = G.ilu[G.idiag[i - 1] - 1] * s;
+    }
     return;
   }
   const double AEPS = 10.0 * DBL_EPSILON;

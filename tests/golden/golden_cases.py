@@ -14,6 +14,13 @@ CASES = {
     "cg_elastic3": {"wl": ("elastic", 3), "method": 0, "precond": 1, "tol": 1e-10, "maxit": 2000},
     "cg_tet4_4": {"wl": ("tet4", 4), "method": 0, "precond": 1, "tol": 1e-10, "maxit": 1000},
     "cg_maxit_lognormal8": {"wl": ("poisson", 8, 1.0), "method": 0, "precond": 1, "tol": 1e-10, "maxit": 7},
+    # ILU(k) preconditioning: precond = 2 + k
+    "cg_ilu0_lognormal8": {"wl": ("poisson", 8, 1.0), "method": 0, "precond": 2, "tol": 1e-10, "maxit": 1000},
+    "cg_ilu1_poisson8": {"wl": ("poisson", 8, 0.0), "method": 0, "precond": 3, "tol": 1e-10, "maxit": 1000},
+    "cg_ilu0_poisson32": {"wl": ("poisson", 32, 0.0), "method": 0, "precond": 2, "tol": 1e-10, "maxit": 2000},
+    "cg_ilu0_elastic3": {"wl": ("elastic", 3), "method": 0, "precond": 2, "tol": 1e-10, "maxit": 2000},
+    "bicgstab_ilu0_convdiff16": {"wl": ("convdiff", 16), "method": 1, "precond": 2, "tol": 1e-10, "maxit": 1000},
+    "bicgstab_ilu2_convdiff16": {"wl": ("convdiff", 16), "method": 1, "precond": 4, "tol": 1e-10, "maxit": 1000},
 }
 
 

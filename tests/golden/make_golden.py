@@ -5,7 +5,7 @@ compiled from the reference sources into oracle/_ref (make -C oracle ref),
 on the scaled systems IterSolver would see for small workloads, and stores
 iteration counts, HUTI_INFO, per-iteration residuals and solutions.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]     # all cases, or the named ones
 """
 import os
 import sys
@@ -23,7 +23,9 @@ from golden_cases import CASES, scaled_case  # noqa: E402
 def main():
     if not O.ref_available():
         sys.exit("oracle/_ref/libhuti_ref.so is not built (make -C oracle ref)")
-    for name, spec in CASES.items():
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
+        spec = CASES[name]
         pat, v, b, x0 = scaled_case(spec)
         info, k, x, hist = O.ref_huti_solve(pat.rows, pat.cols, pat.diag, v, b, x0, method=spec["method"],
                                             precond=spec["precond"], tol=spec["tol"], maxit=spec["maxit"])

@@ -63,6 +63,10 @@ def lib():
                                              C.c_double, C.c_int, vp, vp, vp]
         L.or_compute_norm.argtypes = [C.c_int, vp]
         L.or_compute_norm.restype = C.c_double
+        L.or_ilu_pattern.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp]
+        L.or_free.argtypes = [vp]
+        L.or_ilu_factor.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]
+        L.or_lu_solve.argtypes = [C.c_int, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -81,6 +85,7 @@ def ref():
         R.ref_ddot.restype = C.c_double
         R.ref_dnrm2.argtypes = [C.c_int, vp]
         R.ref_dnrm2.restype = C.c_double
+        R.ref_set_ilu.argtypes = [vp, vp, vp, vp]
         _ref = R
     return _ref
 
@@ -224,11 +229,47 @@ def huti_solve(rows, cols, diag, values, b, x0, method=0, precond=1, tol=1e-10, 
     return info, k, x, hist[:k]
 
 
+def ilu_pattern(rows, cols, diag, order):
+    """1-based ILU(order) pattern (irows, icols, idiag) of a CRS pattern."""
+    n = len(rows) - 1
+    pr, pc, pd = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    rc = lib().or_ilu_pattern(n, _p(rows), _p(cols), _p(diag), order, C.byref(pr), C.byref(pc), C.byref(pd))
+    assert rc == 0
+    ir = np.ctypeslib.as_array(C.cast(pr, C.POINTER(C.c_int)), (n + 1,)).copy()
+    ic = np.ctypeslib.as_array(C.cast(pc, C.POINTER(C.c_int)), (int(ir[n]) - 1,)).copy()
+    idg = np.ctypeslib.as_array(C.cast(pd, C.POINTER(C.c_int)), (n,)).copy()
+    for q in (pr, pc, pd):
+        lib().or_free(q)
+    return ir, ic, idg
+
+
+def ilu_factor(rows, cols, values, irows, icols, idiag):
+    """CRS_IncompleteLU factors on the given ILU pattern (U(i,i) stored inverted)."""
+    n = len(rows) - 1
+    ilu = np.zeros(int(irows[n]) - 1)
+    rc = lib().or_ilu_factor(n, _p(rows), _p(cols), _p(np.ascontiguousarray(values, np.float64)), _p(irows),
+                             _p(icols), _p(idiag), _p(ilu))
+    assert rc == 0
+    return ilu
+
+
+def lu_solve(irows, icols, idiag, ilu, b):
+    x = np.ascontiguousarray(b, np.float64).copy()
+    lib().or_lu_solve(len(irows) - 1, _p(irows), _p(icols), _p(idiag), _p(ilu), _p(x))
+    return x
+
+
 def ref_huti_solve(rows, cols, diag, values, b, x0, method=0, precond=1, tol=1e-10, maxit=10000):
     n = len(rows) - 1
     x = np.ascontiguousarray(x0, np.float64).copy()
     it = np.zeros(1, np.int32)
     hist = np.zeros(maxit)
+    keep = None
+    if precond >= 2:
+        # the factors IterSolver would compute, handed to the driver's CRS_LUPrecondition
+        ir, ic, idg = ilu_pattern(rows, cols, diag, precond - 2)
+        keep = (ir, ic, idg, ilu_factor(rows, cols, values, ir, ic, idg))
+        ref().ref_set_ilu(*[_p(a) for a in keep])
     info = ref().ref_huti_solve(method, precond, n, _p(rows), _p(cols), _p(diag), _p(values), _p(b), _p(x), tol,
                                 maxit, _p(it), _p(hist))
     k = int(it[0])

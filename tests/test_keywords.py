@@ -39,8 +39,31 @@ def test_other_methods_refused(name):
 def test_preconditioning():
     assert from_solver_section(sec(**{"Linear System Preconditioning": "Diagonal"})).precond == _lib.PRECOND_DIAGONAL
     assert from_solver_section(sec(**{"Linear System Preconditioning": "None"})).precond == _lib.PRECOND_NONE
+    # unknown names fall back to none (IterSolve.src:408-410)
+    assert from_solver_section(sec(**{"Linear System Preconditioning": "foo"})).precond == _lib.PRECOND_NONE
+    for name in ("ILUT", "Multigrid", "Vanka"):
+        with pytest.raises(KeywordError):
+            from_solver_section(sec(**{"Linear System Preconditioning": name}))
+
+
+@pytest.mark.parametrize("name,extra,dofs,k", [
+    ("ILU0", {}, 1, 0), ("ILU1", {}, 1, 1), ("ilu3", {}, 1, 3), ("ILU9", {}, 1, 9),
+    ("ILU", {}, 1, 0),                                   # str(4:4) = ' ' -> -16 -> 0
+    ("ILUx", {}, 1, 0),                                  # not a digit -> 0
+    ("ILU", {"Linear System ILU Order": "2"}, 1, 2),     # the order keyword wins over the name
+    ("ILU1", {"Linear System ILU Order": "12"}, 1, 0),   # out of 0..9 -> 0
+    ("BILU2", {}, 1, 2),                                 # block ILU of one DOF is ILU
+])
+def test_ilu_orders(name, extra, dofs, k):
+    kw = from_solver_section(sec(**dict({"Linear System Preconditioning": name}, **extra)), dofs=dofs)
+    assert kw.precond == _lib.PRECOND_ILU0 + k
+
+
+def test_ilu_outside_path():
     with pytest.raises(KeywordError):
-        from_solver_section(sec(**{"Linear System Preconditioning": "ILU0"}))
+        from_solver_section(sec(**{"Linear System Preconditioning": "BILU0"}), dofs=3)
+    with pytest.raises(KeywordError):
+        from_solver_section(sec(**{"Linear System Preconditioning": "ILU0", "Linear System Symmetric ILU": "True"}))
 
 
 def test_required_keywords_and_direct_solver():
