@@ -78,12 +78,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sys_, kw, iters: int):
-    """The oracle's HUTI CG (serial C restatement of the reference loop with
-    reference-BLAS ddot/dnrm2) on the same scaled system, for a bounded
-    number of iterations on one host core."""
+# Time of the oracle's HUTI CG over the reference's own HUTI CG + BLAS
+# compiled from its sources, same system and threads: python
+# oracle/time_ratio.py --n 64 --threads 8 in the build container (8-core
+# Xeon @ 2.70 GHz) printed 2.078 s / 1.904 s.
+PORT_VS_REFERENCE = 1.091
+
+
+def cpu_baseline(sys_, kw, iters: int, asm_n: int):
+    """The reference CPU path restated (the oracle: HUTI CG with
+    reference-BLAS ddot/dnrm2 and CRS_MatrixVectorProd's row-parallel SpMV)
+    on all host threads: the full CG + Jacobi solve of the same scaled
+    system to 1e-10 (or `iters` iterations when > 0), and the serial element
+    loop + glue of an asm_n^3 sample of the same mesh family."""
     sys.path.insert(0, os.path.join(HERE, "tests"))
     import oracle_lib as O
+    from elmerfem_amd import workloads as W
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    threads = O.lib().or_set_threads(threads)
     perm, rows, cols, diag = sys_.pattern()
     vals, rhs = sys_.values()
 
@@ -93,13 +105,26 @@ def cpu_baseline(sys_, kw, iters: int):
     pat.n, pat.rows, pat.cols, pat.diag = sys_.n, rows, cols, diag
     v, b, x0 = O.scaled_system(pat, vals, rhs)
     t0 = time.perf_counter()
-    info, k, x, hist = O.huti_solve(rows, cols, diag, v, b, x0, method=0, precond=1, tol=1e-300, maxit=iters)
+    tol, maxit = (1e-300, iters) if iters > 0 else (1e-10, 20000)
+    info, k, x, hist = O.huti_solve(rows, cols, diag, v, b, x0, method=0, precond=1, tol=tol, maxit=maxit)
     dt = time.perf_counter() - t0
-    done = min(k, iters)
-    return {"value": sys_.n * done / dt, "unit": "DOF-iter/s", "cores": 1, "kind": "port",
-            "sample": "%d HUTI-faithful CG iterations (2 SpMV + ddot/dnrm2 each) of the 128^3 system, "
-                      "serial oracle restatement, %.1f s" % (done, dt),
-            "seconds": dt}
+    done = min(k, maxit)
+    # CPU assembly on a smaller mesh of the same family
+    w = W.poisson_hex8(asm_n, sigma=1.0, seed=20261202)
+    m = w["mesh"]
+    apat = O.Pattern(m.elements, m.nnodes, 1, optimize=True, extra_elements=[m.bnd_elements])
+    t1 = time.perf_counter()
+    O.assemble(0, m, apat, w["coef"], load=w["load"])
+    ta = time.perf_counter() - t1
+    what = ("full HUTI-faithful CG + Jacobi solve to 1e-10 (%d iterations, info %d)" % (done, info) if iters <= 0
+            else "%d HUTI-faithful CG iterations" % done)
+    return {"value": sys_.n * done / dt, "unit": "DOF-iter/s", "cores": threads, "kind": "port",
+            "sample": "%s of the same scaled system, 2 SpMV (row-parallel) + reference-BLAS ddot/dnrm2 per "
+                      "iteration, oracle restatement on %d host threads, %.1f s; port/reference time ratio %.3f "
+                      "(oracle/time_ratio.py, 8 threads)" % (what, threads, dt, PORT_VS_REFERENCE),
+            "seconds": dt, "iters": done, "port_vs_reference": PORT_VS_REFERENCE,
+            "assembly": {"elements_per_s": m.nelem / ta, "seconds": ta, "threads": 1,
+                         "sample": "serial element loop + CRS glue (ModelPDE Hex8, %d^3 elements)" % asm_n}}
 
 
 def form_batch(args):
@@ -301,6 +326,38 @@ def form_serve(args):
     return 0
 
 
+def form_ilu(args):
+    """FP64 CG preconditioned with ILU(k) (--ilu-order) on the n^3 Hex8
+    Poisson case (lognormal(0, sigma) conductivity), to 1e-10 in HUTI's loop
+    order, against the same solve with Jacobi."""
+    from elmerfem_amd import _lib, workloads as W
+    _lib.init(0)
+    w = W.poisson_hex8(args.n, sigma=args.sigma, seed=args.seed)
+    sys_, _ = W.build_system(w, optimize_bandwidth=True)
+    out = {}
+    for name, pc in (("ilu%d" % args.ilu_order, "ilu%d" % args.ilu_order), ("diagonal", "diagonal")):
+        kw = W._kw("cg", pc, 1e-10, 20000)
+        for rep in range(1 + args.steps):
+            t = time.perf_counter()
+            _, r = sys_.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False, true_residual_stop=True)
+            wall = 1000.0 * (time.perf_counter() - t)
+            log("%-9s info=%d iters=%d res_true=%.3e wall=%.1f ms" % (name, r["info"], r["iters"], r["res_true"], wall))
+        out[name] = {"iters": r["iters"], "ms": wall, "info": r["info"], "res_true": r["res_true"],
+                     "dof_iters_per_s": sys_.n * r["iters"] / (wall / 1000.0)}
+    lf, lb = sys_.ilu_levels()
+    out["ilu%d" % args.ilu_order].update({"levels_forward": lf, "levels_backward": lb})
+    head = out["ilu%d" % args.ilu_order]
+    rec = {"metric": "time to 1e-10, FP64 CG + ILU(%d) (HUTI order)" % args.ilu_order, "value": head["ms"],
+           "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": head["ms"],
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "poisson_hex8_%d_lognormal" % args.n, "dofs": sys_.n, "nnz": sys_.nnz,
+                      "sigma": args.sigma, "seed": args.seed, "ilu_order": args.ilu_order},
+           "forms": out}
+    print(json.dumps(rec), flush=True)
+    sys_.free()
+    return 0
+
+
 def form_elastic(args):
     """Precision form on 3-D linear elasticity: structured Hex8 cube (or Hex27
     with --quadratic), E = 1, nu = 0.3, x = 0 clamped, traction (0, 0, -1e-3)
@@ -356,10 +413,11 @@ def form_elastic(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", choices=["headline", "batch", "serve", "elastic"], default="headline",
+    ap.add_argument("--form", choices=["headline", "batch", "serve", "elastic", "ilu"], default="headline",
                     help="headline: the 128^3 solve; batch / serve: the batching and serving forms; "
                          "elastic: the precision form on elasticity (--n 96, or --quadratic --n 48)")
     ap.add_argument("--quadratic", action="store_true", help="elastic form: Hex27 elements")
+    ap.add_argument("--ilu-order", type=int, default=0, help="ilu form: fill order k of ILU(k)")
     ap.add_argument("--inner-tol", type=float, default=0.0,
                     help="mixed CG: residual drop between FP64 replacements (0: the library default)")
     ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
@@ -378,7 +436,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--sigma", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=20261202)
-    ap.add_argument("--cpu-iters", type=int, default=150)
+    ap.add_argument("--cpu-iters", type=int, default=0, help="CPU baseline: 0 = full solve, else iterations")
+    ap.add_argument("--cpu-asm-n", type=int, default=64, help="CPU baseline: assembly sample mesh (n^3)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-forms", action="store_true", help="time only the headline form")
     args = ap.parse_args()
@@ -391,6 +450,8 @@ def main():
         return form_batch(args)
     if args.form == "serve":
         return form_serve(args)
+    if args.form == "ilu":
+        return form_ilu(args)
     if args.form == "elastic":
         if args.n == 128:
             args.n = 48 if args.quadratic else 96
@@ -479,6 +540,25 @@ def main():
         for name in ("fp64_fast", "mixed_ir"):
             forms[name]["rel_err_vs_fp64"] = float(np.abs(sols[name] - ref).max() / np.abs(ref).max())
 
+    # FP64 CG preconditioned with ILU(0) (HUTI order, level-scheduled LU solves)
+    if not args.skip_forms:
+        kw_ilu = W._kw("cg", "ilu0", 1e-10, 20000)
+        reps = []
+        for _ in range(2):
+            t = time.perf_counter()
+            _, r = sys_.solve(kw_ilu, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False,
+                              true_residual_stop=True)
+            r["wall_ms"] = 1000.0 * (time.perf_counter() - t)
+            reps.append(r)
+            log("%-14s info=%d iters=%d res_true=%.3e wall=%.1f ms krylov=%.1f ms" % (
+                "fp64_ilu0", r["info"], r["iters"], r["res_true"], r["wall_ms"], r["t_krylov_ms"]))
+        r = reps[-1]
+        lev = sys_.ilu_levels()
+        forms["fp64_ilu0"] = {"iters": r["iters"], "ms": r["wall_ms"], "krylov_ms": r["t_krylov_ms"],
+                              "res_scaled": r["res_scaled"], "res_true": r["res_true"], "info": r["info"],
+                              "dof_iters_per_s": n * r["iters"] / (r["wall_ms"] / 1000.0),
+                              "levels_forward": lev[0], "levels_backward": lev[1]}
+
     # roofline of the dominant kernel: the fused FP32 SpMV of the inner CG,
     # its launches sampled with HIP events on the system's stream in the timed solves
     c32 = sys_.spmv_config(1)
@@ -494,31 +574,44 @@ def main():
         kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
                                                                   "ColsU16" if bits == 16 else "ColsI32")
     us = float(np.mean([r["spmv_us"] for r in sampled if r["spmv_samples"] > 0]))
+    # algorithmic bytes of one fused FP32 CG step: FP32 values, int32 column
+    # indices, row pointers, z read, q and p read and written
+    alg32 = 8 * nnz + 4 * (n + 1) + 20 * n
+    # what this layout reads for the same work (16-bit column offsets, no row pointers)
     bytes32 = fused_cg32_bytes(nnz, n, bits, c32["format"], c32["chunks"])
-    achieved = bytes32 / (us * 1e-6) / 1e9
+    achieved = alg32 / (us * 1e-6) / 1e9
     traffic = measured_traffic(kname)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": kname + " (fused FP32 SpMV + CG update)",
-                "traffic_source": "profiles/hbm_traffic_poisson_hex8_128.json" if traffic else None,
-                "bytes_per_launch": bytes32, "mean_us": us,
-                "samples": int(sum(r["spmv_samples"] for r in sampled))}
+                "bytes_per_launch": alg32, "mean_us": us,
+                "samples": int(sum(r["spmv_samples"] for r in sampled)),
+                "layout_bytes_per_launch": bytes32, "layout_gbs": bytes32 / (us * 1e-6) / 1e9,
+                "physical_gbs": traffic / (us * 1e-6) / 1e9 if traffic else None,
+                "physical_frac": traffic / (us * 1e-6) / 1e9 / HBM_PEAK_GBS if traffic else None,
+                "traffic_source": "profiles/hbm_traffic_poisson_hex8_128.json" if traffic else None}
     c64 = sys_.spmv_config(0)
     b64 = spmv_bytes(nnz, n, False, c64["index_bits"], c64["format"], c64["chunks"])
     b32 = spmv_bytes(nnz, n, True, bits, c32["format"], c32["chunks"])
-    spmv = {"fp64_us": spmv64_us, "fp64_gbs": b64 / (spmv64_us * 1e-6) / 1e9, "fp64_bytes": b64,
-            "fp32_us": spmv32_us, "fp32_gbs": b32 / (spmv32_us * 1e-6) / 1e9, "fp32_bytes": b32,
-            "fp64_gbs_int32_equiv": spmv_bytes(nnz, n) / (spmv64_us * 1e-6) / 1e9,
+    alg64 = 12 * nnz + 4 * (n + 1) + 16 * n   # FP64 values, int32 indices, row pointers, x, y
+    spmv = {"fp64_us": spmv64_us, "fp64_alg_bytes": alg64, "fp64_gbs": alg64 / (spmv64_us * 1e-6) / 1e9,
+            "fp64_layout_bytes": b64, "fp64_layout_gbs": b64 / (spmv64_us * 1e-6) / 1e9,
+            "fp32_us": spmv32_us, "fp32_alg_bytes": 8 * nnz + 4 * (n + 1) + 8 * n,
+            "fp32_gbs": (8 * nnz + 4 * (n + 1) + 8 * n) / (spmv32_us * 1e-6) / 1e9,
+            "fp32_layout_bytes": b32, "fp32_layout_gbs": b32 / (spmv32_us * 1e-6) / 1e9,
             "index_bits": bits, "copy_gbs_measured": copy_gbs,
             "config64": sys_.spmv_config(0), "config32": sys_.spmv_config(1)}
     spmv["fp64_frac_peak"] = spmv["fp64_gbs"] / HBM_PEAK_GBS
+    spmv["fp64_layout_frac_peak"] = spmv["fp64_layout_gbs"] / HBM_PEAK_GBS
     assembly = {"ms": t_asm_ms, "elements_per_s": m.nelem / (t_asm_ms / 1000.0),
                 "gbs_algorithmic": asm_bytes / (t_asm_ms * 1e-3) / 1e9, "host_setup_s": t_setup}
     cpu = None
     if not args.skip_cpu:
-        log("cpu baseline: %d oracle CG iterations ..." % args.cpu_iters)
-        cpu = cpu_baseline(sys_, kw, args.cpu_iters)
-        log("cpu baseline %.3e DOF-iter/s (%.1fs)" % (cpu["value"], cpu["seconds"]))
+        log("cpu baseline: oracle CG (%s) ..." % ("full solve" if args.cpu_iters <= 0 else
+                                                     "%d iterations" % args.cpu_iters))
+        cpu = cpu_baseline(sys_, kw, args.cpu_iters, args.cpu_asm_n)
+        log("cpu baseline %.3e DOF-iter/s on %d threads (%.1fs), assembly %.3e elements/s" % (
+            cpu["value"], cpu["cores"], cpu["seconds"], cpu["assembly"]["elements_per_s"]))
     rec = {
         "metric": "DOF-iters/s (mixed-precision refined CG to 1e-10) + SpMV HBM GB/s",
         "value": value, "unit": "DOF-iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -82,6 +82,7 @@ SIGNATURES = {
                                          C.POINTER(Report)]),
     "ehip_ilu_factor": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ehip_lu_solve": (C.c_int, [_vp, _vp, _vp]),
+    "ehip_ilu_info": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ehip_spmv": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),

@@ -386,3 +386,13 @@ extern "C" int ehip_lu_solve(ehip_system *s, const double *b, double *x) {
   EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
   return EHIP_OK;
 }
+
+extern "C" int ehip_ilu_info(const ehip_system *s, int *order, int64_t *ilu_nnz, int *levels_forward,
+                             int *levels_backward) {
+  if (!s) return EHIP_EARG;
+  if (order) *order = s->ilu_order;
+  if (ilu_nnz) *ilu_nnz = s->ilu_nnz;
+  if (levels_forward) *levels_forward = s->ilu_loff.empty() ? 0 : (int)s->ilu_loff.size() - 1;
+  if (levels_backward) *levels_backward = s->ilu_uoff.empty() ? 0 : (int)s->ilu_uoff.size() - 1;
+  return EHIP_OK;
+}

@@ -189,6 +189,12 @@ class System:
                    "ehip_ilu_factor")
         return ir, ic, idg, f
 
+    def ilu_levels(self):
+        """(forward, backward) dependency levels of the current ILU factors."""
+        lf, lb = C.c_int(), C.c_int()
+        _lib.check(_lib.load().ehip_ilu_info(self._h, None, None, C.byref(lf), C.byref(lb)), "ehip_ilu_info")
+        return lf.value, lb.value
+
     def lu_solve(self, b):
         """CRS_LUPrecondition with the last factors: (LU)^-1 b."""
         b = _f64(b, self.n)

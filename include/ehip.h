@@ -220,6 +220,11 @@ int ehip_ilu_factor(ehip_system *s, int order, int64_t *ilu_nnz, int *irows, int
 /* CRS_LUPrecondition (CRSMatrix.src:3259-3268): x = (LU)^-1 b with the last
  * factors of ehip_ilu_factor or of an ILU-preconditioned ehip_solve. */
 int ehip_lu_solve(ehip_system *s, const double *b, double *x);
+/* The factors' shape: fill order (-1: none yet), pattern size and the
+ * number of dependency levels of the forward and backward sweeps (one
+ * launch each per LU solve).  Any pointer may be NULL.  No counterpart in
+ * the reference (its LU solve runs row by row on one core). */
+int ehip_ilu_info(const ehip_system *s, int *order, int64_t *ilu_nnz, int *levels_forward, int *levels_backward);
 
 /* ---- measurement helpers ---- */
 /* v = A u on the device-resident system (host vectors), precision 0 = FP64

@@ -30,6 +30,9 @@
 #include "oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
@@ -1096,7 +1099,11 @@ double or_dnrm2(int n, const double *x) {
 }
 
 /* CRS_MatrixVectorProd non-transposed branch (CRSMatrix.src:3565-3573). */
+/* CRS_MatrixVectorProd (CRSMatrix.src:3515-3640): rows split over the
+ * threads like its !$omp parallel do (:3565-3573); each row is summed left
+ * to right from zero, so the bits do not depend on the thread count. */
 void or_spmv(int n, const int *rows, const int *cols, const double *values, const double *u, double *v) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; i++) {
     double s = 0.0;
     for (int j = rows[i]; j < rows[i + 1]; j++) s = s + values[j - 1] * u[cols[j - 1] - 1];
@@ -1414,6 +1421,17 @@ int or_huti_solve(int method, int precond, int n, const int *rows, const int *co
   free(c.idiag);
   free(c.ilu);
   return info;
+}
+
+/* Threads of the row-parallel SpMV (the reference's OMP_NUM_THREADS). */
+int or_set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  return omp_get_max_threads();
+#else
+  (void)nthreads;
+  return 1;
+#endif
 }
 
 /* ComputeNorm NormDim 2 (SolverUtils.src:4887-5006). */

@@ -157,6 +157,9 @@ int or_solve_linear_system(int method, int precond, int scaling, int n,
                            double *values, double *b, double *x, double tol,
                            int maxit, int *iters, double *hist, double *norm);
 
+/* Threads of or_spmv's row loop (0: keep); returns the count in use. */
+int or_set_threads(int nthreads);
+
 /* ComputeNorm, NormDim 2 (SolverUtils.src:4887-5006). */
 double or_compute_norm(int n, const double *x);
 

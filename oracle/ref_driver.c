@@ -56,6 +56,7 @@ void ref_set_ilu(const int *irows, const int *icols, const int *idiag, const dou
 
 static void cb_matvec(double *u, double *v, int *ipar) {
   (void)ipar;
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < G.n; i++) {
     double s = 0.0;
     for (int j = G.rows[i]; j < G.rows[i + 1]; j++) s = s + G.values[j - 1] * u[G.cols[j - 1] - 1];

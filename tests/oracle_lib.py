@@ -63,6 +63,7 @@ def lib():
                                              C.c_double, C.c_int, vp, vp, vp]
         L.or_compute_norm.argtypes = [C.c_int, vp]
         L.or_compute_norm.restype = C.c_double
+        L.or_set_threads.argtypes = [C.c_int]
         L.or_ilu_pattern.argtypes = [C.c_int, vp, vp, vp, C.c_int, vp, vp, vp]
         L.or_free.argtypes = [vp]
         L.or_ilu_factor.argtypes = [C.c_int, vp, vp, vp, vp, vp, vp, vp]

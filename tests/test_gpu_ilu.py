@@ -43,6 +43,8 @@ def test_ilu_factors_bit_exact(ehip, case, order):
     rng = np.random.default_rng(order)
     r = rng.standard_normal(pat.n)
     assert np.array_equal(s.lu_solve(r), O.lu_solve(oir, oic, oidg, of, r))
+    lf, lb = s.ilu_levels()
+    assert 1 <= lf <= pat.n and 1 <= lb <= pat.n
     s.free()
 
 
