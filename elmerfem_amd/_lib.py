@@ -72,6 +72,14 @@ SIGNATURES = {
     "ehip_system_set_values": (C.c_int, [_vp, _vp, _vp]),
     "ehip_system_get_values": (C.c_int, [_vp, _vp, _vp]),
     "ehip_system_bytes": (_i64, [_vp]),
+    "ehip_mesh_read": (C.c_int, [C.c_char_p, _vp]),
+    "ehip_mesh_sizes": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_mesh_get": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ehip_mesh_free": (C.c_int, [_vp]),
+    "ehip_mesh_write": (C.c_int, [C.c_char_p, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, _vp,
+                                  _vp]),
+    "ehip_result_write": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, _vp, C.c_int, _vp, C.c_int,
+                                    C.c_int, C.c_double]),
     "ehip_assemble": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp]),
     "ehip_assemble_traction": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, _vp, _vp]),
     "ehip_dirichlet": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int]),

@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 HIP_SOURCES = ["system.hip", "krylov.hip", "assembly.hip", "batch.hip", "noalloc.hip", "ilu.hip"]
-CXX_SOURCES = ["pattern.cpp"]
+CXX_SOURCES = ["pattern.cpp", "meshio.cpp"]
 
 # Packed FP32 instructions are disabled for every HIP translation unit.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-munsafe-fp-atomics",

@@ -90,16 +90,26 @@ static bool make_etype(int code, EType &t) {
   static const double Q9U[9] = {-1, 1, 1, -1, 0, 1, 0, -1, 0}, Q9V[9] = {-1, -1, 1, 1, -1, 0, 1, 0, 0};
   static const int Q9B[9] = {1, 2, 3, 5, 6, 7, 9, 10, 11};
   static const int B404[4] = {1, 2, 5, 6}, B504[4] = {1, 2, 5, 17}, B808[8] = {1, 2, 5, 6, 17, 18, 21, 22}, B202[2] = {1, 2};
+  static const int B303[3] = {1, 2, 5};
+  static const double T10U[10] = {0, 1, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0};
+  static const double T10V[10] = {0, 0, 1, 0, 0, 0.5, 0.5, 0, 0, 0.5};
+  static const double T10W[10] = {0, 0, 0, 1, 0, 0, 0, 0.5, 0.5, 0.5};
+  static const int B510[10] = {1, 2, 3, 5, 6, 9, 17, 18, 21, 33};
   const int *B = nullptr;
   t.code = code;
   switch (code) {
     case 202: t.dim = 1; t.npe = 2; B = B202; t.U = {-1, 1}; t.V = {0, 0}; t.W = {0, 0}; break;
+    case 303: t.dim = 2; t.npe = 3; B = B303; t.U = {0, 1, 0}; t.V = {0, 0, 1}; t.W = {0, 0, 0}; break;
     case 404: t.dim = 2; t.npe = 4; B = B404; t.U = {-1, 1, 1, -1}; t.V = {-1, -1, 1, 1}; t.W = {0, 0, 0, 0}; break;
     case 409: t.dim = 2; t.npe = 9; B = Q9B; t.U.assign(Q9U, Q9U + 9); t.V.assign(Q9V, Q9V + 9); t.W.assign(9, 0.0); break;
     case 504: t.dim = 3; t.npe = 4; B = B504; t.U = {0, 1, 0, 0}; t.V = {0, 0, 1, 0}; t.W = {0, 0, 0, 1}; break;
     case 808:
       t.dim = 3; t.npe = 8; B = B808;
       t.U = {-1, 1, 1, -1, -1, 1, 1, -1}; t.V = {-1, -1, 1, 1, -1, -1, 1, 1}; t.W = {-1, -1, -1, -1, 1, 1, 1, 1};
+      break;
+    case 510:
+      t.dim = 3; t.npe = 10; B = B510;
+      t.U.assign(T10U, T10U + 10); t.V.assign(T10V, T10V + 10); t.W.assign(T10W, T10W + 10);
       break;
     case 827:
       t.dim = 3; t.npe = 27; B = H27B;
@@ -140,6 +150,11 @@ static bool make_etype(int code, EType &t) {
   if (t.dim == 1) {
     gauss1d(2, pts, wts);
     for (int i = 0; i < 2; i++) { t.gu.push_back(pts[i]); t.gv.push_back(0); t.gw.push_back(0); t.gs.push_back(wts[i]); }
+  } else if (code == 303) {
+    // GaussPointsTriangle, 3 points (Integration.src:84-92, 776-781)
+    static const double u3[3] = {0.16666666666666667, 0.66666666666666667, 0.16666666666666667};
+    static const double v3[3] = {0.16666666666666667, 0.16666666666666667, 0.66666666666666667};
+    for (int i = 0; i < 3; i++) { t.gu.push_back(u3[i]); t.gv.push_back(v3[i]); t.gw.push_back(0); t.gs.push_back(0.33333333333333333 / 2.0); }
   } else if (t.dim == 2) {
     const int m = code == 409 ? 3 : 2;
     gauss1d(m, pts, wts);
@@ -151,13 +166,26 @@ static bool make_etype(int code, EType &t) {
     static const double w4[4] = {0.2180665126782654, 0.5635595064952189, 0.0350112499848832, 0.1833627308416330};
     for (int i = 0; i < 4; i++) { t.gu.push_back(u4[i]); t.gv.push_back(v4[i]); t.gw.push_back(w4[i]); t.gs.push_back(0.25 / 6.0); }
   } else {
-    const int m = code == 827 ? 3 : 2;
+    const int m = (code == 827 || code == 510) ? 3 : 2;
     gauss1d(m, pts, wts);
     for (int i = 0; i < m; i++)
       for (int j = 0; j < m; j++)
         for (int k = 0; k < m; k++) {
-          t.gu.push_back(pts[k]); t.gv.push_back(pts[j]); t.gw.push_back(pts[i]);
-          t.gs.push_back(wts[i] * wts[j] * wts[k]);
+          double u = pts[k], v = pts[j], w = pts[i], sw = wts[i] * wts[j] * wts[k];
+          if (code == 510) {
+            // Tet10's 27 points: the brick rule collapsed onto the tetrahedron
+            // (GaussPointsTetra, Integration.src:944-963)
+            double f = 0.5;
+            u = (u + 1) * f; v = (v + 1) * f; w = (w + 1) * f;
+            sw = sw * (f * f * f);
+            f = 1.0 - w;
+            u = u * f; v = v * f;
+            sw = sw * (f * f);
+            f = 1.0 - v / f;
+            u = u * f;
+            sw = sw * f;
+          }
+          t.gu.push_back(u); t.gv.push_back(v); t.gw.push_back(w); t.gs.push_back(sw);
         }
   }
   t.ngp = (int)t.gu.size();
@@ -448,10 +476,13 @@ static int dispatch_element(ehip_system *s, int physics, int code, int cdim, con
     if (code == 808 && cdim == 3) { launch_element<8, 8, 3, 3, 0, 8, 256>(s, a); return EHIP_OK; }
     if (code == 504 && cdim == 3) { launch_element<4, 4, 3, 3, 0, 16, 256>(s, a); return EHIP_OK; }
     if (code == 404 && cdim == 2) { launch_element<4, 4, 2, 2, 0, 16, 256>(s, a); return EHIP_OK; }
+    if (code == 303 && cdim == 2) { launch_element<3, 3, 2, 2, 0, 16, 256>(s, a); return EHIP_OK; }
+    if (code == 510 && cdim == 3) { launch_element<10, 27, 3, 3, 0, 2, 256>(s, a); return EHIP_OK; }
     if (code == 827 && cdim == 3) { launch_element<27, 27, 3, 3, 0, 1, 256>(s, a); return EHIP_OK; }
   } else {
     if (code == 808) { launch_element<8, 8, 3, 3, 1, 8, 256>(s, a); return EHIP_OK; }
     if (code == 827) { launch_element<27, 27, 3, 3, 1, 1, 256>(s, a); return EHIP_OK; }
+    if (code == 510) { launch_element<10, 27, 3, 3, 1, 2, 256>(s, a); return EHIP_OK; }
   }
   set_error("ehip_assemble: element %d / physics %d / cdim %d not supported", code, physics, cdim);
   return EHIP_EUNSUP;

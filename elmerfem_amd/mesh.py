@@ -75,7 +75,8 @@ class Mesh:
     def boundary_nodes(self, ids=None) -> np.ndarray:
         """Sorted unique 1-based nodes of the boundary elements with the given ids."""
         sel = np.ones(len(self.bnd_ids), bool) if ids is None else np.isin(self.bnd_ids, list(ids))
-        return np.unique(self.bnd_elements[sel].ravel()).astype(np.int32)
+        nodes = np.unique(self.bnd_elements[sel].ravel())
+        return nodes[nodes > 0].astype(np.int32)  # rows of smaller element types are 0-padded
 
 
 def _grid3(nx: int) -> np.ndarray:
@@ -227,3 +228,43 @@ def tet4_cube(n: int, jitter: float = 0.2, seed: int = DEFAULT_SEED) -> Mesh:
     tid = np.concatenate([qids, qids])
     return Mesh(3, c, 504, el, 303, tri.astype(np.int32), tid.astype(np.int32),
                 {"n": n, "kind": "tet4", "jitter": jitter, "seed": seed})
+
+
+def tet10_cube(n: int, jitter: float = 0.2, seed: int = DEFAULT_SEED) -> Mesh:
+    """tet4_cube with a node at the middle of every edge: Tet10 elements in
+    elements.def 510 order (vertices, then mid-edges (1,2) (2,3) (3,1) (1,4)
+    (2,4) (3,4)); boundary Tri6 (306: vertices, then (1,2) (2,3) (3,1)) on the
+    faces that belong to one tetrahedron, with the cube-face ids of tet4_cube."""
+    t4 = tet4_cube(n, jitter=jitter, seed=seed)
+    el4 = t4.elements.astype(np.int64)
+    pairs = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)]
+    e = np.concatenate([np.sort(el4[:, [a, b]], axis=1) for a, b in pairs])
+    uniq, inv = np.unique(e, axis=0, return_inverse=True)
+    mid = t4.nnodes + 1 + inv.reshape(len(pairs), -1).T          # (nelem, 6) mid-edge node ids
+    coords = np.vstack([t4.coords, 0.5 * (t4.coords[uniq[:, 0] - 1] + t4.coords[uniq[:, 1] - 1])])
+    el = np.hstack([el4, mid]).astype(np.int32)
+    edge_id = {(int(a), int(b)): t4.nnodes + 1 + k for k, (a, b) in enumerate(uniq)}
+    faces = {}
+    for t in el4:
+        for f in ((t[0], t[1], t[2]), (t[0], t[1], t[3]), (t[0], t[2], t[3]), (t[1], t[2], t[3])):
+            key = tuple(sorted(int(v) for v in f))
+            faces[key] = faces.get(key, 0) + 1
+    bnd, ids = [], []
+    for (a, b, c), cnt in faces.items():
+        if cnt != 1:
+            continue
+        tri = [a, b, c]
+        m = [edge_id[tuple(sorted((tri[i], tri[(i + 1) % 3])))] for i in range(3)]
+        bnd.append(tri + m)
+        x = t4.coords[np.array(tri) - 1]
+        for d, (lo, hi) in enumerate(((1, 2), (3, 4), (5, 6))):
+            if np.all(x[:, d] == 0.0):
+                ids.append(lo)
+                break
+            if np.all(x[:, d] == 1.0):
+                ids.append(hi)
+                break
+        else:
+            raise ValueError("boundary face off the cube faces")
+    return Mesh(3, coords, 510, el, 306, np.array(bnd, np.int32), np.array(ids, np.int32),
+                {"n": n, "kind": "tet10", "jitter": jitter, "seed": seed})

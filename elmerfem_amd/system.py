@@ -64,6 +64,11 @@ class System:
         L = _lib.init()
         el = np.ascontiguousarray(mesh.elements, np.int32)
         be = np.ascontiguousarray(mesh.bnd_elements, np.int32) if with_boundary and len(mesh.bnd_elements) else None
+        if be is not None:
+            # boundary elements of a smaller type (0-padded rows, e.g. 101 points
+            # read from a mesh file) add only pairs the bulk elements already have
+            be = np.ascontiguousarray(be[(be > 0).all(axis=1)])
+            be = be if len(be) else None
         h = C.c_void_p()
         _lib.check(L.ehip_system_from_mesh(mesh.nnodes, el.shape[0], el.shape[1], _ptr(el),
                                            0 if be is None else be.shape[0], 0 if be is None else be.shape[1],

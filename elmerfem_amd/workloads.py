@@ -96,6 +96,27 @@ def heat_tet4(n: int, seed: int = DEFAULT_SEED, index: int = 1, jitter: float = 
     }
 
 
+def heat_tet10(n: int, seed: int = DEFAULT_SEED, index: int = 1, jitter: float = 0.2) -> dict:
+    m = M.tet10_cube(n, jitter=jitter, seed=seed + index)
+    k = M.lognormal(seed + index, m.nelem, 0.0, 0.5, stream=3)
+    return {
+        "name": "heat_tet10_%d" % n, "mesh": m, "physics": 0, "dofs": 1,
+        "coef": k, "conv": None, "velo": None, "load": np.ones(m.nelem),
+        "bc_nodes": m.boundary_nodes(), "kw": _kw("cg"),
+        "info": {"n": n, "seed": seed, "index": index},
+    }
+
+
+def elasticity_tet10(n: int) -> dict:
+    m = M.tet10_cube(n, jitter=0.0)
+    return {
+        "name": "elasticity_tet10_%d" % n, "mesh": m, "physics": 1, "dofs": 3,
+        "coef": np.ones(m.nelem), "poisson": 0.3,
+        "bc_nodes": m.boundary_nodes([1]), "kw": _kw("cg"),
+        "info": {"n": n, "E": 1.0, "nu": 0.3},
+    }
+
+
 def build_system(w: dict, optimize_bandwidth: bool = False):
     """Creates the device system, assembles and applies the Dirichlet rows.
     Returns (system, assembly ms)."""

@@ -148,13 +148,44 @@ int ehip_system_get_values(ehip_system *s, double *values, double *rhs);
 /* Device memory the system holds, in bytes. */
 int64_t ehip_system_bytes(const ehip_system *s);
 
+/* ---- mesh and result files ----
+ * An ElmerGrid mesh directory as LoadMesh reads it (MeshUtils.src:313-1009
+ * through EIO): mesh.header, mesh.nodes ("tag partition x y z"),
+ * mesh.elements ("tag body type nodes..."), mesh.boundary ("tag bc parent1
+ * parent2 type nodes...").  Nodes are renumbered 1..nnodes in file order and
+ * element node references mapped through the node tags (LocalPerm,
+ * :493-499).  Element nodes per type = type mod 100. */
+typedef struct ehip_mesh ehip_mesh;
+int ehip_mesh_read(const char *dir, ehip_mesh **out);
+/* max_npe_*: nodes of the largest bulk / boundary element type (row widths below) */
+int ehip_mesh_sizes(const ehip_mesh *m, int *nnodes, int *nbulk, int *nbnd, int *max_npe_bulk, int *max_npe_bnd);
+/* coords: nnodes x 3; bulk_nodes: nbulk x max_npe_bulk and bnd_nodes: nbnd x
+ * max_npe_bnd, 1-based node numbers, rows of smaller types 0-padded;
+ * bnd_parents: nbnd x 2 (0: none).  Any pointer may be NULL. */
+int ehip_mesh_get(const ehip_mesh *m, double *coords, int *bulk_types, int *bulk_bodies, int *bulk_nodes,
+                  int *bnd_types, int *bnd_tags, int *bnd_parents, int *bnd_nodes);
+int ehip_mesh_free(ehip_mesh *m);
+/* Writes the four files of a serial ElmerGrid mesh (partition -1) with one
+ * bulk and one boundary element type; bulk_bodies, bnd_tags and bnd_parents
+ * may be NULL (body 1, tag 1, no parents). */
+int ehip_mesh_write(const char *dir, int nnodes, const double *coords, int nbulk, int bulk_type,
+                    const int *bulk_nodes, const int *bulk_bodies, int nbnd, int bnd_type, const int *bnd_nodes,
+                    const int *bnd_tags, const int *bnd_parents);
+/* SaveResult (ModelDescription.src:2442-2865), ASCII format 3, one scalar
+ * nodal variable: the header when append = 0, then the "Time:" line, the
+ * variable name, its Perm (perm[nnodes], 1-based, 0 = no value; NULL writes
+ * "Perm: NULL" and values in node order) and the values. */
+int ehip_result_write(const char *path, const char *var, const char *equation, int nnodes, const int *perm,
+                      int dofs, const double *values, int append, int step, double time);
+
 /* ---- assembly ---- */
 #define EHIP_PHYS_MODELPDE 0   /* ModelPDE: diffusion + convection + source (ModelPDE.src:75-166) */
 #define EHIP_PHYS_ELASTIC 1    /* StressSolve isotropic elasticity (Stress.src:59-521) */
 /* Bulk element loop + DefaultUpdateEquations glue, in element order, as a
  * deterministic gather over the element-to-slot map (no float atomics).
  * coords: nnodes x 3 (row-major).  Per-element arrays (length nelem, or
- * 3*nelem for velo) may be NULL (= 0).  elem_code: 404, 504, 808, 827.
+ * 3*nelem for velo) may be NULL (= 0).  elem_code: 303 (cdim 2), 404
+ * (cdim 2), 504, 510, 808, 827 (cdim 3); elasticity takes 808, 827, 510.
  * Values and RHS are zeroed first (DefaultInitialize). */
 int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdim, const double *coords,
                   const double *elem_coef, const double *elem_conv, const double *elem_velo,

@@ -119,6 +119,7 @@ static const double WT4[4] = {0.2180665126782654, 0.5635595064952189,
 int or_default_gauss_n(int code) {
   switch (code) {
     case OR_LINE2: return 2;
+    case OR_TRI3: return 3;
     case OR_QUAD4: return 4;
     case OR_QUAD9: return 9;
     case OR_TET4: return 4;
@@ -137,6 +138,13 @@ int or_gauss_points(int code, int np, double *u, double *v, double *w, double *s
     if (or_gauss1d(np, pts, wts)) return -1;
     for (int i = 0; i < np; i++) { u[i] = pts[i]; v[i] = 0; w[i] = 0; s[i] = wts[i]; }
     return np;
+  }
+  if (fam == 3 && np == 3) {
+    /* GaussPointsTriangle, 3 points (Integration.src:84-92, 776-781) */
+    static const double u3[3] = {0.16666666666666667, 0.66666666666666667, 0.16666666666666667};
+    static const double v3[3] = {0.16666666666666667, 0.16666666666666667, 0.66666666666666667};
+    for (int i = 0; i < 3; i++) { u[i] = u3[i]; v[i] = v3[i]; w[i] = 0.0; s[i] = 0.33333333333333333 / 2.0; }
+    return 3;
   }
   if (fam == 4) {
     int n = (int)(sqrt((float)np) + 0.5f);
@@ -199,7 +207,7 @@ typedef struct {
   int ready;
 } etype_t;
 
-static etype_t g_types[8];
+static etype_t g_types[9];
 
 static const double HEX27_U[27] = {-1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, 0, 0, 0};
 static const double HEX27_V[27] = {-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, 0, 0, 0};
@@ -263,6 +271,7 @@ static etype_t *get_type(int code) {
     case OR_TET10: slot = 4; break;
     case OR_HEX8: slot = 5; break;
     case OR_HEX27: slot = 6; break;
+    case OR_TRI3: slot = 7; break;
     default: return NULL;
   }
   etype_t *t = &g_types[slot];
@@ -279,6 +288,11 @@ static etype_t *get_type(int code) {
       t->dim = 1; t->n = 2; B = B202;
       t->U[0] = -1; t->U[1] = 1;
       break;
+    case OR_TRI3: {
+      static const int B303[3] = {1, 2, 5};
+      t->dim = 2; t->n = 3; B = B303;
+      t->U[1] = 1; t->V[2] = 1;
+    } break;
     case OR_QUAD4: {
       static const double u[4] = {-1, 1, 1, -1}, v[4] = {-1, -1, 1, 1};
       t->dim = 2; t->n = 4; B = B404;

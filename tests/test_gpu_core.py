@@ -381,12 +381,14 @@ def test_pattern_hand_traced_reordering(ehip, name, dofs):
 
 # ------------------------------------------------------------------ assembly + Dirichlet
 
-@pytest.mark.parametrize("wl", ["poisson", "lognormal", "convdiff", "tet4", "elastic8", "elastic27", "poisson27"])
+@pytest.mark.parametrize("wl", ["poisson", "lognormal", "convdiff", "tet4", "elastic8", "elastic27", "poisson27",
+                                "tet10", "elastic10"])
 def test_assembly_matches_oracle(ehip, wl):
     w = {"poisson": lambda: W.poisson_hex8(6), "lognormal": lambda: W.poisson_hex8(6, sigma=1.0),
          "convdiff": lambda: W.convdiff_quad4(12), "tet4": lambda: W.heat_tet4(4),
          "elastic8": lambda: W.elasticity_hex(3), "elastic27": lambda: W.elasticity_hex(2, quadratic=True),
-         "poisson27": lambda: W.poisson_hex27(2)}[wl]()
+         "poisson27": lambda: W.poisson_hex27(2), "tet10": lambda: W.heat_tet10(3),
+         "elastic10": lambda: W.elasticity_tet10(2)}[wl]()
     m = w["mesh"]
     pat = O.Pattern(m.elements, m.nnodes, w["dofs"])
     vals, rhs = O.assemble(w["physics"], m, pat, w["coef"], conv=w.get("conv"), velo=w.get("velo"),

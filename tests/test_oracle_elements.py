@@ -12,23 +12,29 @@ REF_NODES = {
     808: np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]],
                   float),
     827: M._H27.astype(float),
+    303: np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float),
+    # elements.def 510: vertices, then mid-edges (1,2) (2,3) (3,1) (1,4) (2,4) (3,4)
+    510: np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [.5, 0, 0], [.5, .5, 0], [0, .5, 0], [0, 0, .5],
+                   [.5, 0, .5], [0, .5, .5]], float),
 }
+VOL = {404: 4, 504: 1 / 6, 808: 8, 827: 8, 303: 1 / 2, 510: 1 / 6}   # reference element measure
+TWO_D = (303, 404)
 A3 = np.array([[0.7, 0.2, -0.1], [0.05, 0.9, 0.3], [-0.2, 0.1, 1.1]])  # affine map, det != 1, not symmetric
 B3 = np.array([0.3, -0.4, 2.0])
 
 
 def mapped(code):
     X = REF_NODES[code]
-    if code == 404:
+    if code in TWO_D:
         A = np.array([[0.7, 0.2, 0], [0.05, 0.9, 0], [0, 0, 1]])
         return X @ A.T + np.array([0.1, 0.2, 0.0]), A
     return X @ A3.T + B3, A3
 
 
-@pytest.mark.parametrize("code", [404, 504, 808, 827])
+@pytest.mark.parametrize("code", [303, 404, 504, 510, 808, 827])
 def test_element_info_invariants(code):
     X, A = mapped(code)
-    cdim = 2 if code == 404 else 3
+    cdim = 2 if code in TWO_D else 3
     u, v, w, s = O.gauss_points(code, O.lib().or_default_gauss_n(code))
     for g in range(len(u)):
         ok, basis, db, detj = O.element_info(code, cdim, X, u[g], v[g], w[g])
@@ -39,17 +45,14 @@ def test_element_info_invariants(code):
         J = X[:, :cdim].T @ db[:, :cdim]
         assert np.abs(J - np.eye(cdim)).max() < 1e-12
         d = abs(np.linalg.det(A[:cdim, :cdim]))
-        if code == 504:
-            assert abs(detj - d) < 1e-14      # reference tet has volume 1/6; detJ maps it
-        else:
-            assert abs(detj - d) < 1e-14
+        assert abs(detj - d) < 1e-14      # detJ of the affine map (reference measure in the weights)
 
 
-@pytest.mark.parametrize("code", [404, 504, 808, 827])
+@pytest.mark.parametrize("code", [303, 404, 504, 510, 808, 827])
 def test_basis_kronecker(code):
     X = REF_NODES[code]
     for a in range(len(X)):
-        ok, basis, db, detj = O.element_info(code, 2 if code == 404 else 3, X, X[a, 0], X[a, 1], X[a, 2])
+        ok, basis, db, detj = O.element_info(code, 2 if code in TWO_D else 3, X, X[a, 0], X[a, 1], X[a, 2])
         e = np.zeros(len(X))
         e[a] = 1
         assert np.abs(basis - e).max() < 1e-13
@@ -85,14 +88,39 @@ def test_tet4_closed_form():
     assert np.abs(F - V / 4).max() < 1e-15
 
 
-@pytest.mark.parametrize("code", [404, 504, 808, 827])
+def test_tri3_closed_form():
+    X, A = mapped(303)
+    K, F = O.local_modelpde(303, 2, X, diff=1.5, load=1.0)
+    T = np.hstack([np.ones((3, 1)), X[:, :2]])
+    G = np.linalg.inv(T)[1:, :].T          # gradients of the barycentric coordinates
+    area = abs(np.linalg.det(T)) / 2
+    assert np.abs(K - 1.5 * area * G @ G.T).max() < 1e-14
+    assert np.abs(F - area / 3).max() < 1e-15
+
+
+def test_tet10_quadratic_field_energy():
+    """Tet10 represents x^2 exactly and the 27-point collapsed-brick rule
+    integrates |grad x^2|^2 = 4 x^2 exactly on an affine element:
+    int_T x^2 = vol/20 (sum of x_i^2 + (sum of x_i)^2) over the 4 vertices."""
+    X, A = mapped(510)
+    K, F = O.local_modelpde(510, 3, X, diff=1.0, load=1.0)
+    u = X[:, 0] ** 2
+    xv = X[:4, 0]
+    vol = abs(np.linalg.det(A)) / 6
+    exact = 4 * vol / 20 * ((xv ** 2).sum() + xv.sum() ** 2)
+    assert abs(u @ K @ u - exact) < 1e-13
+    # the source vector: int phi_p, vertices -vol/20 and mid-edges vol/5
+    assert np.abs(F[:4] + vol / 20).max() < 1e-15 and np.abs(F[4:] - vol / 5).max() < 1e-15
+
+
+@pytest.mark.parametrize("code", [303, 404, 504, 510, 808, 827])
 def test_linear_field_energy(code):
     X, A = mapped(code)
-    cdim = 2 if code == 404 else 3
+    cdim = 2 if code in TWO_D else 3
     K, F = O.local_modelpde(code, cdim, X, diff=1.0, load=1.0)
     g = np.array([0.3, -1.2, 0.5])[:cdim]
     uvec = X[:, :cdim] @ g
-    vol = abs(np.linalg.det(A[:cdim, :cdim])) * {404: 4, 504: 1 / 6, 808: 8, 827: 8}[code]
+    vol = abs(np.linalg.det(A[:cdim, :cdim])) * VOL[code]
     assert abs(uvec @ K @ uvec - g @ g * vol) < 1e-12
     assert np.abs(K - K.T).max() < 1e-14
     assert np.abs(K.sum(axis=1)).max() < 1e-13
@@ -112,7 +140,7 @@ def test_convection_term():
     assert np.abs(C - C.T).max() > 1e-3                  # non-symmetric
 
 
-@pytest.mark.parametrize("code", [808, 827])
+@pytest.mark.parametrize("code", [808, 827, 510])
 def test_elasticity_invariants(code):
     X, A = mapped(code)
     K, F = O.local_elasticity(code, X, young=1.0, poisson=0.3)
@@ -139,7 +167,7 @@ def test_elasticity_invariants(code):
     u = np.zeros((n, 3))
     u[:, 0] = eps * X[:, 0]
     c11 = E * (1 - nu) / ((1 + nu) * (1 - 2 * nu))
-    vol = abs(np.linalg.det(A)) * 8
+    vol = abs(np.linalg.det(A)) * VOL[code]
     assert abs(u.ravel() @ K @ u.ravel() - c11 * eps ** 2 * vol) < 1e-14
 
 

@@ -33,7 +33,7 @@ def test_gauss1d_exact_degree(n):
 
 
 @pytest.mark.parametrize("code,np_,nexp", [(404, 4, 4), (409, 9, 9), (808, 8, 8), (827, 27, 27), (504, 4, 4),
-                                           (510, 27, 27)])
+                                           (510, 27, 27), (303, 3, 3)])
 def test_default_rules(code, np_, nexp):
     u, v, w, s = O.gauss_points(code, np_)
     assert len(u) == nexp
@@ -47,6 +47,10 @@ def test_default_rules(code, np_, nexp):
         # brick ordering: w outer, v, u inner (GaussPointsBrick)
         m = round(len(u) ** (1 / 3))
         assert np.all(u[:m] == u[m:2 * m]) and np.all(w[:m * m] == w[0])
+    elif fam == 3:
+        assert abs(s.sum() - 0.5) < 1e-15                       # area of the unit triangle
+        assert abs(np.sum(s * u ** 2) - 1 / 12) < 1e-15         # degree-2 exactness
+        assert abs(np.sum(s * u * v) - 1 / 24) < 1e-15
     else:
         assert abs(s.sum() - 1 / 6) < 1e-14                     # volume of the unit tetrahedron
         # degree-2 exactness of the 4-point rule: int x^2 over the tet = 1/60
