@@ -63,6 +63,13 @@ struct KStateT {
   int iter, info, done, maxit, snorm_exit, rec_conv, pad0, pad1;
   T tol, divtol, tol_u, bb_u, res_u;
   unsigned int ticket[4];
+  // folded FP32 CG steps: rho of the last vector step, carried from the SpMV
+  // step that sums it to the vector step that uses it; fresh = 1 when rho and
+  // beta are current (start, residual replacement, end of a block)
+  T rho_next;
+  int fresh;
+  // a folded SpMV step's stop, made `done` by the vector step that follows
+  int stop_pending;
 };
 using KState = KStateT<double>;
 using KState32 = KStateT<float>;

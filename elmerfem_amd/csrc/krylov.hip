@@ -36,6 +36,14 @@ namespace ehip {
 thread_local GraphRec *tl_graph_rec = nullptr;
 thread_local hipError_t tl_launch_err = hipSuccess;
 
+// Whether the mixed form's FP32 CG runs its folded two-launch steps
+// (EHIP_CG_FOLD=0 turns them off: three launches per iteration).  Read at
+// every block of iterations, so a process can compare the two.
+static bool cg_fold_enabled() {
+  const char *f = std::getenv("EHIP_CG_FOLD");
+  return !(f && f[0] == '0');
+}
+
 bool graphs_enabled() {
   static const bool on = [] {
     const char *g = std::getenv("EHIP_GRAPHS");
@@ -393,7 +401,7 @@ struct CgTile {
 // do not depend on alpha and go out before the reduction, so the two
 // latencies overlap.  Then x += alpha p ; r -= alpha q ; z = r/dg with the
 // partials of r.z (and r.r when NA = 2).
-template <typename T, int NA>
+template <typename T, int NA, bool Fold = false>
 __device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<T> *st, const T *pq_part, int pq_g,
                                                T *partials) {
   const int stride = gridDim.x * kVecBlock * kVI;
@@ -402,10 +410,17 @@ __device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<
   cur.load(v, n, base);
   T pq[1];
   block_reduce_all<T, 1>(pq_part, 0, pq_g, pq);
-  const T alpha = st->rho / pq[0];
+  // folded steps: rho was summed by the SpMV step before (rho_next); this
+  // launch makes it current and counts the iteration
+  const T alpha = (Fold ? st->rho_next : st->rho) / pq[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pq = pq[0];
     st->alpha = alpha;
+    if constexpr (Fold) {
+      st->rho = st->rho_next;
+      st->iter = st->iter + 1;
+      st->fresh = 0;
+    }
   }
   T acc[NA];
 #pragma unroll
@@ -421,11 +436,18 @@ __device__ __forceinline__ void cg_vector_step(int n, const Vecs<T> &v, KStateT<
 }
 
 // step 2 of the one-SpMV CG: partials r.z, r.r
-template <typename T>
+template <typename T, bool Fold = false>
 __global__ __launch_bounds__(kVecBlock) void k_cg_fast2(int n, Vecs<T> v, KStateT<T> *st, const T *pq_part,
                                                         int pq_g, T *partials) {
   if (solve_done(st)) return;
-  cg_vector_step<T, 2>(n, v, st, pq_part, pq_g, partials);
+  if constexpr (Fold) {
+    // the SpMV step before stopped the sequence (its state is already final)
+    if (__builtin_amdgcn_readfirstlane(st->stop_pending)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->done = 1;
+      return;
+    }
+  }
+  cg_vector_step<T, 2, Fold>(n, v, st, pq_part, pq_g, partials);
 }
 
 // The stopping tests on the recurrence residual after a CG vector step, in
@@ -470,6 +492,111 @@ struct FinCgFast2 {
     st->iter = it;
     if (hist && it - 1 < st->maxit) hist[it - 1] = (double)res;
     cg_apply(cg_check(res, tot[0], it, st), st);
+  }
+};
+
+// ---- folded FP32 CG steps (the inner sequence of the mixed form): two
+// launches per iteration instead of three.  The SpMV step opens by summing
+// the previous vector step's partials (r.z, r.r) itself -- every workgroup,
+// in the one-workgroup finalize's order, so every workgroup gets that
+// finalize's bits -- and runs its stopping tests.  Fields of the state each
+// launch reads and writes, so that no workgroup reads a field that another
+// workgroup of the same launch writes:
+//   SpMV step   reads  done, fresh, rho, iter, maxit, tol, divtol, ru, rhsnorm (beta when fresh);
+//               writes rho_next, rr, res, and on a stop rho_new, rho, beta, info, rec_conv,
+//               stop_pending.  A stop is decided by every workgroup alike from the same
+//               sums (neither the decision nor the work it skips reads rho or beta).
+//   vector step reads  done, stop_pending, rho_next; writes pq, alpha, rho = rho_next,
+//               iter + 1, fresh = 0 -- or, after a stop, only done = 1.
+// The SpMV step must not write `done` itself: every kernel tests it on entry
+// once per wavefront, and a wavefront that started after the write would
+// return while the other wavefronts of its workgroup, which started before
+// it, go on into the workgroup barriers of the reduction without it and sum
+// unwritten LDS slots -- a wrong beta for their rows.  (It happened only when
+// the workgroups of one launch were spread out in time, i.e. with other
+// solves running on the device.)  stop_pending is written by one launch and
+// read, alike by every wavefront, by the next.
+// A block of folded iterations ends with FinCgFold, the finalize of the last
+// vector step, which leaves rho and beta current (fresh = 1) as the start
+// and a residual replacement do.  The state then matches the unfolded
+// sequence's after the same number of iterations, bit for bit.
+constexpr int kFoldPer = 4;
+static_assert(kFusedCap <= kFoldPer * 256, "vector-step partials must fit the SpMV prologue's loads");
+
+template <typename T>
+struct EpiCgFold {
+  static constexpr int NV = 1;
+  EHIP_EPI_MEMBERS(T)
+  const T *rz_part;  // the vector step's partials: r.z at [0, rz_g), r.r at [rz_g, 2 rz_g)
+  int rz_g;
+  T beta;
+  bool skip;
+  __device__ void init() {
+    skip = false;
+    if (st->fresh) {
+      beta = st->beta;
+      if (blockIdx.x == 0 && threadIdx.x == 0) st->rho_next = st->rho;
+      return;
+    }
+    T tot[2];
+    block_reduce_few<T, 2, kFoldPer>(rz_part, rz_g, rz_g, tot);
+    const T rho_old = st->rho;
+    const T res = sqrt(tot[1]) / st->rhsnorm;
+    const CgCheck c = cg_check(res, tot[0], st->iter, st);
+    beta = tot[0] / rho_old;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (c.done) {
+      skip = true;
+      if (lead) {
+        st->rho_new = rho_old;
+        st->rho = tot[0];
+        st->beta = beta;
+        st->rr = tot[1];
+        st->res = res;
+        if (c.rec_conv) st->rec_conv = c.rec_conv;
+        if (c.info) st->info = c.info;
+        st->stop_pending = 1;
+      }
+      return;
+    }
+    if (lead) {
+      st->rho_next = tot[0];
+      st->rr = tot[1];
+      st->res = res;
+    }
+  }
+  struct Pre { T q, z, p; };
+  __device__ Pre load(int i) const { return Pre{v.q[i], v.z[i], v.p[i]}; }
+  __device__ void row(int i, T s, const Pre &pr, T (&acc)[1]) {
+    const T qi = s + beta * pr.q;
+    const T pi = pr.z + beta * pr.p;
+    v.q[i] = qi;
+    v.p[i] = pi;
+    acc[0] += pi * qi;
+  }
+  __device__ void finish(T (&)[1]) {}
+};
+
+// end of a block of folded iterations: FinCgFast2's scalar update and tests
+// for the last vector step (whose launch already counted the iteration),
+// leaving rho and beta current
+template <typename T>
+struct FinCgFold {
+  static constexpr int NV = 2;
+  KStateT<T> *st;
+  T *partials;
+  int stride;
+  __device__ void finish(T (&tot)[2]) {
+    if (threadIdx.x != 0) return;
+    const T rho_old = st->rho;
+    st->rho_new = rho_old;
+    st->rho = tot[0];
+    st->beta = tot[0] / rho_old;
+    st->rr = tot[1];
+    const T res = sqrt(tot[1]) / st->rhsnorm;
+    st->res = res;
+    cg_apply(cg_check(res, tot[0], st->iter, st), st);
+    st->fresh = 1;
   }
 };
 
@@ -906,6 +1033,9 @@ __global__ void k_init_state(KStateT<T> *st, T tol, int maxit, T divtol, int ite
   st->res_true = (T)-1;
   st->pad0 = 0;
   st->pad1 = 0;
+  st->rho_next = (T)0;
+  st->fresh = 1;
+  st->stop_pending = 0;
 }
 
 // ---- mixed precision
@@ -1052,6 +1182,8 @@ struct EpiRU {
       st32->ru = (float)(res * delta);
       st32->rec_conv = 0;
       st32->done = 0;
+      st32->stop_pending = 0;
+      st32->fresh = 1;
     }
   }
 };
@@ -1524,6 +1656,19 @@ static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, boo
   const int n = s->n, gf = fused_grid(n), pg = kEpiStride;
   KState32 *st32 = s->st32;
   float *vpart = s->partials32 + kVecPartOff;
+  if (cg_fold_enabled()) {
+    // two launches per iteration; the block closes with the last step's finalize
+    for (int k = 0; k < count; ++k) {
+      EpiCgFold<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], vpart, gf, 0.f, false};
+      if (prof) prof_begin(s);
+      launch_epi<float>(s, s->plan32, s->vals32, v.z, e1, st32);
+      if (prof) prof_end(s);
+      launch_k((k_cg_fast2<float, true>), dim3(gf), dim3(kVecBlock), s->stream, n, v, st32,
+               (const float *)s->partials32, epi_grid<float>(s, s->plan32), vpart);
+    }
+    fin<float>(s, FinCgFold<float>{st32, vpart, gf}, gf, st32);
+    return;
+  }
   for (int k = 0; k < count; ++k) {
     EpiCgFast1<float> e1{v, st32, s->partials32, pg, &st32->ticket[0], 0.f};
     if (prof) prof_begin(s);
@@ -1541,7 +1686,7 @@ static uint64_t cg32_key(const ehip_system *s) {
                             (uint64_t)(uintptr_t)s->sell32, (uint64_t)(uintptr_t)s->partials32,
                             (uint64_t)(uintptr_t)s->st32, (uint64_t)(uintptr_t)s->plan32.chunk_row,
                             (uint64_t)s->fmt32, (uint64_t)s->plan32.block, (uint64_t)s->plan32.nb,
-                            (uint64_t)s->plan32.grid, (uint64_t)s->n};
+                            (uint64_t)s->plan32.grid, (uint64_t)s->n, (uint64_t)cg_fold_enabled()};
   uint64_t h = 1469598103934665603ull;
   for (uint64_t x : parts) {
     h ^= x;

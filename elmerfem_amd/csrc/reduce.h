@@ -88,6 +88,32 @@ __device__ __forceinline__ void block_reduce_all(const T *partials, int stride, 
   }
 }
 
+// The same sum as block_reduce_all for g <= PER * blockDim.x, with PER
+// loads per thread instead of kFinPer (the dropped terms are zeros, so the
+// totals are bit-identical): for kernels whose every workgroup sums a short
+// list of partials before its own work.
+template <typename T, int NV, int PER>
+__device__ __forceinline__ void block_reduce_few(const T *partials, int stride, int g, T (&tot)[NV]) {
+  __shared__ T sh[16 * NV];
+  const int nt = blockDim.x;
+  T v[NV][PER];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int i = threadIdx.x + u * nt;  // load clamped (g >= 1), keep only i < g
+      const T x = partials[k * stride + min(i, g - 1)];
+      v[k][u] = i < g ? x : (T)0;
+    }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    T a = (T)0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) a += v[k][u];
+    tot[k] = block_sum(a, sh + 16 * k);
+  }
+}
+
 // XCD-aware chunk walk: gridDim.x is a multiple of 8 and the hardware
 // dispatches workgroup b to XCD b % 8, so XCD x owns the contiguous chunk
 // range [x*C/8, (x+1)*C/8) and its L2 keeps that range's x-vector window.

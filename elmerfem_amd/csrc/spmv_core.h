@@ -14,6 +14,8 @@
 //            row operands it needs are loaded before phase 1 so their latency
 //            hides behind the streaming.
 #pragma once
+#include <type_traits>
+#include <utility>
 #include "reduce.h"
 
 namespace ehip {
@@ -99,6 +101,18 @@ __device__ __forceinline__ void stream_chunk(const V *__restrict__ vals, const C
 //   __device__ void row(int r, A s, const Pre &, A (&acc)[NV]);
 //   __device__ void finish(A (&tot)[NV]);    run by the finalize kernel
 //   A *partials; int stride; unsigned *ticket;
+// and may provide `bool skip`, set by init() the same way in every
+// workgroup: the workgroup then returns before any row (and publishes no
+// partials).
+template <class E, class = void>
+struct EpiHasSkip : std::false_type {};
+template <class E>
+struct EpiHasSkip<E, std::void_t<decltype(std::declval<E &>().skip)>> : std::true_type {};
+template <class E>
+__device__ __forceinline__ bool epi_skipped(const E &e) {
+  if constexpr (EpiHasSkip<E>::value) return e.skip;
+  else return false;
+}
 
 // One chunk in flight: its bounds, the streamed values / column indices, and
 // this lane's first row's bounds and epilogue operands.  All of it is loaded
@@ -136,6 +150,7 @@ __device__ __forceinline__ void spmv_stream_body(const int *__restrict__ rows,
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
+  if (epi_skipped(epi)) return;
   const int tid = (int)threadIdx.x;
   const XcdWalk w(nchunks);
 
@@ -261,6 +276,7 @@ __device__ __forceinline__ void spmv_sell_body(const int *__restrict__ rows, int
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
+  if (epi_skipped(epi)) return;
   const int lane = (int)threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const XcdWalk w((nslice + WPB - 1) / WPB);
@@ -321,6 +337,7 @@ __device__ __forceinline__ void spmv_sell2_body(const int *__restrict__ rows, in
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
+  if (epi_skipped(epi)) return;
   const int lane = (int)threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const XcdWalk w((nslice + WPB - 1) / WPB);
@@ -385,6 +402,7 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
 #pragma unroll
   for (int k = 0; k < NA; ++k) acc[k] = (A)0;
   epi.init();
+  if (epi_skipped(epi)) return;
   const int lane = (int)threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const XcdWalk w((nslice + WPB - 1) / WPB);

@@ -1,10 +1,12 @@
 """Concurrent single-system solves: several host threads, each solving its
 own system on its own stream, the way the server's concurrent lane runs
 them.  Every solve must converge and repeat its own result exactly, however
-the solves overlap on the device.  (A CG variant that took its stopping test
-inside the SpMV step, in every workgroup, once corrupted a few per cent of
-such solves -- iteration-limit stops or NaN -- while sequential solves were
-unaffected; this is its regression test.)"""
+the solves overlap on the device.  (A CG variant that wrote its stop flag
+from inside the SpMV launch that also tests the flag on entry -- once per
+wavefront -- corrupted a few per cent of such solves, iteration-limit stops
+or NaN, while sequential solves were unaffected: a late wavefront returned
+while its siblings summed its unwritten LDS slots.  This is its regression
+test.)"""
 import queue
 import threading
 

@@ -609,3 +609,26 @@ def test_mixed_graph_replay_equals_direct_launches(ehip, case):
     for x, it in out[1:]:
         assert it == out[0][1] and np.array_equal(x, out[0][0])
     s.free()
+
+
+@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL, _lib.SPMV_CSR])
+@pytest.mark.parametrize("case", ["cg_poisson32", "cg_lognormal8", "cg_tet4_4", "cg_elastic3"])
+def test_mixed_folded_steps_equal_unfolded(ehip, case, fmt, monkeypatch):
+    """The mixed CG's folded FP32 steps (two launches per iteration: the next
+    SpMV step sums the vector step's partials in every workgroup, in the
+    one-workgroup finalize's order) give the bits of the three-launch steps:
+    the same iterations, replacements and solution."""
+    w, pat, vals, rhs, rows = assembled_case(CASES[case])
+    s = System.from_csr(pat.rows, pat.cols, pat.diag)
+    s.set_values(vals, rhs)
+    set_format(s, fmt)
+    out = []
+    for fold in ("1", "0", "1"):
+        monkeypatch.setenv("EHIP_CG_FOLD", fold)
+        s.set_values(vals, rhs)
+        x, rep = s.solve(w["kw"], precision=_lib.MIXED_IR, mode=_lib.FAST, true_residual_stop=True)
+        assert rep["info"] == 1 and rep["res_true"] <= RES_BAR
+        out.append((x, rep["iters"], rep["outer_iters"]))
+    for x, it, up in out[1:]:
+        assert it == out[0][1] and up == out[0][2] and np.array_equal(x, out[0][0])
+    s.free()
