@@ -131,10 +131,11 @@ def form_batch(args):
     """Batching form: independent steady 2-D convection-diffusion problems on
     structured Quad4 grids, side N ~ U{32..512} elements, velocity of random
     direction with Peclet ~ U[1, 100], BiCGStab + Jacobi to 1e-10, all in one
-    graph-replayed multi-system launch sequence -- FP64 (HUTI order), then
-    mixed precision -- beside the same systems solved one at a time in FP64."""
+    graph-replayed multi-system launch sequence -- FP64 in HUTI's order, FP64
+    with the recurrence residual and true-residual restarts, then mixed
+    precision -- beside the same systems solved one at a time in FP64."""
     from elmerfem_amd import _lib, workloads as W
-    from elmerfem_amd.batch import solve_batch
+    from elmerfem_amd.batch import schedule, solve_batch
     from elmerfem_amd.mesh import uniforms
     _lib.init(0)
     k = args.systems
@@ -151,11 +152,18 @@ def form_batch(args):
     nnz = int(sum(s.nnz for s in systems))
     kw = W._kw("bicgstab", "diagonal", 1e-10, 20000)
     out = {}
-    # (the multi-system launches run FP64 in HUTI's loop order whatever the
-    # mode, so there is no separate FP64 fast batched line)
+    sched = {}
+    # FP64 in HUTI's loop order (its S-norm exit included), FP64 with the
+    # recurrence residual and true-residual restarts, and mixed precision.
+    # The warm-up solve of each form runs the schedule search (chunk size
+    # per size class, then workgroup count) unless --no-tune; the timed
+    # solve runs under the table it left.
     for name, prec, mode in (("fp64_faithful_batched", _lib.FP64, _lib.FAITHFUL),
+                             ("fp64_fast_batched", _lib.FP64, _lib.FAST),
                              ("mixed_batched", _lib.MIXED_IR, _lib.FAST)):
-        solve_batch(systems, kw, precision=prec, mode=mode, true_residual_stop=True)  # warm-up: first-use work
+        t = time.perf_counter()
+        solve_batch(systems, kw, precision=prec, mode=mode, true_residual_stop=True, tune=0 if args.no_tune else 1)
+        warm = time.perf_counter() - t
         t = time.perf_counter()
         xs, reps, ms = solve_batch(systems, kw, precision=prec, mode=mode, true_residual_stop=True)
         wall = time.perf_counter() - t
@@ -163,7 +171,12 @@ def form_batch(args):
         conv = sum(1 for r in reps if r["info"] == 1)
         out[name] = {"wall_s": wall, "device_ms": ms, "dof_iters_per_s": float((dofs * it).sum() / wall),
                      "iters_mean": float(it.mean()), "iters_max": int(it.max()), "converged": conv,
-                     "res_true_max": float(max(r["res_true"] for r in reps))}
+                     "res_true_max": float(max(r["res_true"] for r in reps)),
+                     "warmup_with_search_s": warm}
+        if mode == _lib.FAST and prec == _lib.FP64:
+            out[name]["restarts_mean"] = float(np.mean([r["outer_iters"] for r in reps]))
+        classes = sorted({int(s.nnz).bit_length() for s in systems})
+        sched[name] = {str(c): schedule(kw, 1 << (c - 1), precision=prec, mode=mode) for c in classes}
         log("%-14s %.2f s, %d/%d converged, %.3e DOF-iter/s" % (name, wall, conv, k, out[name]["dof_iters_per_s"]))
     for name, mode in (("fp64_faithful_one_at_a_time", _lib.FAITHFUL), ("fp64_fast_one_at_a_time", _lib.FAST)):
         t = time.perf_counter()
@@ -185,11 +198,14 @@ def form_batch(args):
                       "nnz_total": nnz, "side_min": int(min(sizes)), "side_max": int(max(sizes)),
                       "seed": args.seed, "method": "bicgstab", "precond": "diagonal", "tol": 1e-10},
            "forms": out,
+           "schedule": {"note": "chunk nonzeros and workgroups per form, keyed by the bit length of the "
+                                "systems' nonzero counts", "chosen": sched},
            "note": "HUTI-order FP64 BiCGStab stops with HUTI_BICGSTAB_SNORM (36) on some convection-dominated "
                    "systems, as the reference HUTI does on the same matrices (checked against the compiled "
                    "reference); the fast forms confirm convergence on the true residual and restart",
            "speedup_vs_fp64_fast_one_at_a_time": {n: out["fp64_fast_one_at_a_time"]["wall_s"] / out[n]["wall_s"]
-                                                  for n in ("fp64_faithful_batched", "mixed_batched")}}
+                                                  for n in ("fp64_faithful_batched", "fp64_fast_batched",
+                                                            "mixed_batched")}}
     print(json.dumps(rec), flush=True)
     for s in systems:
         s.free()
@@ -421,6 +437,7 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0,
                     help="mixed CG: residual drop between FP64 replacements (0: the library default)")
     ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
+    ap.add_argument("--no-tune", action="store_true", help="batch form: no schedule search (default table)")
     ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request systems")
     ap.add_argument("--requests", type=int, default=1000, help="serve form: requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve form: arrivals per second (0: load x capacity)")

@@ -110,6 +110,9 @@ SIGNATURES = {
     "ehip_server_free": (C.c_int, [_vp]),
     "ehip_server_set_lane": (C.c_int, [_vp, _i64, C.c_int]),
     "ehip_batch_set_lane": (C.c_int, [_vp, _i64, C.c_int]),
+    "ehip_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ehip_batch_get_schedule": (C.c_int, [C.POINTER(Params), _i64, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "ehip_batch_set_schedule": (C.c_int, [C.POINTER(Params), _i64, C.c_int, C.c_int]),
 }
 
 _lib = None

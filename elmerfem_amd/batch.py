@@ -16,10 +16,17 @@ from . import _lib
 
 
 def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=0.0, inner_maxit=0,
-                check_every=0, lane_dofs=None, lane_workers=None, true_residual_stop=False):
+                check_every=0, lane_dofs=None, lane_workers=None, true_residual_stop=False, tune=0,
+                tune_replays=0):
     """Returns (solutions, reports, ms).  With lane_workers > 0, systems of
     at least lane_dofs unknowns (default 40000) are solved whole on that many
-    concurrent threads instead of in the multi-system launches."""
+    concurrent threads instead of in the multi-system launches.
+
+    FP64 runs HUTI's loop order with mode FAITHFUL and the recurrence
+    residual with true-residual restarts with mode FAST.  tune = 1 runs the
+    schedule search for the size classes the process-wide schedule table
+    does not hold yet (tune = 2: for all) before solving; the chosen chunk
+    sizes and workgroup count stay in the table (see `schedule`)."""
     L = _lib.init()
     arr = (C.c_void_p * len(systems))(*[s._h for s in systems])
     h = C.c_void_p()
@@ -31,6 +38,8 @@ def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=
                        "ehip_batch_set_lane")
         p = kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, inner_tol, inner_maxit, check_every,
                                                                 true_residual_stop=true_residual_stop)
+        if tune:
+            _lib.check(L.ehip_batch_set_tuning(h, int(tune), int(tune_replays)), "ehip_batch_set_tuning")
         reps = (_lib.Report * len(systems))()
         ms = C.c_double()
         _lib.check(L.ehip_batch_solve(h, C.byref(p), reps, C.byref(ms)), "ehip_batch_solve")
@@ -42,6 +51,29 @@ def solve_batch(systems, kw, precision=_lib.FP64, mode=_lib.FAITHFUL, inner_tol=
         return xs, [r.as_dict() for r in reps], ms.value
     finally:
         L.ehip_batch_free(h)
+
+
+def _params(kw, precision, mode):
+    return kw if isinstance(kw, _lib.Params) else kw.to_params(precision, mode, 0.0, 0, 0)
+
+
+def schedule(kw, nnz, precision=_lib.FP64, mode=_lib.FAITHFUL):
+    """(chunk nonzeros, workgroups) the multi-system launches use for a
+    system of `nnz` nonzeros in this solver form."""
+    L = _lib.init()
+    p = _params(kw, precision, mode)
+    nb, grid = C.c_int(), C.c_int()
+    _lib.check(L.ehip_batch_get_schedule(C.byref(p), int(nnz), C.byref(nb), C.byref(grid)), "ehip_batch_get_schedule")
+    return nb.value, grid.value
+
+
+def set_schedule(kw, nnz, chunk_nnz=-1, grid=-1, precision=_lib.FP64, mode=_lib.FAITHFUL):
+    """Sets the table entry for the size class of `nnz` (chunk_nnz 1024,
+    2048 or 4096; 0 = default; -1 = unchanged) and the form's workgroup
+    count (a multiple of 8; 0 = default; -1 = unchanged)."""
+    L = _lib.init()
+    p = _params(kw, precision, mode)
+    _lib.check(L.ehip_batch_set_schedule(C.byref(p), int(nnz), int(chunk_nnz), int(grid)), "ehip_batch_set_schedule")
 
 
 class Server:

@@ -13,11 +13,19 @@
 // sums with plain stores, and a per-slot finalize launch (bk_fin) adds the
 // slot's chunk partials in chunk order and runs the scalar update.
 //
-// FP64 mode runs HUTI's CG / BiCGStab loop order per slot (true residual
-// every iteration, hutiter/src/huti_cg.src:145-227, huti_bicgstab.src:
-// 150-262).  Mixed mode runs FP32 inner Krylov (FP32 values, vectors and
-// accumulation; recurrence residual) inside per-slot FP64 iterative
-// refinement (true residual, correction and Jacobi diagonal in FP64).
+// FP64 faithful mode runs HUTI's CG / BiCGStab loop order per slot (true
+// residual every iteration, hutiter/src/huti_cg.src:145-227,
+// huti_bicgstab.src:150-262).  FP64 fast mode runs the recurrence residual
+// and confirms on the true residual, restarting the Krylov sequence from it
+// when it misses (also on HUTI's S-norm, omega and rho exits).  Mixed mode
+// runs FP32 inner Krylov (FP32 values, vectors and accumulation; recurrence
+// residual) inside per-slot FP64 iterative refinement (true residual,
+// correction and Jacobi diagonal in FP64).
+//
+// Chunk size (nonzeros per work item) is per system, from a process-wide
+// table keyed by solver form and size class and filled by a timed schedule
+// search; the workgroup count is per form.  Neither depends on the other
+// systems of the batch, so a system's bits are the same alone and in a mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,6 +77,7 @@ struct SlotPtrs {
   float *w32;         // NV32 vectors of n floats
   const double *du;   // the scaling's D when the unscaled residual is tested, else null
   int n, ck0, nck, method;
+  int nb;             // the system's chunk size (nonzeros); a longer chunk is one long row
 };
 
 struct BSys {
@@ -77,6 +86,7 @@ struct BSys {
   int iter, info, done, phase, maxit, outer, inner_iter, inner_total, inner_maxit, snorm_exit;
   int inner_bad;   // the inner solve ended on a non-finite residual: its correction is dropped
   int bad_count;   // consecutive such inner solves
+  int restarts;    // FP64 fast mode: restarts from the true residual so far
   // unscaled residual test (tol_u > 0): ||b - A x|| / ||b|| = ||r_s / D|| / ||b_s / D||
   double tol_u, bb_u, res_u;
 };
@@ -122,10 +132,54 @@ __device__ __forceinline__ void slot_done(BatchArgs &a, BSys &st, int info) {
   atomicSub(a.active, 1);
 }
 
+// One chunk's SpMV through LDS: ITEMS nonzeros per thread, streamed
+// branch-free (indices past the chunk are clamped to its last nonzero, so
+// every load is unconditional and in bounds; the extra products land in LDS
+// slots no row reads), then every row's products summed left to right.
+template <class Op, int BS, int ITEMS>
+__device__ __forceinline__ void chunk_spmv(const Op &op, typename Op::Ctx &cx, const SlotPtrs &sp, const Chunk &ck,
+                                           int j0, int len, typename Op::T *prod,
+                                           typename Op::T (&acc)[Op::NV > 0 ? Op::NV : 1]) {
+  using T = typename Op::T;
+  const T *xin = op.xin(cx);
+  const auto *vals = op.vals(sp);
+  using VT = typename std::remove_cv<typename std::remove_pointer<decltype(vals)>::type>::type;
+  VT vv[ITEMS];
+  int cc[ITEMS];
+  T xv[ITEMS];
+  const int rr = ck.r0 + (int)threadIdx.x;
+  const int rl = min(rr, ck.r1 - 1);
+  const int ra = sp.rows[rl], rb = sp.rows[rl + 1];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const int j = lane_nz<BS>(j0, len, k);
+    vv[k] = ntload(vals + j);
+    cc[k] = ntload(sp.cols + j);
+  }
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) xv[k] = xin[cc[k]];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) prod[threadIdx.x + k * BS] = (T)vv[k] * xv[k];
+  lds_barrier();
+  if (rr < ck.r1) {
+    T s = (T)0;
+    for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
+    op.row(cx, rr, s, acc);
+  }
+  for (int r = rr + BS; r < ck.r1; r += BS) {
+    const int s0 = sp.rows[r] - j0, s1 = sp.rows[r + 1] - j0;
+    T s = (T)0;
+    for (int k = s0; k < s1; ++k) s = s + prod[k];
+    op.row(cx, r, s, acc);
+  }
+  lds_barrier();
+}
+
 // One generic kernel: every work item whose slot the op accepts is handled by
 // one workgroup -- SpMV ops stage the chunk's products in LDS and give each
 // row its left-to-right sum, vector ops visit the rows -- and the op's finish
-// runs in the workgroup that completes the slot's last chunk.
+// runs in the per-slot finalize launch (bk_fin).  NB is the largest chunk
+// size among the systems in the work list.
 template <class Op, int BS, int NB>
 __global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
   using T = typename Op::T;
@@ -169,52 +223,27 @@ __global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
       if constexpr (Op::SPMV) {
         const int j0 = sp.rows[ck.r0], j1 = sp.rows[ck.r1];
         const int len = j1 - j0;
-        const T *xin = op.xin(cx);
-        const auto *vals = op.vals(sp);
-        if (len <= NB) {
-          // branch-free stream: indices past the chunk are clamped to its last
-          // nonzero (every load unconditional and in bounds); the extra
-          // products land in LDS slots no row reads
-          constexpr int ITEMS = NB / BS;
-          using VT = typename std::remove_cv<typename std::remove_pointer<decltype(vals)>::type>::type;
-          VT vv[ITEMS];
-          int cc[ITEMS];
-          T xv[ITEMS];
-          const int rr = ck.r0 + (int)threadIdx.x;
-          const int rl = min(rr, ck.r1 - 1);
-          const int ra = sp.rows[rl], rb = sp.rows[rl + 1];
-#pragma unroll
-          for (int k = 0; k < ITEMS; ++k) {
-            const int j = lane_nz<BS>(j0, len, k);
-            vv[k] = ntload(vals + j);
-            cc[k] = ntload(sp.cols + j);
-          }
-#pragma unroll
-          for (int k = 0; k < ITEMS; ++k) xv[k] = xin[cc[k]];
-#pragma unroll
-          for (int k = 0; k < ITEMS; ++k) prod[threadIdx.x + k * BS] = (T)vv[k] * xv[k];
-          lds_barrier();
-          if (rr < ck.r1) {
-            T s = (T)0;
-            for (int k = ra - j0; k < rb - j0; ++k) s = s + prod[k];
-            op.row(cx, rr, s, acc);
-          }
-          for (int r = rr + BS; r < ck.r1; r += BS) {
-            const int s0 = sp.rows[r] - j0, s1 = sp.rows[r + 1] - j0;
-            T s = (T)0;
-            for (int k = s0; k < s1; ++k) s = s + prod[k];
-            op.row(cx, r, s, acc);
-          }
-          lds_barrier();
-        } else {
+        // the branch-free stream's depth follows the chunk's length, not the
+        // kernel's capacity NB: every depth gives each row the same
+        // left-to-right sum, so a system's results depend on its chunking only
+        if (len > sp.nb) {
+          // one row longer than the system's chunk size: strided partial sums
           for (int r = ck.r0; r < ck.r1; ++r) {
             T part = (T)0;
+            const auto *vals = op.vals(sp);
+            const T *xin = op.xin(cx);
             for (int j = sp.rows[r] + threadIdx.x; j < sp.rows[r + 1]; j += BS)
               part += (T)ntload(vals + j) * xin[ntload(sp.cols + j)];
             T s = block_sum(part, sh);
             if (threadIdx.x == 0) op.row(cx, r, s, acc);
             __syncthreads();
           }
+        } else if (len <= BS * 4) {
+          chunk_spmv<Op, BS, 4>(op, cx, sp, ck, j0, len, prod, acc);
+        } else if (NB >= BS * 8 && len <= BS * 8) {
+          chunk_spmv<Op, BS, (NB >= BS * 8 ? 8 : 4)>(op, cx, sp, ck, j0, len, prod, acc);
+        } else {
+          chunk_spmv<Op, BS, (NB >= BS * 16 ? 16 : 4)>(op, cx, sp, ck, j0, len, prod, acc);
         }
       } else {
         for (int r = ck.r0 + threadIdx.x; r < ck.r1; r += BS) op.row(cx, r, (T)0, acc);
@@ -278,6 +307,7 @@ struct Ctx64 {
   Vecs<double> v;
   double alpha, beta, omega;
   const double *du;
+  int restart;  // OpInit: a restart (the norms of b are known)
 };
 
 __device__ __forceinline__ Ctx64 ctx64(const BSys &st, const SlotPtrs &sp) {
@@ -287,10 +317,24 @@ __device__ __forceinline__ Ctx64 ctx64(const BSys &st, const SlotPtrs &sp) {
   c.alpha = st.alpha;
   c.omega = st.omega;
   c.beta = 0.0;
+  c.restart = st.restarts > 0;
   return c;
 }
 
-// r = b - A x ; z = r/dg ; r.z, r.r, b.b, (b/D).(b/D)
+// FP64 fast mode: the recurrence residual met the tolerance, or the
+// recurrence broke down -- the next iteration recomputes the true residual
+// from x (OpInit), ends the solve if it meets the tolerance and otherwise
+// restarts the Krylov sequence from it (the single-system fast mode's
+// confirmation and restart).
+__device__ __forceinline__ void slot_restart(BSys &st) {
+  st.restarts += 1;
+  st.snorm_exit = 0;
+  st.phase = PH_INIT;
+}
+
+// r = b - A x ; z = r/dg ; r.z, r.r, b.b, (b/D).(b/D).  On a restart (fast
+// mode) b's norms are known and the third partial is (r/D).(r/D) instead:
+// the true residual is tested before the Krylov sequence restarts from it.
 struct OpInit : OpBase<double> {
   static constexpr bool SPMV = true;
   static constexpr int NV = 4;
@@ -306,13 +350,34 @@ struct OpInit : OpBase<double> {
     c.v.z[i] = zi;
     acc[0] += ri * zi;
     acc[1] += ri * ri;
-    acc[2] += bi * bi;
-    const double ui = c.du ? bi / c.du[i] : bi;
-    acc[3] += ui * ui;
+    const double wi = c.restart ? ri : bi;
+    const double ui = c.du ? wi / c.du[i] : wi;
+    acc[2] += c.restart ? ui * ui : bi * bi;
+    acc[3] += c.restart ? 0.0 : ui * ui;
   }
   __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &sp, double (&tot)[4]) const {
-    st.rhsnorm = sqrt(tot[2]);
-    st.bb_u = tot[3];
+    if (st.restarts == 0) {
+      st.rhsnorm = sqrt(tot[2]);
+      st.bb_u = tot[3];
+    } else {
+      // st.iter is one past the iterations done
+      const double res = sqrt(tot[1]) / st.rhsnorm;
+      st.res = res;
+      if (res < st.tol) {
+        if (slot_unscaled_ok(st, tot[2])) {
+          st.iter -= 1;
+          slot_done(a, st, 1);
+          return;
+        }
+        // the scaled test holds but the unscaled residual misses its
+        // tolerance: tighten the recurrence test by the observed ratio
+        st.tol = res * (st.tol_u / st.res_u);
+      }
+      if (st.iter > st.maxit) {
+        slot_done(a, st, 2);
+        return;
+      }
+    }
     st.rho = sp.method == EHIP_CG ? tot[0] : tot[1];
     st.phase = PH_START;
   }
@@ -340,7 +405,7 @@ struct OpStart : OpBase<double> {
   __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&)[1]) const {
     st.alpha = 0.0;
     st.omega = 1.0;
-    st.iter = 1;
+    if (st.restarts == 0) st.iter = 1;  // a restart keeps counting
     st.phase = PH_ITER;
     if (st.rho == 0.0) slot_done(a, st, method == EHIP_CG ? 20 : 35);
   }
@@ -411,6 +476,45 @@ struct OpCgRes : OpBase<double> {  // ||A x - b|| / ||b|| ; p = z + (rho_new/rho
   }
 };
 
+// ---- CG, fast: recurrence residual, true residual only to confirm (OpInit)
+struct OpCgDaF : OpBase<double> {  // x += alpha p ; r -= alpha q ; z = r/dg ; rho_new, r.r
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 2;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&acc)[2]) const {
+    c.v.x[i] = c.v.x[i] + c.alpha * c.v.p[i];
+    const double ri = c.v.r[i] - c.alpha * c.v.q[i];
+    c.v.r[i] = ri;
+    const double zi = ri / c.v.dg[i];
+    c.v.z[i] = zi;
+    acc[0] += ri * zi;
+    acc[1] += ri * ri;
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[2]) const {
+    st.rho_new = tot[0];
+    st.res = sqrt(tot[1]) / st.rhsnorm;
+    st.iter += 1;
+    if (st.res < st.tol || tot[0] == 0.0) slot_restart(st);
+    else if (st.iter > st.maxit) slot_done(a, st, 2);
+  }
+};
+
+struct OpCgP : OpBase<double> {  // p = z + (rho_new/rho) p
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const {
+    Ctx c = ctx64(st, sp);
+    c.beta = st.rho_new / st.rho;
+    return c;
+  }
+  __device__ void row(Ctx &c, int i, double, double (&)[1]) const { c.v.p[i] = c.v.z[i] + c.beta * c.v.p[i]; }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&)[1]) const { st.rho = st.rho_new; }
+};
+
 // ---- BiCGStab, faithful
 template <typename TT>
 struct CtxT {
@@ -451,14 +555,23 @@ struct OpBiS : OpBase<double> {  // s = r - alpha v ; t2v = s/dg ; ||s||
   }
 };
 
-struct OpBiSnorm : OpBase<double> {  // s-norm exit: x += alpha t1v
+struct OpBiSnorm : OpBase<double> {  // s-norm exit: x += alpha t1v (fast mode: then restart)
   static constexpr bool SPMV = false;
   static constexpr int NV = 0;
   using Ctx = Ctx64;
+  int fast;
   __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && st.snorm_exit; }
   __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
   __device__ void row(Ctx &c, int i, double, double (&)[1]) const { c.v.x[i] = c.v.x[i] + c.alpha * c.v.t1v[i]; }
-  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&)[1]) const { slot_done(a, st, 36); }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&)[1]) const {
+    if (!fast) {
+      slot_done(a, st, 36);
+      return;
+    }
+    st.iter += 1;
+    if (st.iter > st.maxit) slot_done(a, st, 2);
+    else slot_restart(st);
+  }
 };
 
 struct OpBiT : OpBase<double> {  // t = A t2v ; omega = t.s / t.t
@@ -523,6 +636,47 @@ struct OpBiRes : OpBase<double> {  // ||A x - b||/||b|| ; p = r + beta (p - omeg
     else if (st.rho_new == 0.0) slot_done(a, st, 35);
     else st.rho = st.rho_new;
   }
+};
+
+// ---- BiCGStab, fast: recurrence residual, true residual only to confirm (OpInit)
+struct OpBiXrF : OpBase<double> {  // x += alpha t1v + omega t2v ; r = s - omega t ; rho_new, r.r
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 2;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const { return ctx64(st, sp); }
+  __device__ void row(Ctx &c, int i, double, double (&acc)[2]) const {
+    c.v.x[i] = c.v.x[i] + c.alpha * c.v.t1v[i] + c.omega * c.v.t2v[i];
+    const double ri = c.v.sv[i] - c.omega * c.v.t[i];
+    c.v.r[i] = ri;
+    acc[0] += c.v.rtld[i] * ri;
+    acc[1] += ri * ri;
+  }
+  __device__ void finish(BatchArgs &a, BSys &st, const SlotPtrs &, double (&tot)[2]) const {
+    st.rho_new = tot[0];
+    st.res = sqrt(tot[1]) / st.rhsnorm;
+    st.iter += 1;
+    if (st.res < st.tol || st.omega == 0.0 || tot[0] == 0.0) slot_restart(st);
+    else if (st.iter > st.maxit) slot_done(a, st, 2);
+  }
+};
+
+struct OpBiPF : OpBase<double> {  // p = r + beta (p - omega v) ; t1v = p/dg
+  static constexpr bool SPMV = false;
+  static constexpr int NV = 0;
+  using Ctx = Ctx64;
+  __device__ bool active(const BSys &st) const { return st.phase == PH_ITER && !st.snorm_exit; }
+  __device__ Ctx begin(const BSys &st, const SlotPtrs &sp) const {
+    Ctx c = ctx64(st, sp);
+    c.beta = (st.rho_new * st.alpha) / (st.rho * st.omega);
+    return c;
+  }
+  __device__ void row(Ctx &c, int i, double, double (&)[1]) const {
+    const double pi = c.v.r[i] + c.beta * (c.v.p[i] - c.omega * c.v.v[i]);
+    c.v.p[i] = pi;
+    c.v.t1v[i] = pi / c.v.dg[i];
+  }
+  __device__ void finish(BatchArgs &, BSys &st, const SlotPtrs &, double (&)[1]) const { st.rho = st.rho_new; }
 };
 
 // ---------------------------------------------------------------- mixed: FP64 refinement per slot
@@ -600,6 +754,10 @@ struct OpRes : OpBase<double> {
 // residual replacement -- p and q = A p are carried over, rescaled from the
 // previous residual's scale to this one, and beta continues the sequence
 // (beta = r.z / (r.z before the last step, rescaled)).  First time: p = q = 0.
+// (Reliable updates for BiCGStab -- rtld kept, p and v rescaled -- were
+// measured on the convection-diffusion batch: about twice the FP64
+// iteration count like the restarts, slower at every replacement threshold,
+// and divergent at 0.1.)
 struct OpPrep : OpBase<float> {
   static constexpr bool SPMV = false;
   static constexpr int NV = 2;
@@ -882,6 +1040,7 @@ struct ehip_batch {
     float *w32 = nullptr;
     double bscale = 1.0;
     int64_t bytes = 0;
+    int nb = 2048;               // chunk size (nonzeros) of this system
   };
   std::vector<Slot> slot;
   bool graph_ok = false;
@@ -889,6 +1048,11 @@ struct ehip_batch {
   hipGraphExec_t exec = nullptr;
   int graph_iters = 0;
   int mixed = 0, method = 0;
+  int fast = 0;                  // FP64: recurrence residual with true-residual restarts
+  int nbmax = 2048;              // largest chunk size in the work list (selects bk_run's capacity)
+  int grid = 2048;               // workgroups of the multi-system launches
+  int tuning = 0;                // ehip_batch_set_tuning: 0 off, 1 untuned classes, 2 everything
+  int tune_replays = 3;
   int64_t bytes = 0;
   int64_t budget = 0;              // serving: HBM budget for solver workspace (0 = none)
   std::vector<int> batch_members;  // for ehip_batch_* : slot of each system (-1: concurrent lane)
@@ -927,16 +1091,53 @@ struct Lane {
 
 namespace ehip {
 
-static constexpr int kChunkNB = 2048;
+// Chunk sizes (nonzeros per work item) the multi-system kernels are built
+// for, and the workgroup counts the schedule search tries.  The workgroup
+// size stays 256: it fixes which rows each thread's partial sums collect,
+// so it is kept the same for every system.
+constexpr int kChunkSizes[3] = {1024, 2048, 4096};
+constexpr int kGrids[3] = {1024, 2048, 4096};
+constexpr int kDefaultNB = 2048;
+constexpr int kSizeClasses = 6;
 
-static std::vector<Chunk> make_chunks(const ehip_system *s) {
+// Size class of a system by its nonzeros: < 2^15, 2^17, 2^19, 2^21, 2^23, more.
+static int size_class(int64_t nnz) {
+  int c = 0;
+  for (int64_t lim = (int64_t)1 << 15; c < kSizeClasses - 1 && nnz >= lim; lim <<= 2) ++c;
+  return c;
+}
+
+// The schedule table: per (precision form, method, fast mode) the chunk
+// size of each size class and the workgroup count; 0 = not tuned (defaults).
+// Process-wide, filled by the schedule search (ehip_batch_set_tuning) or set
+// directly (ehip_batch_set_schedule).  A system's results depend only on the
+// chunk size of its class, never on the other systems of its batch.
+struct SchedTable {
+  std::mutex mu;
+  int nb[2][2][2][kSizeClasses] = {};
+  int grid[2][2][2] = {};
+};
+static SchedTable g_sched;
+
+static int sched_nb(int mixed, int method, int fast, int cls) {
+  std::lock_guard<std::mutex> lk(g_sched.mu);
+  const int v = g_sched.nb[mixed][method][fast][cls];
+  return v ? v : kDefaultNB;
+}
+static int sched_grid(int mixed, int method, int fast) {
+  std::lock_guard<std::mutex> lk(g_sched.mu);
+  const int v = g_sched.grid[mixed][method][fast];
+  return v ? v : max_grid();
+}
+
+static std::vector<Chunk> make_chunks(const ehip_system *s, int nb) {
   std::vector<Chunk> out;
   const int *rows = s->h_rows.data();
   int r = 0, k = 0;
   while (r < s->n) {
     const int start = rows[r];
     int e = r + 1;
-    if (rows[e] - start > kChunkNB) {
+    if (rows[e] - start > nb) {
       out.push_back({0, r, e, k++});
       r = e;
       continue;
@@ -944,7 +1145,7 @@ static std::vector<Chunk> make_chunks(const ehip_system *s) {
     int lo = e, hi = s->n;
     while (lo < hi) {
       int mid = lo + (hi - lo + 1) / 2;
-      if (rows[mid] - start <= kChunkNB) lo = mid; else hi = mid - 1;
+      if (rows[mid] - start <= nb) lo = mid; else hi = mid - 1;
     }
     out.push_back({0, r, lo, k++});
     r = lo;
@@ -958,8 +1159,13 @@ static BatchArgs args_of(ehip_batch *b) {
 
 template <class Op>
 static void launch_op(ehip_batch *b, const Op &op) {
-  const int grid = max_grid();
-  launch_k((bk_run<Op, 256, kChunkNB>), dim3(grid), dim3(256), b->stream, args_of(b), op);
+  const int grid = b->grid;
+  if (b->nbmax <= 1024)
+    launch_k((bk_run<Op, 256, 1024>), dim3(grid), dim3(256), b->stream, args_of(b), op);
+  else if (b->nbmax <= 2048)
+    launch_k((bk_run<Op, 256, 2048>), dim3(grid), dim3(256), b->stream, args_of(b), op);
+  else
+    launch_k((bk_run<Op, 256, 4096>), dim3(grid), dim3(256), b->stream, args_of(b), op);
   const int fgrid = std::min(b->capacity, 1024);
   launch_k((bk_fin<Op>), dim3(fgrid), dim3(256), b->stream, args_of(b), op, b->capacity);
 }
@@ -971,15 +1177,25 @@ static void enqueue_iteration(ehip_batch *b) {
     launch_op(b, OpStart{{}, b->method});
     if (b->method == EHIP_CG) {
       launch_op(b, OpCgQ{});
-      launch_op(b, OpCgDa{});
-      launch_op(b, OpCgRes{});
+      if (b->fast) {
+        launch_op(b, OpCgDaF{});
+        launch_op(b, OpCgP{});
+      } else {
+        launch_op(b, OpCgDa{});
+        launch_op(b, OpCgRes{});
+      }
     } else {
       launch_op(b, OpBiV{});
       launch_op(b, OpBiS{});
-      launch_op(b, OpBiSnorm{});
+      launch_op(b, OpBiSnorm{{}, b->fast});
       launch_op(b, OpBiT{});
-      launch_op(b, OpBiXr{});
-      launch_op(b, OpBiRes{});
+      if (b->fast) {
+        launch_op(b, OpBiXrF{});
+        launch_op(b, OpBiPF{});
+      } else {
+        launch_op(b, OpBiXr{});
+        launch_op(b, OpBiRes{});
+      }
     }
   } else {
     launch_op(b, OpCorr{});
@@ -1023,6 +1239,7 @@ static int build_graph(ehip_batch *b, int iters) {
 static int upload_worklist(ehip_batch *b) {
   std::vector<Chunk> all;
   std::vector<SlotPtrs> tab(b->capacity);
+  int nbmax = 1024;
   for (int i = 0; i < b->capacity; ++i) {
     auto &sl = b->slot[i];
     SlotPtrs sp{};
@@ -1032,6 +1249,8 @@ static int upload_worklist(ehip_batch *b) {
       sp.b = s->rhs; sp.x = s->x; sp.w64 = sl.w64; sp.w32 = sl.w32;
       sp.du = (b->p.true_residual_stop && b->p.scaling) ? s->dscale : nullptr;
       sp.n = s->n; sp.method = b->method;
+      sp.nb = sl.nb;
+      nbmax = std::max(nbmax, sl.nb);
       sp.ck0 = (int)all.size();
       sp.nck = (int)sl.ck.size();
       for (Chunk c : sl.ck) {
@@ -1052,6 +1271,10 @@ static int upload_worklist(ehip_batch *b) {
     EHIP_HIP_TRY(hipMalloc(&b->p32, sizeof(float) * 4 * b->chunk_cap));
     b->graph_ok = false;  // kernel arguments changed
   }
+  // a larger chunk size than the recorded kernels take needs new ones (a
+  // smaller one runs in the recorded kernels unchanged)
+  if (nbmax > b->nbmax) b->graph_ok = false;
+  if (nbmax > b->nbmax || !b->graph_ok) b->nbmax = nbmax;
   const int nch = (int)all.size();
   if (nch) EHIP_HIP_TRY(hipMemcpyAsync(b->chunks, all.data(), sizeof(Chunk) * nch, hipMemcpyHostToDevice, b->stream));
   EHIP_HIP_TRY(hipMemcpyAsync(b->nchunks, &nch, sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -1069,6 +1292,8 @@ static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
   b->slot.resize(capacity);
   b->mixed = p->precision == EHIP_MIXED_IR;
   b->method = p->method;
+  b->fast = !b->mixed && p->mode == EHIP_FAST;
+  b->grid = sched_grid(b->mixed, b->method == EHIP_CG ? 0 : 1, b->fast);
   EHIP_HIP_TRY(hipMalloc(&b->nchunks, sizeof(int)));
   EHIP_HIP_TRY(hipMalloc(&b->slots, sizeof(SlotPtrs) * capacity));
   EHIP_HIP_TRY(hipMalloc(&b->st, sizeof(BSys) * capacity));
@@ -1156,7 +1381,7 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   z.res_u = -1.0;
   z.maxit = b->p.maxit;
   // CG: residual drop between replacements; BiCGStab: restarted inner tolerance
-  z.itol = (float)(b->p.inner_tol > 0 ? b->p.inner_tol : (b->p.method == EHIP_CG ? 1e-2 : 1e-4));
+  z.itol = (float)(b->p.inner_tol > 0 ? b->p.inner_tol : 1e-2);
   z.inner_maxit = b->p.inner_maxit > 0 ? b->p.inner_maxit : 1000;
   z.phase = b->mixed ? PH_RES : PH_INIT;
   z.omega = 1.0;
@@ -1164,7 +1389,8 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
   sl.s = s;
   sl.used = true;
-  sl.ck = make_chunks(s);
+  sl.nb = sched_nb(b->mixed, b->method == EHIP_CG ? 0 : 1, b->fast, size_class(s->nnz));
+  sl.ck = make_chunks(s, sl.nb);
   b->bytes += sl.bytes;
   // the active counter counts this slot
   hipLaunchKernelGGL(k_inc, dim3(1), dim3(1), 0, b->stream, b->active);
@@ -1180,7 +1406,7 @@ static int engine_retire(ehip_batch *b, int slot, ehip_report *r) {
   ehip_report rep{};
   rep.info = h.info;
   rep.iters = b->mixed ? h.inner_total : h.iter;
-  rep.outer_iters = b->mixed ? h.outer : 0;
+  rep.outer_iters = b->mixed ? h.outer : h.restarts;
   rep.res_scaled = h.res;
   int rc;
   if (b->p.scaling)
@@ -1216,6 +1442,125 @@ static int engine_step(ehip_batch *b) {
   EHIP_HIP_TRY(hipMemcpyAsync(b->h_st, b->st, sizeof(BSys) * b->capacity, hipMemcpyDeviceToHost, b->stream));
   EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
   return EHIP_OK;
+}
+
+// Schedule search over the admitted slots: for every size class present
+// (smallest first) each chunk size is timed over `replays` replays of the
+// iteration graph from the admitted state, the others held at their current
+// choice; then the workgroup count.  Between candidates every slot goes back
+// to the admitted state (its Krylov state and x; everything else is
+// recomputed from them), and after the search the slots are left there with
+// the chosen schedule, which the table keeps for later batches and servers.
+// retune = 0 searches only what the table does not hold yet.
+static int engine_tune(ehip_batch *b, int replays, bool retune) {
+  const int mi = b->method == EHIP_CG ? 0 : 1;
+  std::vector<int> used;
+  int64_t xn = 0;
+  for (int i = 0; i < b->capacity; ++i)
+    if (b->slot[i].used) {
+      used.push_back(i);
+      xn += b->slot[i].s->n;
+    }
+  if (used.empty()) return EHIP_OK;
+  bool todo[kSizeClasses] = {};
+  bool any = false;
+  {
+    std::lock_guard<std::mutex> lk(g_sched.mu);
+    for (int i : used) {
+      const int c = size_class(b->slot[i].s->nnz);
+      if (retune || !g_sched.nb[b->mixed][mi][b->fast][c]) todo[c] = any = true;
+    }
+    if (retune || !g_sched.grid[b->mixed][mi][b->fast]) any = true;
+  }
+  if (!any) return EHIP_OK;
+  // the admitted state
+  EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+  std::vector<BSys> st0(b->capacity);
+  int active0 = 0;
+  EHIP_HIP_TRY(hipMemcpy(st0.data(), b->st, sizeof(BSys) * b->capacity, hipMemcpyDeviceToHost));
+  EHIP_HIP_TRY(hipMemcpy(&active0, b->active, sizeof(int), hipMemcpyDeviceToHost));
+  double *x0 = nullptr;
+  EHIP_HIP_TRY(hipMalloc(&x0, sizeof(double) * xn));
+  std::vector<int64_t> xoff(b->capacity, 0);
+  {
+    int64_t o = 0;
+    for (int i : used) {
+      xoff[i] = o;
+      const ehip_system *s = b->slot[i].s;
+      EHIP_HIP_TRY(hipMemcpy(x0 + o, s->x, sizeof(double) * s->n, hipMemcpyDeviceToDevice));
+      o += s->n;
+    }
+  }
+  auto restore = [&]() -> int {
+    EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+    for (int i : used) {
+      auto &sl = b->slot[i];
+      EHIP_HIP_TRY(hipMemcpyAsync(sl.s->x, x0 + xoff[i], sizeof(double) * sl.s->n, hipMemcpyDeviceToDevice,
+                                  b->stream));
+      sl.ck = make_chunks(sl.s, sl.nb);
+    }
+    EHIP_HIP_TRY(hipMemcpyAsync(b->st, st0.data(), sizeof(BSys) * b->capacity, hipMemcpyHostToDevice, b->stream));
+    EHIP_HIP_TRY(hipMemcpyAsync(b->active, &active0, sizeof(int), hipMemcpyHostToDevice, b->stream));
+    b->graph_ok = false;
+    return upload_worklist(b);
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EHIP_HIP_TRY(hipEventCreate(&e0));
+  EHIP_HIP_TRY(hipEventCreate(&e1));
+  auto timed = [&](double *ms) -> int {
+    int rc;
+    if ((rc = restore())) return rc;
+    if (graphs_enabled() && (rc = build_graph(b, b->p.check_every > 0 ? b->p.check_every : 8))) return rc;
+    EHIP_HIP_TRY(hipEventRecord(e0, b->stream));
+    for (int k = 0; k < replays; ++k)
+      if ((rc = engine_step(b))) return rc;
+    EHIP_HIP_TRY(hipEventRecord(e1, b->stream));
+    EHIP_HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    EHIP_HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    return EHIP_OK;
+  };
+  int rc = EHIP_OK;
+  for (int c = 0; c < kSizeClasses && rc == EHIP_OK; ++c) {
+    if (!todo[c]) continue;
+    double best = 0;
+    int pick = kDefaultNB;
+    for (int nb : kChunkSizes) {
+      for (int i : used)
+        if (size_class(b->slot[i].s->nnz) == c) b->slot[i].nb = nb;
+      double ms = 0;
+      if ((rc = timed(&ms))) break;
+      if (nb == kChunkSizes[0] || ms < best) { best = ms; pick = nb; }
+    }
+    for (int i : used)
+      if (size_class(b->slot[i].s->nnz) == c) b->slot[i].nb = pick;
+    std::lock_guard<std::mutex> lk(g_sched.mu);
+    if (rc == EHIP_OK) g_sched.nb[b->mixed][mi][b->fast][c] = pick;
+  }
+  bool do_grid;
+  {
+    std::lock_guard<std::mutex> lk(g_sched.mu);
+    do_grid = retune || !g_sched.grid[b->mixed][mi][b->fast];
+  }
+  if (rc == EHIP_OK && do_grid) {
+    double best = 0;
+    int pick = b->grid;
+    for (int g : kGrids) {
+      b->grid = g;
+      double ms = 0;
+      if ((rc = timed(&ms))) break;
+      if (g == kGrids[0] || ms < best) { best = ms; pick = g; }
+    }
+    b->grid = pick;
+    std::lock_guard<std::mutex> lk(g_sched.mu);
+    if (rc == EHIP_OK) g_sched.grid[b->mixed][mi][b->fast] = pick;
+  }
+  if (rc == EHIP_OK) rc = restore();
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(x0);
+  return rc;
 }
 
 static void lane_stop(ehip_batch *b) {
@@ -1354,6 +1699,8 @@ extern "C" int ehip_batch_solve(ehip_batch *bh, const ehip_params *p, ehip_repor
     bh->batch_members[i] = slot;
   }
   if ((rc = upload_worklist(e))) { engine_free(e); return rc; }
+  if (bh->tuning && !eng_idx.empty())
+    if ((rc = engine_tune(e, bh->tune_replays, bh->tuning == 2))) { engine_free(e); return rc; }
   EHIP_HIP_TRY(hipEventRecord(e0, e->stream));
   if (!eng_idx.empty()) {
     // every replay runs graph_iters iterations of every live slot
@@ -1404,6 +1751,45 @@ extern "C" int ehip_batch_set_lane(ehip_batch *b, int64_t min_dofs, int workers)
   if (!b || workers < 0 || workers > 64 || min_dofs < 0) return EHIP_EARG;
   b->lane_min_dofs = min_dofs;
   b->lane_workers = workers;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_set_tuning(ehip_batch *b, int mode, int replays) {
+  if (!b || mode < 0 || mode > 2 || replays < 0 || replays > 64) return EHIP_EARG;
+  b->tuning = mode;
+  b->tune_replays = replays > 0 ? replays : 3;
+  return EHIP_OK;
+}
+
+static int sched_key(const ehip_params *p, int *mixed, int *mi, int *fast) {
+  if (!p || (p->method != EHIP_CG && p->method != EHIP_BICGSTAB)) return EHIP_EARG;
+  *mixed = p->precision == EHIP_MIXED_IR;
+  *mi = p->method == EHIP_CG ? 0 : 1;
+  *fast = !*mixed && p->mode == EHIP_FAST;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_get_schedule(const ehip_params *p, int64_t nnz, int *chunk_nnz, int *grid) {
+  int mixed, mi, fast;
+  if (nnz < 0 || sched_key(p, &mixed, &mi, &fast)) return EHIP_EARG;
+  if (chunk_nnz) *chunk_nnz = sched_nb(mixed, mi, fast, size_class(nnz));
+  if (grid) *grid = sched_grid(mixed, mi, fast);
+  return EHIP_OK;
+}
+
+extern "C" int ehip_batch_set_schedule(const ehip_params *p, int64_t nnz, int chunk_nnz, int grid) {
+  int mixed, mi, fast;
+  if (nnz < 0 || sched_key(p, &mixed, &mi, &fast)) return EHIP_EARG;
+  bool nb_ok = chunk_nnz == 0 || chunk_nnz == -1;
+  for (int c : kChunkSizes) nb_ok = nb_ok || chunk_nnz == c;
+  if (!nb_ok || grid < -1 || grid > 65536 || (grid > 0 && grid % 8)) {
+    set_error("ehip_batch_set_schedule: chunk size must be -1, 0, 1024, 2048 or 4096 and the grid -1, 0 or a "
+              "multiple of 8 up to 65536");
+    return EHIP_EARG;
+  }
+  std::lock_guard<std::mutex> lk(g_sched.mu);
+  if (chunk_nnz >= 0) g_sched.nb[mixed][mi][fast][size_class(nnz)] = chunk_nnz;
+  if (grid >= 0) g_sched.grid[mixed][mi][fast] = grid;
   return EHIP_OK;
 }
 

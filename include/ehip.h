@@ -74,7 +74,7 @@ typedef struct {
   int mode;          /* EHIP_FAITHFUL | EHIP_FAST */
   double inner_tol;  /* mixed form: residual reduction per refinement step.  CG: drop of the
                         FP32 residual between two FP64 residual replacements (0 = 1e-2);
-                        BiCGStab: relative tolerance of each restarted inner solve (0 = 1e-4) */
+                        BiCGStab: relative tolerance of each restarted inner solve (0 = 1e-2) */
   int inner_maxit;   /* mixed BiCGStab: inner iteration cap per refinement step (0 = 1000) */
   int check_every;   /* host polls convergence every k iterations (0 = default) */
   double divtol;     /* "Linear System Divergence Tolerance" (0 = none) */
@@ -325,10 +325,33 @@ int ehip_batch_set_lane(ehip_batch *b, int64_t lane_min_dofs, int lane_workers);
 /* reports[nsys]; solutions stay on the device (ehip_batch_get_solution).
  * t_ms: device time of the multi-system launches, or the wall time of the
  * whole batch when some systems took the concurrent lane.  In the
- * multi-system launches p->precision selects FP64 (HUTI's loop order,
- * whatever p->mode says) or mixed precision; p->mode applies to systems on
- * the concurrent lane, which run ehip_solve. */
+ * multi-system launches p->precision selects FP64 or mixed precision.  FP64
+ * with EHIP_FAITHFUL runs HUTI's loop order (true residual every iteration,
+ * HUTI's exits); with EHIP_FAST it runs the recurrence residual and, when
+ * that meets tol, or on the S-norm, omega or rho breakdown, recomputes the
+ * true residual from x and either ends the solve or restarts the Krylov
+ * sequence from it (report.outer_iters = restarts).  Systems on the
+ * concurrent lane run ehip_solve with p as given. */
 int ehip_batch_solve(ehip_batch *b, const ehip_params *p, ehip_report *reports, double *t_ms);
+/* Schedule search before the next ehip_batch_solve: mode 1 times the chunk
+ * sizes (1024, 2048, 4096 nonzeros per work item) for every size class of
+ * the batch that the process-wide schedule table does not hold yet, then
+ * the workgroup count (1024, 2048, 4096), each over `replays` graph replays
+ * (0 = 3) from the admitted state; mode 2 searches everything again; 0
+ * (the default) uses the table, or chunk size 2048 and 2048 workgroups where
+ * it holds nothing.  Size classes by nonzeros: < 2^15, < 2^17, < 2^19,
+ * < 2^21, < 2^23, more.  A system's results depend on its class's chunk
+ * size only (never on the other systems of the batch or the workgroup
+ * count), so a system solved alone and in a batch under the same table
+ * gives the same bits.  No counterpart in the reference. */
+int ehip_batch_set_tuning(ehip_batch *b, int mode, int replays);
+/* The table entry for solver form p (precision, method, mode) and a system
+ * of nnz nonzeros: its chunk size, and the form's workgroup count. */
+int ehip_batch_get_schedule(const ehip_params *p, int64_t nnz, int *chunk_nnz, int *grid);
+/* Sets it: chunk_nnz 1024 | 2048 | 4096, 0 = default, -1 = unchanged;
+ * grid a multiple of 8 up to 65536, 0 = default, -1 = unchanged.  Applies to
+ * engines created afterwards (batches and servers). */
+int ehip_batch_set_schedule(const ehip_params *p, int64_t nnz, int chunk_nnz, int grid);
 int ehip_batch_get_solution(ehip_batch *b, int i, double *x);
 
 /* ---- serving: continuous batching ----

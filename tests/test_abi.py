@@ -98,3 +98,34 @@ end program l
                                "-lehip", "-Wl,-rpath," + libdir], cwd=d)
         a, b = map(int, subprocess.check_output([exe], text=True).split())
     assert a == C.sizeof(_lib.Params) and b == C.sizeof(_lib.Report)
+
+
+def test_schedule_table_host_logic():
+    """The multi-system schedule table is host state: size classes by
+    nonzeros, defaults, per-form keys, and argument checks."""
+    L = _lib.load()
+    p = _lib.Params()
+    p.method, p.precision, p.mode = _lib.BICGSTAB, _lib.FP64, _lib.FAST
+    nb, grid = C.c_int(), C.c_int()
+
+    def get(nnz, q=p):
+        assert L.ehip_batch_get_schedule(C.byref(q), nnz, C.byref(nb), C.byref(grid)) == 0
+        return nb.value, grid.value
+
+    assert get(100) == (2048, 2048)
+    assert L.ehip_batch_set_schedule(C.byref(p), (1 << 17) + 5, 4096, 1024) == 0
+    # same class: [2^17, 2^19); neighbours keep the default chunk size, the grid is per form
+    assert get(1 << 17) == (4096, 1024) and get((1 << 19) - 1) == (4096, 1024)
+    assert get((1 << 17) - 1) == (2048, 1024) and get(1 << 19) == (2048, 1024)
+    assert get(1 << 40) == (2048, 1024)
+    q = _lib.Params()
+    q.method, q.precision, q.mode = _lib.BICGSTAB, _lib.FP64, _lib.FAITHFUL   # another form
+    assert get(1 << 17, q) == (2048, 2048)
+    assert L.ehip_batch_set_schedule(C.byref(p), 1 << 17, -1, -1) == 0      # unchanged
+    assert get(1 << 17) == (4096, 1024)
+    for bad_nb, bad_grid in ((3000, 0), (0, 12), (0, -2), (8192, 0), (0, 1 << 20)):
+        assert L.ehip_batch_set_schedule(C.byref(p), 1, bad_nb, bad_grid) == _lib.EARG
+    assert L.ehip_batch_set_schedule(C.byref(p), 1 << 17, 0, 0) == 0        # back to defaults
+    assert get(1 << 17) == (2048, 2048)
+    p.method = 7
+    assert L.ehip_batch_get_schedule(C.byref(p), 1, C.byref(nb), C.byref(grid)) == _lib.EARG
