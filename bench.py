@@ -427,9 +427,47 @@ def form_elastic(args):
     return 0
 
 
+def form_assembly(args):
+    """Assembly form: GPU assembly (element blocks + deterministic gather into
+    CRS Values and RHS) of the n^3 Hex8 Poisson case with lognormal
+    conductivity, timed over `--steps` calls after `--warmup`; then the same
+    number of calls that keep the coordinates on the device and only enqueue
+    (the re-assembly a parameter sweep makes), timed on the host including the
+    coefficient upload."""
+    from elmerfem_amd import _lib, workloads as W
+    _lib.init(0)
+    w = W.poisson_hex8(args.n, sigma=args.sigma, seed=args.seed)
+    m = w["mesh"]
+    t0 = time.perf_counter()
+    s, _ = W.build_system(w, optimize_bandwidth=True)
+    setup = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        s.assemble(w["physics"], m.coords, elem_coef=w["coef"], elem_load=w["load"])
+    ts = [s.assemble(w["physics"], None, elem_coef=w["coef"], elem_load=w["load"]) for _ in range(args.steps)]
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        s.assemble(w["physics"], None, elem_coef=w["coef"], elem_load=w["load"], wait=False)
+    s.values()
+    host_ms = 1000.0 * (time.perf_counter() - t) / args.steps
+    npe = m.elements.shape[1]
+    asm_bytes = 24 * m.nnodes + 8 * m.nelem + 4 * npe * m.nelem + 4 * npe * npe * m.nelem + 8 * s.nnz + 8 * s.n
+    ms = float(np.median(ts))
+    rec = {"metric": "GPU assembly time (element blocks + gather into CRS)", "value": ms, "unit": "ms",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "poisson_hex8_%d_lognormal" % args.n, "elements": int(m.nelem),
+                      "nodes": int(m.nnodes), "nnz": int(s.nnz), "seed": args.seed},
+           "elements_per_s": m.nelem / (ms * 1e-3), "alg_bytes": asm_bytes,
+           "gbs_algorithmic": asm_bytes / (ms * 1e-3) / 1e9, "reassembly_host_ms": host_ms,
+           "host_setup_s": setup}
+    print(json.dumps(rec), flush=True)
+    s.free()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--form", choices=["headline", "batch", "serve", "elastic", "ilu"], default="headline",
+    ap.add_argument("--form", choices=["headline", "batch", "serve", "elastic", "ilu", "assembly"], default="headline",
                     help="headline: the 128^3 solve; batch / serve: the batching and serving forms; "
                          "elastic: the precision form on elasticity (--n 96, or --quadratic --n 48)")
     ap.add_argument("--quadratic", action="store_true", help="elastic form: Hex27 elements")
@@ -469,6 +507,8 @@ def main():
         return form_serve(args)
     if args.form == "ilu":
         return form_ilu(args)
+    if args.form == "assembly":
+        return form_assembly(args)
     if args.form == "elastic":
         if args.n == 128:
             args.n = 48 if args.quadratic else 96

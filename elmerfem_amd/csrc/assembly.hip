@@ -276,7 +276,7 @@ struct ElemArgs {
   const double *gs;     // [ngp]
   const double *coef, *conv, *velo, *load;  // per element (may be null)
   double poisson;
-  double *emat;         // nelem * ld * ld, row-major local matrices
+  double *emat;         // nelem blocks: ld x ld row-major, or packed upper triangles
   double *evec;         // nelem * ld
   int *bad;             // degenerate element counter
 };
@@ -432,14 +432,202 @@ __global__ __launch_bounds__(BS) void k_element(ElemArgs a) {
   }  // element batches
 }
 
-// Values[s] = sum over the slot's contributions in element order, from 0
-__global__ void k_gather(int64_t nslots, const int *__restrict__ start, const int *__restrict__ contrib,
-                         const double *__restrict__ emat, double *__restrict__ vals) {
-  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
-    const int a = start[s], b = start[s + 1];
-    double v = 0.0;
-    for (int k = a; k < b; ++k) v = v + emat[(uint32_t)contrib[k]];
-    vals[s] = v;
+// Thread-per-element ModelPDE blocks for the first-order elements (Tri3,
+// Quad4, Tet4, Hex8): coordinates, every Gauss point's metric and global
+// basis gradients, and the NPE x NPE block stay in registers; the element
+// tables are read with wave-uniform loads.  The arithmetic is k_element's,
+// term for term and in the same order.  SYM (no convection term): only
+// p <= q is accumulated and the block is stored as its packed upper
+// triangle (row p holds q = p..NPE-1).
+template <int NPE>
+__host__ __device__ constexpr int tri_index(int p, int q) {
+  return p * NPE - p * (p - 1) / 2 + (q - p);
+}
+
+// The element tables by value: they travel in the kernel arguments, so every
+// read is a scalar load shared by the wave (2.1 KB for Hex8).
+template <int NPE, int NGP>
+struct RegTables {
+  double basis[NGP * NPE];
+  double dlb[NGP * NPE * 3];
+  double gs[NGP];
+};
+
+template <int NPE, int NGP, int DIM, int CDIM, bool SYM>
+__global__ __launch_bounds__(256) void k_element_reg(ElemArgs a, RegTables<NPE, NGP> t) {
+  constexpr int NK = SYM ? NPE * (NPE + 1) / 2 : NPE * NPE;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.nelem; e += gridDim.x * blockDim.x) {
+    // node ids stay in registers; the coordinates are read again at every
+    // Gauss point (cache hits) rather than held, to keep the block, the
+    // gradients and the metric in registers without spilling
+    int gid[NPE];
+#pragma unroll
+    for (int nd = 0; nd < NPE; nd++) gid[nd] = a.el[(size_t)e * NPE + nd];
+    const double c0 = a.coef ? a.coef[e] : 0.0;
+    const double c1 = (!SYM && a.conv) ? a.conv[e] : 0.0;
+    const double v0 = (!SYM && a.velo) ? a.velo[3 * (size_t)e] : 0.0;
+    const double v1 = (!SYM && a.velo) ? a.velo[3 * (size_t)e + 1] : 0.0;
+    const double v2 = (!SYM && a.velo) ? a.velo[3 * (size_t)e + 2] : 0.0;
+    const double ld = a.load ? a.load[e] : 0.0;
+    double K[NK], F[NPE];
+#pragma unroll
+    for (int k = 0; k < NK; k++) K[k] = 0.0;
+#pragma unroll
+    for (int p = 0; p < NPE; p++) F[p] = 0.0;
+#pragma unroll 1
+    for (int gp = 0; gp < NGP; gp++) {
+      double dx[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        double X[NPE];
+#pragma unroll
+        for (int nd = 0; nd < NPE; nd++) X[nd] = a.xyz[3 * (size_t)gid[nd] + c];
+#pragma unroll
+        for (int i = 0; i < DIM; i++) {
+          double sdx = 0.0;
+#pragma unroll
+          for (int nd = 0; nd < NPE; nd++) sdx += X[nd] * t.dlb[((size_t)gp * NPE + nd) * 3 + i];
+          dx[c][i] = sdx;
+        }
+      }
+      double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, detj = 0.0;
+      if (!metric<DIM, CDIM>(dx, L, detj)) {
+        atomicAdd(a.bad, 1);
+        detj = 0.0;
+      }
+      const double W = t.gs[gp] * detj;
+      double sb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int nd = 0; nd < NPE; nd++) {
+        const double b = t.basis[gp * NPE + nd];
+        sb[0] += b * c0;
+        sb[1] += b * c1;
+        sb[2] += v0 * b;
+        sb[3] += v1 * b;
+        sb[4] += v2 * b;
+        sb[5] += b * ld;
+      }
+      double dB[NPE][3];
+#pragma unroll
+      for (int nd = 0; nd < NPE; nd++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          double sv = 0.0;
+          if (j < CDIM)
+#pragma unroll
+            for (int kk = 0; kk < DIM; kk++) sv = sv + t.dlb[((size_t)gp * NPE + nd) * 3 + kk] * L[j][kk];
+          dB[nd][j] = sv;
+        }
+      const double WD = W * sb[0];
+      const double C = sb[1];
+#pragma unroll
+      for (int p = 0; p < NPE; p++) {
+#pragma unroll
+        for (int q = SYM ? p : 0; q < NPE; q++) {
+          const double m = dB[p][0] * dB[q][0] + dB[p][1] * dB[q][1] + dB[p][2] * dB[q][2];
+          double acc = K[SYM ? tri_index<NPE>(p, q) : p * NPE + q];
+          acc = acc + WD * m;
+          if (!SYM && C != 0.0) {
+            double sa = 0.0;
+#pragma unroll
+            for (int kk = 0; kk < CDIM; kk++) sa += sb[2 + kk] * dB[q][kk];
+            acc = acc + W * C * sa * t.basis[gp * NPE + p];
+          }
+          K[SYM ? tri_index<NPE>(p, q) : p * NPE + q] = acc;
+        }
+        F[p] = F[p] + W * sb[5] * t.basis[gp * NPE + p];
+      }
+    }
+    // packed (SYM) or row-major; the full row-major block with the lower
+    // triangle mirrored measured the same in total (element kernel +31%,
+    // gather -19% at 128^3 Hex8) with 470 MB more traffic
+    double *out = a.emat + (size_t)e * NK;
+#pragma unroll
+    for (int k = 0; k < NK; k++) out[k] = K[k];
+#pragma unroll
+    for (int p = 0; p < NPE; p++) a.evec[(size_t)e * NPE + p] = F[p];
+  }
+}
+
+// Gather (the additions of DefaultUpdateEquations / CRS_GlueLocalMatrix,
+// CRSMatrix.src:507-590) over the row-owner maps: one thread per matrix row
+// adds, for every element touching the row in element order, that
+// element's block row into the row's slots at the column positions of the
+// maps, and its force entry into the right-hand side -- every slot receives
+// its contributions in element order, starting from zero.  A workgroup's
+// BS rows have consecutive slots: when they fit in CAP doubles they are
+// accumulated in LDS (zeroed, added, written out with coalesced stores),
+// otherwise in place in Values.
+template <int LD, bool SYM, typename OffT>
+__device__ __forceinline__ double row_accumulate(int r, double *dst, const int *__restrict__ rstart,
+                                                 const int *__restrict__ rcontrib, const OffT *__restrict__ roff,
+                                                 const double *__restrict__ emat, const double *__restrict__ evec) {
+  constexpr int NK = SYM ? LD * (LD + 1) / 2 : LD * LD;
+  // the block row in pieces of CH columns, and the entries of small blocks
+  // in groups of G: a piece's loads (of every entry in the group) go out
+  // before its additions, which stay in entry order, within an entry in
+  // column order
+  constexpr int CH = LD <= 16 ? LD : (LD % 9 == 0 ? 9 : (LD % 10 == 0 ? 10 : (LD % 8 == 0 ? 8 : 1)));
+  constexpr int G = LD <= 8 ? 4 : 1;
+  double f = 0.0;
+  const int k0 = rstart[r], k1 = rstart[r + 1];
+  for (int kg = k0; kg < k1; kg += G) {
+    int code[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) code[g] = rcontrib[min(kg + g, k1 - 1)];
+    for (int l0 = 0; l0 < LD; l0 += CH) {
+      double v[G][CH];
+      int o[G][CH];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int e = code[g] / LD, li = code[g] - e * LD;
+        const double *blk = emat + (size_t)e * NK;
+        const OffT *off = roff + (size_t)min(kg + g, k1 - 1) * LD;
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+          const int lj = l0 + u;
+          const int idx = SYM ? (li <= lj ? tri_index<LD>(li, lj) : tri_index<LD>(lj, li)) : li * LD + lj;
+          v[g][u] = blk[idx];
+          o[g][u] = (int)off[lj];
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (kg + g < k1)
+#pragma unroll
+          for (int u = 0; u < CH; ++u) dst[o[g][u]] += v[g][u];
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (kg + g < k1) f = f + evec[code[g]];
+  }
+  return f;
+}
+
+template <int LD, bool SYM, typename OffT, int BS, int CAP>
+__global__ __launch_bounds__(BS) void k_row_gather(int n, const int *__restrict__ rows,
+                                                    const int *__restrict__ rstart, const int *__restrict__ rcontrib,
+                                                    const OffT *__restrict__ roff, const double *__restrict__ emat,
+                                                    const double *__restrict__ evec, double *vals, double *rhs) {
+  __shared__ double acc[CAP];
+  const XcdWalk w((n + BS - 1) / BS);
+  for (int rb = w.first; rb < w.last; rb += w.step) {
+    const int r0 = rb * BS;
+    const int r1 = min(r0 + BS, n);
+    const int base = rows[r0], len = rows[r1] - base;
+    const int r = r0 + (int)threadIdx.x;
+    if (len <= CAP) {
+      for (int j = threadIdx.x; j < len; j += BS) acc[j] = 0.0;
+      __syncthreads();
+      if (r < r1)
+        rhs[r] = row_accumulate<LD, SYM, OffT>(r, acc + (rows[r] - base), rstart, rcontrib, roff, emat, evec);
+      __syncthreads();
+      for (int j = threadIdx.x; j < len; j += BS) vals[base + j] = acc[j];
+      __syncthreads();
+    } else if (r < r1) {
+      for (int j = rows[r]; j < rows[r + 1]; ++j) vals[j] = 0.0;
+      rhs[r] = row_accumulate<LD, SYM, OffT>(r, vals + rows[r], rstart, rcontrib, roff, emat, evec);
+    }
   }
 }
 
@@ -447,22 +635,50 @@ struct DevTables {
   double *basis = nullptr, *dlb = nullptr, *gs = nullptr;
 };
 
-static int upload_tables(ehip_system *s, const EType &t, DevTables &d) {
-  int rc;
-  if ((rc = dmalloc(nullptr, (void **)&d.basis, sizeof(double) * t.basis.size()))) return rc;
-  if ((rc = dmalloc(nullptr, (void **)&d.dlb, sizeof(double) * t.dlb.size()))) return rc;
-  if ((rc = dmalloc(nullptr, (void **)&d.gs, sizeof(double) * t.gs.size()))) return rc;
-  EHIP_HIP_TRY(hipMemcpyAsync(d.basis, t.basis.data(), sizeof(double) * t.basis.size(), hipMemcpyHostToDevice, s->stream));
-  EHIP_HIP_TRY(hipMemcpyAsync(d.dlb, t.dlb.data(), sizeof(double) * t.dlb.size(), hipMemcpyHostToDevice, s->stream));
-  EHIP_HIP_TRY(hipMemcpyAsync(d.gs, t.gs.data(), sizeof(double) * t.gs.size(), hipMemcpyHostToDevice, s->stream));
-  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
-  return EHIP_OK;
+__global__ void k_zero_int(int *p) { *p = 0; }
+
+// Rows per workgroup and LDS doubles: 128 rows in 32 KB for blocks of up to
+// 8 rows (rows of up to 32 entries on average stay in LDS), 64 rows in 64 KB
+// for blocks of up to 30 rows, 16 rows in 64 KB for the 81-row Hex27
+// elasticity blocks (rows of up to 375 entries)
+template <int LD, bool SYM>
+static void row_gather_as(ehip_system *s) {
+  constexpr int BS = LD <= 8 ? 128 : (LD <= 30 ? 64 : 16);
+  constexpr int CAP = LD <= 8 ? 4096 : 8192;
+  const int grid = (std::min((s->n + BS - 1) / BS, 32768) + 7) / 8 * 8;  // a multiple of 8 (XcdWalk)
+  if (s->row_off_bytes == 1)
+    hipLaunchKernelGGL((k_row_gather<LD, SYM, uint8_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
+                       s->rows, s->rhs_start, s->rhs_contrib, (const uint8_t *)s->row_off, s->emat, s->evec, s->vals,
+                       s->rhs);
+  else
+    hipLaunchKernelGGL((k_row_gather<LD, SYM, uint16_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
+                       s->rows, s->rhs_start, s->rhs_contrib, (const uint16_t *)s->row_off, s->emat, s->evec,
+                       s->vals, s->rhs);
 }
 
-static void free_tables(DevTables &d) {
-  dfree(nullptr, d.basis, 0);
-  dfree(nullptr, d.dlb, 0);
-  dfree(nullptr, d.gs, 0);
+// The gathers compiled: block row lengths of the supported element types
+// (Tri3, Quad4/Tet4, Hex8, Tet10, Hex27 scalar; Hex8, Tet10, Hex27 with 3 dofs)
+static int launch_row_gather(ehip_system *s, int ld, bool packed) {
+  if (packed) {
+    switch (ld) {
+      case 3: row_gather_as<3, true>(s); return EHIP_OK;
+      case 4: row_gather_as<4, true>(s); return EHIP_OK;
+      case 8: row_gather_as<8, true>(s); return EHIP_OK;
+    }
+  } else {
+    switch (ld) {
+      case 3: row_gather_as<3, false>(s); return EHIP_OK;
+      case 4: row_gather_as<4, false>(s); return EHIP_OK;
+      case 8: row_gather_as<8, false>(s); return EHIP_OK;
+      case 10: row_gather_as<10, false>(s); return EHIP_OK;
+      case 24: row_gather_as<24, false>(s); return EHIP_OK;
+      case 27: row_gather_as<27, false>(s); return EHIP_OK;
+      case 30: row_gather_as<30, false>(s); return EHIP_OK;
+      case 81: row_gather_as<81, false>(s); return EHIP_OK;
+    }
+  }
+  set_error("ehip_assemble: no gather for element blocks of %d rows", ld);
+  return EHIP_EUNSUP;
 }
 
 template <int NPE, int NGP, int DIM, int CDIM, int PHYS, int EPB, int BS>
@@ -471,12 +687,31 @@ static void launch_element(ehip_system *s, const ElemArgs &a) {
   hipLaunchKernelGGL((k_element<NPE, NGP, DIM, CDIM, PHYS, EPB, BS>), dim3(grid), dim3(BS), 0, s->stream, a);
 }
 
-static int dispatch_element(ehip_system *s, int physics, int code, int cdim, const ElemArgs &a) {
+template <int NPE, int NGP, int DIM, int CDIM>
+static void launch_element_reg(ehip_system *s, const ElemArgs &a, const EType &et, bool *packed) {
+  const int grid = std::min((a.nelem + 255) / 256, 16384);
+  RegTables<NPE, NGP> t;
+  std::copy(et.basis.begin(), et.basis.end(), t.basis);
+  std::copy(et.dlb.begin(), et.dlb.end(), t.dlb);
+  std::copy(et.gs.begin(), et.gs.end(), t.gs);
+  *packed = a.conv == nullptr && a.velo == nullptr;
+  if (*packed)
+    hipLaunchKernelGGL((k_element_reg<NPE, NGP, DIM, CDIM, true>), dim3(grid), dim3(256), 0, s->stream, a, t);
+  else
+    hipLaunchKernelGGL((k_element_reg<NPE, NGP, DIM, CDIM, false>), dim3(grid), dim3(256), 0, s->stream, a, t);
+}
+
+// Element blocks: first-order ModelPDE elements one thread per element
+// (packed symmetric blocks without convection), the rest one workgroup per
+// batch of elements.  *packed tells the gather which block layout it reads.
+static int dispatch_element(ehip_system *s, int physics, int code, int cdim, const ElemArgs &a, const EType &et,
+                            bool *packed) {
+  *packed = false;
   if (physics == EHIP_PHYS_MODELPDE) {
-    if (code == 808 && cdim == 3) { launch_element<8, 8, 3, 3, 0, 8, 256>(s, a); return EHIP_OK; }
-    if (code == 504 && cdim == 3) { launch_element<4, 4, 3, 3, 0, 16, 256>(s, a); return EHIP_OK; }
-    if (code == 404 && cdim == 2) { launch_element<4, 4, 2, 2, 0, 16, 256>(s, a); return EHIP_OK; }
-    if (code == 303 && cdim == 2) { launch_element<3, 3, 2, 2, 0, 16, 256>(s, a); return EHIP_OK; }
+    if (code == 808 && cdim == 3) { launch_element_reg<8, 8, 3, 3>(s, a, et, packed); return EHIP_OK; }
+    if (code == 504 && cdim == 3) { launch_element_reg<4, 4, 3, 3>(s, a, et, packed); return EHIP_OK; }
+    if (code == 404 && cdim == 2) { launch_element_reg<4, 4, 2, 2>(s, a, et, packed); return EHIP_OK; }
+    if (code == 303 && cdim == 2) { launch_element_reg<3, 3, 2, 2>(s, a, et, packed); return EHIP_OK; }
     if (code == 510 && cdim == 3) { launch_element<10, 27, 3, 3, 0, 2, 256>(s, a); return EHIP_OK; }
     if (code == 827 && cdim == 3) { launch_element<27, 27, 3, 3, 0, 1, 256>(s, a); return EHIP_OK; }
   } else {
@@ -505,10 +740,12 @@ extern "C" int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *
     set_error("ehip_system_from_mesh: pattern build failed (%d): node ids out of range, unused nodes or nnz >= 2^31", prc);
     return prc == -3 ? EHIP_ECAP : EHIP_EARG;
   }
-  std::vector<int> sstart, scontrib, rstart, rcontrib;
-  prc = build_slot_maps(P, nelem, npe, elements, sstart, scontrib, rstart, rcontrib);
+  std::vector<int> rstart, rcontrib;
+  std::vector<uint16_t> roff;
+  int max_len = 0;
+  prc = build_row_maps(P, nelem, npe, elements, rstart, rcontrib, roff, &max_len);
   if (prc) {
-    set_error("ehip_system_from_mesh: element-to-slot map exceeds int32 (%d)", prc);
+    set_error("ehip_system_from_mesh: element blocks exceed int32 or a row is longer than 65536 (%d)", prc);
     return EHIP_ECAP;
   }
   ehip_system *s = new ehip_system();
@@ -530,36 +767,86 @@ extern "C" int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *
   s->nelem = nelem;
   s->npe = npe;
   const int ld = npe * dofs;
-  s->ncontrib = (int64_t)scontrib.size();
   std::vector<int> el0((size_t)nelem * npe);
   for (size_t k = 0; k < el0.size(); ++k) el0[k] = elements[k] - 1;
+  // column positions in one byte when every row has at most 256 entries
+  s->row_off_bytes = max_len <= 256 ? 1 : 2;
+  std::vector<uint8_t> roff8;
+  if (s->row_off_bytes == 1) {
+    roff8.resize(roff.size());
+    for (size_t k = 0; k < roff.size(); ++k) roff8[k] = (uint8_t)roff[k];
+  }
+  const size_t off_bytes = roff.size() * (size_t)s->row_off_bytes;
   if ((rc = dmalloc(s, (void **)&s->elements, sizeof(int) * el0.size())) ||
-      (rc = dmalloc(s, (void **)&s->slot_start, sizeof(int) * sstart.size())) ||
-      (rc = dmalloc(s, (void **)&s->slot_contrib, sizeof(int) * scontrib.size())) ||
       (rc = dmalloc(s, (void **)&s->rhs_start, sizeof(int) * rstart.size())) ||
       (rc = dmalloc(s, (void **)&s->rhs_contrib, sizeof(int) * rcontrib.size())) ||
+      (rc = dmalloc(s, &s->row_off, off_bytes)) ||
       (rc = dmalloc(s, (void **)&s->emat, sizeof(double) * (size_t)nelem * ld * ld)) ||
       (rc = dmalloc(s, (void **)&s->evec, sizeof(double) * (size_t)nelem * ld))) {
     ehip_system_free(s);
     return rc;
   }
   EHIP_HIP_TRY(hipMemcpy(s->elements, el0.data(), sizeof(int) * el0.size(), hipMemcpyHostToDevice));
-  EHIP_HIP_TRY(hipMemcpy(s->slot_start, sstart.data(), sizeof(int) * sstart.size(), hipMemcpyHostToDevice));
-  EHIP_HIP_TRY(hipMemcpy(s->slot_contrib, scontrib.data(), sizeof(int) * scontrib.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->rhs_start, rstart.data(), sizeof(int) * rstart.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->rhs_contrib, rcontrib.data(), sizeof(int) * rcontrib.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->row_off, s->row_off_bytes == 1 ? (const void *)roff8.data() : (const void *)roff.data(),
+                         off_bytes, hipMemcpyHostToDevice));
   *out = s;
   return EHIP_OK;
 }
 
+namespace ehip {
+
+// Element tables of one type, uploaded once per system and element type.
+static int ensure_tables(ehip_system *s, int code, const EType &t, DevTables &d) {
+  const size_t nb = t.basis.size(), nd = t.dlb.size(), ng = t.gs.size();
+  if (s->asm_code != code || !s->asm_tab) {
+    if (s->asm_tab) dfree(s, s->asm_tab, 0);
+    s->asm_tab = nullptr;
+    s->asm_code = 0;
+    int rc;
+    if ((rc = dmalloc(s, (void **)&s->asm_tab, sizeof(double) * (nb + nd + ng)))) return rc;
+    std::vector<double> h;
+    h.reserve(nb + nd + ng);
+    h.insert(h.end(), t.basis.begin(), t.basis.end());
+    h.insert(h.end(), t.dlb.begin(), t.dlb.end());
+    h.insert(h.end(), t.gs.begin(), t.gs.end());
+    EHIP_HIP_TRY(hipMemcpy(s->asm_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    s->asm_code = code;
+  }
+  d.basis = s->asm_tab;
+  d.dlb = s->asm_tab + nb;
+  d.gs = s->asm_tab + nb + nd;
+  return EHIP_OK;
+}
+
+// The degenerate-metric count of an asynchronous assembly, once it is done.
+int assembly_check(ehip_system *s) {
+  if (!s->asm_pending) return EHIP_OK;
+  s->asm_pending = 0;
+  int hbad = 0;
+  EHIP_HIP_TRY(hipMemcpyAsync(&hbad, s->asm_bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  if (hbad) {
+    set_error("ehip_assemble: %d degenerate Gauss-point metrics (ElementMetric)", hbad);
+    return EHIP_EARG;
+  }
+  return EHIP_OK;
+}
+
+}  // namespace ehip
+
 extern "C" int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdim, const double *coords,
                              const double *elem_coef, const double *elem_conv, const double *elem_velo,
                              const double *elem_load, double poisson, double *t_ms) {
-  if (!s || !coords || !s->emat) {
+  if (!s || !s->emat) {
     set_error("ehip_assemble: system was not created from a mesh");
     return EHIP_ESTATE;
   }
-  const int ld = s->npe * s->dofs;
+  if (!coords && !s->asm_xyz) {
+    set_error("ehip_assemble: no coordinates (none given now or before)");
+    return EHIP_EARG;
+  }
   if ((physics == EHIP_PHYS_ELASTIC) != (s->dofs == 3) || (physics != EHIP_PHYS_ELASTIC && s->dofs != 1)) {
     set_error("ehip_assemble: physics %d does not match %d dofs per node", physics, s->dofs);
     return EHIP_EARG;
@@ -569,72 +856,66 @@ extern "C" int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdi
     set_error("ehip_assemble: element type %d does not match the mesh (%d nodes)", elem_code, s->npe);
     return EHIP_EUNSUP;
   }
-  const int ne = s->nelem;
   int rc;
+  if ((rc = assembly_check(s))) return rc;  // an earlier asynchronous call's verdict
+  const int ne = s->nelem;
   DevTables tab;
-  if ((rc = upload_tables(s, t, tab))) return rc;
-  // inputs to the device
-  std::vector<void *> tmp;
-  auto up = [&](const double *h, size_t cnt) -> double * {
+  if ((rc = ensure_tables(s, elem_code, t, tab))) return rc;
+  // inputs: device buffers sized once per system, refilled on the stream
+  if (!s->asm_xyz && (rc = dmalloc(s, (void **)&s->asm_xyz, sizeof(double) * (size_t)s->nnodes * 3))) return rc;
+  if (!s->asm_in && (rc = dmalloc(s, (void **)&s->asm_in, sizeof(double) * (size_t)ne * 6))) return rc;
+  if (!s->asm_bad && (rc = dmalloc(s, (void **)&s->asm_bad, sizeof(int)))) return rc;
+  if (coords)
+    EHIP_HIP_TRY(hipMemcpyAsync(s->asm_xyz, coords, sizeof(double) * (size_t)s->nnodes * 3, hipMemcpyHostToDevice,
+                                s->stream));
+  auto up = [&](const double *h, double *d, size_t cnt) -> const double * {
     if (!h) return nullptr;
-    double *d = nullptr;
-    if (dmalloc(nullptr, (void **)&d, sizeof(double) * cnt)) return nullptr;
     (void)hipMemcpyAsync(d, h, sizeof(double) * cnt, hipMemcpyHostToDevice, s->stream);
-    tmp.push_back(d);
     return d;
   };
   ElemArgs a{};
   a.nelem = ne;
   a.el = s->elements;
-  a.xyz = up(coords, (size_t)s->nnodes * 3);
+  a.xyz = s->asm_xyz;
   a.basis = tab.basis;
   a.dlb = tab.dlb;
   a.gs = tab.gs;
-  a.coef = up(elem_coef, ne);
-  a.conv = up(elem_conv, ne);
-  a.velo = up(elem_velo, (size_t)ne * 3);
-  a.load = up(elem_load, ne);
+  a.coef = up(elem_coef, s->asm_in, ne);
+  a.conv = up(elem_conv, s->asm_in + ne, ne);
+  a.load = up(elem_load, s->asm_in + 2 * (size_t)ne, ne);
+  a.velo = up(elem_velo, s->asm_in + 3 * (size_t)ne, (size_t)ne * 3);
   a.poisson = poisson;
   a.emat = s->emat;
   a.evec = s->evec;
-  int *bad = nullptr;
-  if ((rc = dmalloc(nullptr, (void **)&bad, sizeof(int)))) return rc;
-  EHIP_HIP_TRY(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
-  a.bad = bad;
-  hipEvent_t e0, e1;
-  EHIP_HIP_TRY(hipEventCreate(&e0));
-  EHIP_HIP_TRY(hipEventCreate(&e1));
-  EHIP_HIP_TRY(hipEventRecord(e0, s->stream));
-  rc = dispatch_element(s, physics, elem_code, cdim, a);
-  if (!rc) {
-    const int g = 2048;
-    hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, s->stream, s->nnz, s->slot_start, s->slot_contrib, s->emat,
-                       s->vals);
-    touch_vals(s, true, false);
-    hipLaunchKernelGGL(k_gather, dim3(g), dim3(256), 0, s->stream, (int64_t)s->n, s->rhs_start, s->rhs_contrib,
-                       s->evec, s->rhs);
+  a.bad = s->asm_bad;
+  EHIP_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s->stream, s->asm_bad);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (t_ms) {
+    EHIP_HIP_TRY(hipEventCreate(&e0));
+    EHIP_HIP_TRY(hipEventCreate(&e1));
+    EHIP_HIP_TRY(hipEventRecord(e0, s->stream));
   }
-  EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
-  EHIP_HIP_TRY(hipEventSynchronize(e1));
-  float ms = 0;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  if (t_ms) *t_ms = ms;
-  int hbad = 0;
-  EHIP_HIP_TRY(hipMemcpy(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost));
-  for (void *p : tmp) dfree(nullptr, p, 0);
-  dfree(nullptr, bad, 0);
-  free_tables(tab);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  bool packed = false;
+  rc = dispatch_element(s, physics, elem_code, cdim, a, t, &packed);
+  if (!rc) rc = launch_row_gather(s, s->npe * s->dofs, packed);
+  if (!rc) touch_vals(s, true, false);
+  if (t_ms) {
+    EHIP_HIP_TRY(hipEventRecord(e1, s->stream));
+    EHIP_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *t_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
   EHIP_HIP_TRY(hipGetLastError());
   if (rc) return rc;
-  if (hbad) {
-    set_error("ehip_assemble: %d degenerate Gauss-point metrics (ElementMetric)", hbad);
-    return EHIP_EARG;
-  }
-  (void)ld;
   s->has_values = true;
-  return EHIP_OK;
+  s->asm_pending = 1;
+  // with t_ms the call waits, and reports degenerate elements now; without,
+  // the next call that waits on the system reports them
+  return t_ms ? assembly_check(s) : EHIP_OK;
 }
 
 // StressBoundary traction part (Stress.src:836-1000), evaluated on the host

@@ -1328,6 +1328,7 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
     set_error("batch: system has no values");
     return EHIP_ESTATE;
   }
+  if (int rc0 = assembly_check(s)) return rc0;
   auto &sl = b->slot[slot];
   const int n = s->n;
   int rc;

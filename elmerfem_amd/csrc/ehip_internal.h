@@ -156,16 +156,29 @@ struct ehip_system {
   ehip::KState32 *h_st32 = nullptr;
   double *hist = nullptr;        // device residual history (hist_cap)
   int hist_cap = 0;
-  // element-to-slot gather maps (from ehip_system_from_mesh)
+  // row-owner gather maps (from ehip_system_from_mesh)
   int nelem = 0, npe = 0;
   int *elements = nullptr;        // nelem*npe, 0-based node ids (device)
-  int64_t ncontrib = 0;
-  int *slot_start = nullptr;      // nnz+1
-  int *slot_contrib = nullptr;    // ncontrib: e*ld*ld + local entry
-  int *rhs_start = nullptr;       // n+1
-  int *rhs_contrib = nullptr;     // nelem*ld: e*ld + local row
-  double *emat = nullptr;         // nelem*ld*ld element matrices (row-major)
+  int *rhs_start = nullptr;       // n+1: each row's entries in rhs_contrib
+  int *rhs_contrib = nullptr;     // nelem*ld: e*ld + the row's local index, element order
+  void *row_off = nullptr;        // per entry, ld column positions in the row (uint8 or uint16)
+  int row_off_bytes = 0;          // 1 or 2
+  double *emat = nullptr;         // nelem*ld*ld element matrices (row-major, or packed upper triangles)
   double *evec = nullptr;         // nelem*ld element force vectors
+  // assembly inputs kept on the device between calls (no allocation, free or
+  // table upload per call once the first call has sized them)
+  double *asm_xyz = nullptr;      // nnodes*3, valid after the first call with coordinates
+  double *asm_in = nullptr;       // coef, conv, load (nelem each), velo (3*nelem)
+  double *asm_tab = nullptr;      // basis, dLBasis, weights of element type asm_code
+  int asm_code = 0;
+  int *asm_bad = nullptr;         // degenerate Gauss-point metrics of the last call
+  int asm_pending = 0;            // that count is not checked yet (asynchronous call)
+  // Dirichlet rows and values of the last ehip_dirichlet call (device flags
+  // and values per row; host copies to recognise the same call again)
+  signed char *bc_flag = nullptr;
+  double *bc_val = nullptr;
+  std::vector<int> h_bc_rows;
+  std::vector<double> h_bc_vals;
   int64_t bytes = 0;
   // sampled SpMV timing inside the Krylov loop
   int prof_every = 0, prof_count = 0, prof_n = 0;
@@ -245,6 +258,10 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
 int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale);
 
 int max_grid();
+
+// Reports the degenerate Gauss-point metrics of an ehip_assemble call made
+// without waiting (EHIP_EARG), once; waits for the system's stream.
+int assembly_check(ehip_system *s);
 
 // ILU(k) (ilu.hip): pattern and level schedule of order k (kept while the
 // order stays), factorization of the current Values, and the LU solve

@@ -200,6 +200,10 @@ struct DirichletValues {
   }
 };
 
+__global__ void k_fill1(int n, double *d) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) d[i] = 1.0;
+}
+
 // rhs: b*D, Dirichlet b = val/D, symmetric elimination b -= a_ij val_j/D_j,
 // then b/D.  `vals` are still the unscaled, pre-Dirichlet values here.
 __global__ void k_dirichlet_rhs(int n, const int *__restrict__ rows, const int *__restrict__ cols,
@@ -228,45 +232,53 @@ extern "C" int ehip_dirichlet(ehip_system *s, int nbc, const int *bc_rows, const
                               int scaling) {
   if (!s || nbc < 0 || (nbc > 0 && !bc_rows)) return EHIP_EARG;
   const int n = s->n;
-  std::vector<signed char> flag(n, 0);
-  std::vector<double> val(n, 0.0);
-  for (int k = 0; k < nbc; ++k) {
-    int r = bc_rows[k];
-    if (r < 1 || r > n) {
-      set_error("ehip_dirichlet: row %d out of range", r);
-      return EHIP_EARG;
-    }
-    flag[r - 1] = 1;
-    val[r - 1] = bc_vals ? bc_vals[k] : 0.0;
-  }
   int rc;
-  if ((rc = ensure_work(s, 2))) return rc;
-  signed char *dflag = nullptr;
-  if ((rc = dmalloc(s, (void **)&dflag, n))) return rc;
-  double *dval = s->work;  // reuse workspace for the values
+  if ((rc = assembly_check(s))) return rc;
+  // the rows and values go to the device once; a call with the same ones
+  // (re-assembly with new coefficients on the same mesh) only enqueues kernels
+  const bool same = s->bc_flag && (int)s->h_bc_rows.size() == nbc &&
+                    std::equal(bc_rows, bc_rows + nbc, s->h_bc_rows.begin()) &&
+                    (bc_vals ? (s->h_bc_vals.size() == (size_t)nbc &&
+                                std::equal(bc_vals, bc_vals + nbc, s->h_bc_vals.begin()))
+                             : s->h_bc_vals.empty());
+  if (!same) {
+    std::vector<signed char> flag(n, 0);
+    std::vector<double> val(n, 0.0);
+    for (int k = 0; k < nbc; ++k) {
+      int r = bc_rows[k];
+      if (r < 1 || r > n) {
+        set_error("ehip_dirichlet: row %d out of range", r);
+        return EHIP_EARG;
+      }
+      flag[r - 1] = 1;
+      val[r - 1] = bc_vals ? bc_vals[k] : 0.0;
+    }
+    if (!s->bc_flag && (rc = dmalloc(s, (void **)&s->bc_flag, n))) return rc;
+    if (!s->bc_val && (rc = dmalloc(s, (void **)&s->bc_val, sizeof(double) * n))) return rc;
+    EHIP_HIP_TRY(hipMemcpyAsync(s->bc_flag, flag.data(), n, hipMemcpyHostToDevice, s->stream));
+    EHIP_HIP_TRY(hipMemcpyAsync(s->bc_val, val.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
+    EHIP_HIP_TRY(hipStreamSynchronize(s->stream));  // the host vectors go out of scope
+    s->h_bc_rows.assign(bc_rows, bc_rows + nbc);
+    if (bc_vals) s->h_bc_vals.assign(bc_vals, bc_vals + nbc);
+    else s->h_bc_vals.clear();
+  }
+  if ((rc = ensure_state(s))) return rc;
   double *D = s->dscale;
-  EHIP_HIP_TRY(hipMemcpyAsync(dflag, flag.data(), n, hipMemcpyHostToDevice, s->stream));
-  EHIP_HIP_TRY(hipMemcpyAsync(dval, val.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
   if (scaling) {
     hipLaunchKernelGGL(k_diag_scale, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rows, s->diag,
                        s->vals, D);
   } else {
-    std::vector<double> one(n, 1.0);
-    EHIP_HIP_TRY(hipMemcpyAsync(D, one.data(), sizeof(double) * n, hipMemcpyHostToDevice, s->stream));
-    EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+    hipLaunchKernelGGL(k_fill1, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, D);
   }
   hipLaunchKernelGGL(k_dirichlet_rhs, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->rows, s->cols,
-                     s->vals, s->rhs, D, dflag, dval, symmetric);
-  DirichletValues f{s->vals, s->cols, D, dflag, symmetric};
+                     s->vals, s->rhs, D, s->bc_flag, s->bc_val, symmetric);
+  DirichletValues f{s->vals, s->cols, D, s->bc_flag, symmetric};
   hipLaunchKernelGGL(k_nonzeros<DirichletValues>, dim3(s->plan64.grid), dim3(256), 0, s->stream, s->rows,
                      s->plan64.chunk_row, s->plan64.nchunks, f);
   touch_vals(s, true, false);
   EHIP_HIP_TRY(hipGetLastError());
-  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
-  dfree(s, dflag, n);
   return EHIP_OK;
 }
-
 
 namespace ehip {
 
@@ -2174,6 +2186,7 @@ struct SolveClock {
 
 int solve_impl(ehip_system *s, const ehip_params *p, double *x, ehip_report *r, double *hist, bool events) {
   if (!s || !p) return EHIP_EARG;
+  if (int rc0 = assembly_check(s)) return rc0;
   if (!s->has_values) {
     set_error("ehip_solve: the system has no values (assemble or set them first)");
     return EHIP_ESTATE;

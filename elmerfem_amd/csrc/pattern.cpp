@@ -251,19 +251,32 @@ int build_host_pattern(int nnodes, int nelem, int npe, const int *el, int nbnd, 
   return 0;
 }
 
-int build_slot_maps(const HostPattern &P, int nelem, int npe, const int *el, std::vector<int> &slot_start,
-                    std::vector<int> &slot_contrib, std::vector<int> &rhs_start, std::vector<int> &rhs_contrib) {
+int build_row_maps(const HostPattern &P, int nelem, int npe, const int *el, std::vector<int> &row_start,
+                   std::vector<int> &row_contrib, std::vector<uint16_t> &row_off, int *max_len) {
   const int dofs = P.dofs, n = P.n, nnodes = P.nnodes;
   const int ld = npe * dofs;
   if ((int64_t)nelem * ld * ld >= INT32_MAX) return -3;
   Incidence I = incidence(nnodes, nelem, npe, el);
-  // count per slot: every element touching row node a adds one contribution
-  // to the slots of its npe*dofs columns
-  slot_start.assign((size_t)P.nnz + 1, 0);
-  rhs_start.assign((size_t)n + 1, 0);
   std::vector<int> inv(nnodes + 1);
   for (int i = 0; i < nnodes; ++i) inv[P.perm[i]] = i + 1;
-  // pass 1: counts
+  // one entry per (row, touching element): counts, then offsets
+  row_start.assign((size_t)n + 1, 0);
+  int lmax = 0;
+  for (int j = 1; j <= nnodes; ++j) {
+    const int a = inv[j];
+    for (int l = 0; l < dofs; ++l) {
+      const int r = dofs * (j - 1) + l;
+      row_start[r + 1] = (int)(I.start[a + 1] - I.start[a]);
+      lmax = std::max(lmax, P.rows[r + 1] - P.rows[r]);
+    }
+  }
+  for (int r = 0; r < n; ++r) row_start[r + 1] += row_start[r];
+  *max_len = lmax;
+  if (lmax > 65536) return -3;
+  row_contrib.resize((size_t)row_start[n]);
+  row_off.resize((size_t)row_start[n] * ld);
+  // fill: the incidence lists are ascending, so every row's elements -- and
+  // with them every slot's contributions -- come in element order
 #pragma omp parallel for schedule(dynamic, 1024)
   for (int j = 1; j <= nnodes; ++j) {
     const int a = inv[j];
@@ -272,53 +285,18 @@ int build_slot_maps(const HostPattern &P, int nelem, int npe, const int *el, std
       const int64_t b0 = P.rows[r] - 1, b1 = P.rows[r + 1] - 1;
       const int *cb = P.cols.data() + b0;
       const int len = (int)(b1 - b0);
-      rhs_start[r + 1] = (int)(I.start[a + 1] - I.start[a]);
-      for (int64_t k = I.start[a]; k < I.start[a + 1]; ++k) {
-        const int *nd = el + (int64_t)I.elem[k] * npe;
+      int64_t k = row_start[r];
+      for (int64_t q = I.start[a]; q < I.start[a + 1]; ++q, ++k) {
+        const int e = I.elem[q];
+        const int li = I.local[q] * dofs + l;
+        const int *nd = el + (int64_t)e * npe;
+        row_contrib[k] = e * ld + li;
         for (int bl = 0; bl < npe; ++bl) {
           const int kb = P.perm[nd[bl] - 1];
           for (int m = 1; m <= dofs; ++m) {
             const int col = dofs * (kb - 1) + m;
             const int pos = (int)(std::lower_bound(cb, cb + len, col) - cb);
-            // counts accumulate per slot; slots of one row are private to this thread
-            slot_start[b0 + pos + 1]++;
-          }
-        }
-      }
-    }
-  }
-  for (int64_t s = 0; s < P.nnz; ++s) slot_start[s + 1] += slot_start[s];
-  for (int r = 0; r < n; ++r) rhs_start[r + 1] += rhs_start[r];
-  slot_contrib.resize((size_t)slot_start[P.nnz]);
-  rhs_contrib.resize((size_t)rhs_start[n]);
-  // pass 2: fill in ascending element order (incidence lists are ascending)
-#pragma omp parallel
-  {
-    std::vector<int> fillpos;
-#pragma omp for schedule(dynamic, 1024)
-    for (int j = 1; j <= nnodes; ++j) {
-      const int a = inv[j];
-      for (int l = 0; l < dofs; ++l) {
-        const int r = dofs * (j - 1) + l;
-        const int64_t b0 = P.rows[r] - 1, b1 = P.rows[r + 1] - 1;
-        const int *cb = P.cols.data() + b0;
-        const int len = (int)(b1 - b0);
-        fillpos.assign(len, 0);
-        int rp = rhs_start[r];
-        for (int64_t k = I.start[a]; k < I.start[a + 1]; ++k) {
-          const int e = I.elem[k];
-          const int la = I.local[k];
-          const int li = la * dofs + l;
-          const int *nd = el + (int64_t)e * npe;
-          rhs_contrib[rp++] = e * ld + li;
-          for (int bl = 0; bl < npe; ++bl) {
-            const int kb = P.perm[nd[bl] - 1];
-            for (int m = 1; m <= dofs; ++m) {
-              const int col = dofs * (kb - 1) + m;
-              const int pos = (int)(std::lower_bound(cb, cb + len, col) - cb);
-              const int lj = bl * dofs + (m - 1);
-              slot_contrib[slot_start[b0 + pos] + fillpos[pos]++] = e * ld * ld + li * ld + lj;
-            }
+            row_off[(size_t)k * ld + bl * dofs + (m - 1)] = (uint16_t)pos;
           }
         }
       }

@@ -20,12 +20,13 @@ struct HostPattern {
 int build_host_pattern(int nnodes, int nelem, int npe, const int *el, int nbnd, int npe_b, const int *bel,
                        int dofs, int optimize, HostPattern &out);
 
-// Gather maps for deterministic assembly.  For every CRS slot the list of
-// (element, local entry) contributions in ascending element order -- the
-// order in which CRS_GlueLocalMatrix adds them (CRSMatrix.src:507-590) --
-// encoded as e*ld*ld + (i*ld + j) with ld = npe*dofs and the local matrix
-// stored row-major.  Likewise for RHS rows: e*ld + i.
-int build_slot_maps(const HostPattern &P, int nelem, int npe, const int *el, std::vector<int> &slot_start,
-                    std::vector<int> &slot_contrib, std::vector<int> &rhs_start, std::vector<int> &rhs_contrib);
+// Row-owner gather maps for deterministic assembly.  For every matrix row
+// the elements touching it in ascending element order -- the order in which
+// CRS_GlueLocalMatrix adds their blocks (CRSMatrix.src:507-590) -- each as
+// e*ld + i (i the row's local index in the element, ld = npe*dofs), and for
+// every such entry the position within the row of each of the element's ld
+// columns (row_off[k*ld + j]).  max_len: the longest row.
+int build_row_maps(const HostPattern &P, int nelem, int npe, const int *el, std::vector<int> &row_start,
+                   std::vector<int> &row_contrib, std::vector<uint16_t> &row_off, int *max_len);
 
 }  // namespace ehip

@@ -758,7 +758,8 @@ int ehip_system_free(ehip_system *s) {
   void *ptrs[] = {s->rows,       s->cols,        s->diag,      s->vals,       s->rhs,
                   s->x,          s->vals32,      s->dscale,    s->work,       s->work32,
                   s->partials,   s->st,          s->elements,  s->st32,       s->partials32,
-                  s->hist,       s->slot_start,  s->slot_contrib, s->rhs_start, s->rhs_contrib,
+                  s->hist,       s->row_off,     s->asm_xyz,   s->rhs_start,  s->rhs_contrib,
+                  s->asm_in,     s->asm_tab,     s->asm_bad,   s->bc_flag,    s->bc_val,
                   s->emat,       s->evec,        s->plan64.chunk_row, s->plan32.chunk_row,
                   s->cols16,     s->seg_base,    s->sell_off,  s->sell_c32,
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
@@ -808,6 +809,10 @@ int ehip_system_set_values(ehip_system *s, const double *values, const double *r
 }
 
 int ehip_system_get_values(ehip_system *s, double *values, double *rhs) {
+  if (s) {
+    int rc = assembly_check(s);
+    if (rc) return rc;
+  }
   if (!s) return EHIP_EARG;
   EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
   if (values) EHIP_HIP_TRY(hipMemcpy(values, s->vals, sizeof(double) * s->nnz, hipMemcpyDeviceToHost));

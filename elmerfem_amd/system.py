@@ -115,13 +115,17 @@ class System:
 
     # ------------------------------------------------------------ assembly
     def assemble(self, physics, coords, elem_coef=None, elem_conv=None, elem_velo=None, elem_load=None,
-                 poisson=0.3, elem_code=None, cdim=None) -> float:
-        """Element loop + DefaultUpdateEquations; returns device time in ms."""
+                 poisson=0.3, elem_code=None, cdim=None, wait=True) -> float:
+        """Element loop + DefaultUpdateEquations; returns device time in ms.
+        coords None: the coordinates of the previous call (kept on the
+        device).  wait False: the call only enqueues the work and returns 0;
+        a degenerate element is then reported by the next call that waits on
+        the system (solve, dirichlet, values)."""
         mesh = getattr(self, "mesh", None)
         code = elem_code if elem_code is not None else mesh.elem_code
         cd = cdim if cdim is not None else (mesh.dim if mesh is not None else 3)
         ne = mesh.nelem
-        xyz = _f64(coords)
+        xyz = None if coords is None else _f64(coords)
         t = C.c_double()
         arr = lambda a, m: None if a is None else _f64(np.broadcast_to(a, m).copy() if np.ndim(a) == 0 or np.shape(a) != m else a)
         coef = arr(elem_coef, (ne,))
@@ -129,8 +133,9 @@ class System:
         velo = arr(elem_velo, (ne, 3))
         load = arr(elem_load, (ne,))
         _lib.check(_lib.load().ehip_assemble(self._h, physics, code, cd, _ptr(xyz), _ptr(coef), _ptr(conv), _ptr(velo),
-                                             _ptr(load), float(poisson), C.byref(t)), "ehip_assemble")
-        return t.value
+                                             _ptr(load), float(poisson), C.byref(t) if wait else None),
+                   "ehip_assemble")
+        return t.value if wait else 0.0
 
     def assemble_traction(self, bnd_code, coords, belems, load3):
         be = np.ascontiguousarray(belems, np.int32)

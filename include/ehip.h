@@ -182,11 +182,18 @@ int ehip_result_write(const char *path, const char *var, const char *equation, i
 #define EHIP_PHYS_MODELPDE 0   /* ModelPDE: diffusion + convection + source (ModelPDE.src:75-166) */
 #define EHIP_PHYS_ELASTIC 1    /* StressSolve isotropic elasticity (Stress.src:59-521) */
 /* Bulk element loop + DefaultUpdateEquations glue, in element order, as a
- * deterministic gather over the element-to-slot map (no float atomics).
- * coords: nnodes x 3 (row-major).  Per-element arrays (length nelem, or
+ * deterministic row-owner gather (no float atomics): every slot receives
+ * its element contributions in element order, from zero.
+ * coords: nnodes x 3 (row-major), or NULL for the coordinates of the
+ * previous call (kept on the device).  Per-element arrays (length nelem, or
  * 3*nelem for velo) may be NULL (= 0).  elem_code: 303 (cdim 2), 404
  * (cdim 2), 504, 510, 808, 827 (cdim 3); elasticity takes 808, 827, 510.
- * Values and RHS are zeroed first (DefaultInitialize). */
+ * Values and RHS are overwritten (DefaultInitialize).  t_ms non-NULL: the
+ * call waits and returns the device time; NULL: the work is only enqueued
+ * on the system's stream (no allocation after the first call), and
+ * degenerate elements are reported (EHIP_EARG) by the next call that waits
+ * on the system (ehip_solve, ehip_dirichlet, ehip_system_get_values, a
+ * batch or server admission, or ehip_assemble again). */
 int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdim, const double *coords,
                   const double *elem_coef, const double *elem_conv, const double *elem_velo,
                   const double *elem_load, double poisson, double *t_ms);
@@ -196,7 +203,9 @@ int ehip_assemble_traction(ehip_system *s, int bnd_code, const double *coords, i
                            const int *belems, const double *load3);
 /* DefaultDirichletBCs for nodal Dirichlet rows (DefUtils.src:3530-4030):
  * scale, ZeroRow + unit diagonal + b = val/D (or CRS_SetSymmDirichlet when
- * symmetric != 0), back-scale.  bc_rows are 1-based matrix rows. */
+ * symmetric != 0), back-scale.  bc_rows are 1-based matrix rows.  The rows
+ * and values stay on the device: a call with the same ones as the last only
+ * enqueues the kernels and returns without waiting. */
 int ehip_dirichlet(ehip_system *s, int nbc, const int *bc_rows, const double *bc_vals,
                    int symmetric, int scaling);
 

@@ -407,6 +407,44 @@ def test_assembly_matches_oracle(ehip, wl):
     s.free()
 
 
+def test_reassembly_without_waiting(ehip):
+    """Parameter-sweep re-assembly: new coefficients on the kept coordinates,
+    enqueued without waiting, give the oracle's values and, with the same
+    Dirichlet rows, its solution; a degenerate element in such a call is
+    reported by the next call that waits, once."""
+    w = W.heat_tet4(5)
+    m = w["mesh"]
+    pat = O.Pattern(m.elements, m.nnodes, 1)
+    s = System.from_mesh(m, dofs=1, optimize_bandwidth=False)
+    s.assemble(0, m.coords, elem_coef=w["coef"], elem_load=w["load"])
+    rng = np.random.default_rng(5)
+    for k in range(3):
+        coef = np.exp(0.5 * rng.standard_normal(m.nelem))
+        vals, rhs = O.assemble(0, m, pat, coef, load=w["load"])
+        assert s.assemble(0, None, elem_coef=coef, elem_load=w["load"], wait=False) == 0.0
+        gv, gb = s.values()
+        assert row_rel_err(pat, gv, vals) <= 1e-12
+        assert np.abs(gb - rhs).max() <= 1e-12 * np.abs(rhs).max()
+        O.dirichlet(pat, vals, rhs, pat.perm[w["bc_nodes"] - 1])
+        s.assemble(0, None, elem_coef=coef, elem_load=w["load"], wait=False)
+        s.dirichlet_nodes(w["bc_nodes"])
+        x, rep = s.solve(w["kw"], precision=_lib.FP64, mode=_lib.FAITHFUL, true_residual_stop=True)
+        info, it, xo, hist, nrm = O.solve_linear_system(pat, vals.copy(), rhs, method=0, tol=1e-10, maxit=20000)
+        assert rep["info"] == info == 1
+        assert np.abs(x - xo).max() <= 1e-8 * np.abs(xo).max()
+        ax = np.zeros(pat.n)
+        O.lib().or_spmv(pat.n, O._p(pat.rows), O._p(pat.cols), O._p(vals), O._p(x), O._p(ax))
+        assert np.linalg.norm(rhs - ax) / np.linalg.norm(rhs) <= 1e-10
+    bad = m.coords.copy()
+    bad[m.elements[0] - 1] = bad[m.elements[0][0] - 1]      # element 1 collapsed to a point
+    s.assemble(0, bad, elem_coef=w["coef"], elem_load=w["load"], wait=False)
+    with pytest.raises(_lib.EhipError, match="degenerate"):
+        s.values()
+    s.values()                                              # reported once
+    s.assemble(0, m.coords, elem_coef=w["coef"], elem_load=w["load"])
+    s.free()
+
+
 def test_traction_matches_oracle(ehip):
     w = W.elasticity_hex(3)
     m = w["mesh"]
