@@ -14,7 +14,12 @@ are timed beside it, as are GPU assembly and the SpMV kernels.
 
     python bench.py [--steps K] [--warmup W] [--n 128]
 
-Prints one JSON line.  value = DOFs x inner iterations / solve time.
+Prints one JSON line.  value = DOFs x inner iterations / solve time.  After
+the headline (outside its timed steps) short runs of the batching form (64
+convection-diffusion systems) and the serving form (16 meshes, 200
+requests) add `batch_summary` and `serve_summary` to the same line
+(`--no-summaries` leaves them out; `--form batch` / `--form serve` run
+those forms at full size).
 """
 from __future__ import annotations
 
@@ -128,6 +133,12 @@ def cpu_baseline(sys_, kw, iters: int, asm_n: int):
 
 
 def form_batch(args):
+    rec = run_batch(args)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+def run_batch(args):
     """Batching form: independent steady 2-D convection-diffusion problems on
     structured Quad4 grids, side N ~ U{32..512} elements, velocity of random
     direction with Peclet ~ U[1, 100], BiCGStab + Jacobi to 1e-10, all in one
@@ -206,50 +217,96 @@ def form_batch(args):
            "speedup_vs_fp64_fast_one_at_a_time": {n: out["fp64_fast_one_at_a_time"]["wall_s"] / out[n]["wall_s"]
                                                   for n in ("fp64_faithful_batched", "fp64_fast_batched",
                                                             "mixed_batched")}}
-    print(json.dumps(rec), flush=True)
     for s in systems:
         s.free()
-    return 0
+    return rec
 
 
 def form_serve(args):
-    """Serving form: Poisson arrivals of steady 3-D heat solves on jittered
-    Tet4 cubes (n^3 hexes x 6, n log-uniform in [10, 100]), lognormal(0, 0.5)
-    conductivity, mixed-precision refined CG to 1e-10 on the continuous-
+    rec = run_serve(args)
+    print(json.dumps(rec), flush=True)
+    return 0
+
+
+def run_serve(args):
+    """Serving form: Poisson arrivals of parameter-sweep requests -- steady
+    3-D heat on jittered Tet4 cubes (n^3 hexes x 6, n log-uniform in
+    [10, 100]) with a per-element lognormal(0, 0.5) conductivity field drawn
+    per request -- each assembled on the GPU from its coefficients (the mesh,
+    its pattern and coordinates resident in HBM), Dirichlet rows applied,
+    and solved by mixed-precision refined CG to 1e-10 on the continuous-
     batching server with HBM-footprint admission; latency is arrival to
-    report on the host.  Requests draw their (assembled, device-resident)
-    system from a pool built before the loop; one-at-a-time plain FP64 CG
-    over the same pool is the reference rate."""
+    report on the host.  Meshes come from a pool built before the loop
+    (`--pool`, two resident systems each so a mesh can serve two requests at
+    once); every request takes one of `--variants` coefficient fields of its
+    mesh, generated beforehand in pinned host memory.  The reference rate is
+    the same requests one at a time: assembly, Dirichlet rows and a plain
+    FP64 CG solve, in HUTI's loop order and in the one-SpMV order."""
     from elmerfem_amd import _lib, workloads as W
     from elmerfem_amd.batch import Server
     from elmerfem_amd.mesh import uniforms
+    import torch
     _lib.init(0)
-    P = args.pool
+    P, V, REP = args.pool, max(1, args.variants), 2
     u, _ = uniforms(args.seed, np.arange(1, P + 1, dtype=np.uint64), stream=13)
     sizes = np.clip(np.floor(10.0 ** (1.0 + u)).astype(int), 10, 100).tolist()
     t0 = time.perf_counter()
-    pool = []
+    meshes, sysrep, coefs, bcs = [], [], [], []
+    rng = np.random.default_rng(args.seed)
     for i, nn in enumerate(sizes):
-        pool.append(W.build_system(W.heat_tet4(nn, seed=args.seed, index=i + 1), optimize_bandwidth=True)[0])
+        w = W.heat_tet4(nn, seed=args.seed, index=i + 1)
+        m = w["mesh"]
+        reps_i = []
+        for _ in range(REP):
+            sy = W.build_system(w, optimize_bandwidth=True)[0]
+            reps_i.append(sy)
+        # coefficient fields in pinned host memory: their uploads are DMA copies
+        # that do not hold the request loop
+        vs = []
+        for v in range(V):
+            buf = torch.empty(m.nelem, dtype=torch.float64, pin_memory=True).numpy()
+            buf[:] = np.exp(0.5 * rng.standard_normal(m.nelem))
+            vs.append(buf)
+        load = torch.empty(m.nelem, dtype=torch.float64, pin_memory=True).numpy()
+        load[:] = w["load"]
+        meshes.append((w, load))
+        sysrep.append(reps_i)
+        coefs.append(vs)
+        bcs.append(w["bc_nodes"])
         if (i + 1) % 16 == 0:
-            log("built %d/%d pool systems (%.1fs)" % (i + 1, P, time.perf_counter() - t0))
+            log("built %d/%d pool meshes (%.1fs)" % (i + 1, P, time.perf_counter() - t0))
     kw = W._kw("cg", "diagonal", 1e-10, 20000)
+
+    def prepare(sy, mi, v, wait):
+        """Assembles request (mesh mi, field v) into system sy and applies its Dirichlet rows."""
+        w, load = meshes[mi]
+        sy.assemble(w["physics"], None, elem_coef=coefs[mi][v], elem_load=load, wait=wait)
+        sy.dirichlet_nodes(bcs[mi])
+
     # every pool system is resident and warm in both precisions before any
     # timing (first-use work -- FP32 and SELL value copies, workspace, the
     # recorded iteration graphs -- happens once per system, not per request)
-    for s in pool:
-        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False, true_residual_stop=True)
-        s.solve(kw, precision=_lib.MIXED_IR, mode=_lib.FAST, return_x=False, true_residual_stop=True)
+    for mi in range(P):
+        for sy in sysrep[mi]:
+            for prec, mode in ((_lib.FP64, _lib.FAITHFUL), (_lib.FP64, _lib.FAST), (_lib.MIXED_IR, _lib.FAST)):
+                prepare(sy, mi, 0, True)
+                sy.solve(kw, precision=prec, mode=mode, return_x=False, true_residual_stop=True)
     log("pool warm (%.1fs)" % (time.perf_counter() - t0))
-    # one at a time, plain FP64 CG
-    t = time.perf_counter()
-    seq_lat = []
-    for s in pool:
-        t1 = time.perf_counter()
-        s.solve(kw, precision=_lib.FP64, mode=_lib.FAITHFUL, return_x=False, true_residual_stop=True)
-        seq_lat.append(time.perf_counter() - t1)
-    seq_rate = P / (time.perf_counter() - t)
-    log("one-at-a-time FP64: %.1f solves/s" % seq_rate)
+    # one at a time: assembly + Dirichlet + plain FP64 CG, over one request per mesh
+    seq = {}
+    for name, mode in (("fp64_faithful", _lib.FAITHFUL), ("fp64_fast", _lib.FAST)):
+        lat_s = []
+        t = time.perf_counter()
+        for mi in range(P):
+            t1 = time.perf_counter()
+            sy = sysrep[mi][0]
+            prepare(sy, mi, mi % V, True)
+            sy.solve(kw, precision=_lib.FP64, mode=mode, return_x=False, true_residual_stop=True)
+            lat_s.append(time.perf_counter() - t1)
+        seq[name] = {"solves_per_s": P / (time.perf_counter() - t),
+                     "latency_ms_p50": 1000.0 * float(np.percentile(lat_s, 50)),
+                     "latency_ms_p99": 1000.0 * float(np.percentile(lat_s, 99))}
+        log("one-at-a-time %s: %.1f solves/s" % (name, seq[name]["solves_per_s"]))
     srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)),
                  true_residual_stop=True)
     if args.lane_workers > 0:
@@ -259,7 +316,9 @@ def form_serve(args):
 
     def serve(arrivals):
         """Runs the request stream; returns (elapsed s, latencies s, infos, res_true, max live)."""
-        busy, queue = {}, []
+        busy, queue, prepared = {}, [], {}
+        free = [list(r) for r in sysrep]
+        host = {"prepare_s": 0.0, "submit_s": 0.0, "step_s": 0.0}
         lat = np.zeros(R)
         infos = np.zeros(R, dtype=int)
         res_true = np.zeros(R)
@@ -275,12 +334,27 @@ def form_serve(args):
             # of one path is refused, later ones of that path are not tried
             still, full = [], set()
             for req in queue:
-                s = pool[req % P]
-                path = args.lane_workers > 0 and s.n >= args.lane_dofs
-                if path in full or s._h.value in busy:
+                mi = req % P
+                sy0 = sysrep[mi][0]
+                path = args.lane_workers > 0 and sy0.n >= args.lane_dofs
+                if path in full or (req not in prepared and not free[mi]):
                     still.append(req)
-                elif srv.submit(s):
-                    busy[s._h.value] = req
+                    continue
+                if req not in prepared:
+                    # assembled once, when it first gets a system; a refused
+                    # submission keeps that system for the retry
+                    sy = free[mi].pop()
+                    tp = time.perf_counter()
+                    prepare(sy, mi, (req // P) % V, False)
+                    host["prepare_s"] += time.perf_counter() - tp
+                    prepared[req] = sy
+                sy = prepared[req]
+                tp = time.perf_counter()
+                ok = srv.submit(sy)
+                host["submit_s"] += time.perf_counter() - tp
+                if ok:
+                    del prepared[req]
+                    busy[sy._h.value] = (req, mi, sy)
                 else:
                     full.add(path)
                     still.append(req)
@@ -291,55 +365,63 @@ def form_serve(args):
                 if nxt < R:
                     time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.005)))
                 continue
-            for s, rep in srv.step():
-                req = busy.pop(s._h.value)
+            tp = time.perf_counter()
+            finished = srv.step()
+            host["step_s"] += time.perf_counter() - tp
+            for sy, rep in finished:
+                req, mi, sy_ = busy.pop(sy._h.value)
+                free[mi].append(sy_)
                 lat[req] = time.perf_counter() - t0 - arrivals[req]
                 infos[req] = rep["info"]
                 res_true[req] = rep["res_true"]
                 if rep["info"] != 1 and len(failures) < 20:
-                    failures.append({"request": int(req), "system": int(req % P), "dofs": int(s.n),
+                    failures.append({"request": int(req), "mesh": int(mi), "dofs": int(sy.n),
                                      "info": int(rep["info"]), "iters": int(rep["iters"]),
                                      "outer": int(rep["outer_iters"]), "res_true": float(rep["res_true"]),
-                                     "lane": bool(args.lane_workers > 0 and s.n >= args.lane_dofs)})
+                                     "lane": bool(args.lane_workers > 0 and sy.n >= args.lane_dofs)})
                 done += 1
+        host_times.append(host)
         return time.perf_counter() - t0, lat, infos, res_true, max_live
 
     # capacity: every request waiting at t = 0
+    host_times = []
     el, lat_sat, infos, rt, ml = serve(np.zeros(R))
     capacity = R / el
-    log("server capacity %.1f solves/s (%.1fx one-at-a-time FP64)" % (capacity, capacity / seq_rate))
+    ref = seq["fp64_faithful"]["solves_per_s"]
+    log("server capacity %.1f solves/s (%.1fx one-at-a-time FP64 HUTI order, %.1fx one-SpMV order)" % (
+        capacity, capacity / ref, capacity / seq["fp64_fast"]["solves_per_s"]))
     # latency under load: Poisson arrivals at a fixed fraction of that capacity
     rate = args.rate if args.rate > 0 else args.load * capacity
-    rng = np.random.default_rng(args.seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    arrivals = np.cumsum(np.random.default_rng(args.seed).exponential(1.0 / rate, R))
     el2, lat, infos2, rt2, max_live = serve(arrivals)
     srv.free()
     log("at %.1f arrivals/s: p50 %.2f ms, p99 %.2f ms" % (rate, 1000 * np.percentile(lat, 50),
                                                           1000 * np.percentile(lat, 99)))
     rec = {"metric": "solves/s (capacity) and p50/p99 latency at a fixed arrival rate, continuous-batching "
-                     "mixed-precision CG to 1e-10",
+                     "mixed-precision CG to 1e-10 with per-request GPU assembly",
            "value": capacity, "unit": "solves/s", "n_gpus": 1, "steps": R, "warmup": 0,
            "ms_per_step": 1000.0 * el / R, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 inner / f64 refinement", "data": "synthetic",
-           "config": {"workload": "heat_tet4_serving", "requests": R, "pool_systems": P,
+           "config": {"workload": "heat_tet4_serving_sweep", "requests": R, "pool_meshes": P,
+                      "systems_per_mesh": REP, "coefficient_fields_per_mesh": V,
                       "n_min": int(min(sizes)), "n_max": int(max(sizes)),
-                      "dofs_mean": float(np.mean([s.n for s in pool])), "slots": args.slots,
+                      "dofs_mean": float(np.mean([r[0].n for r in sysrep])), "slots": args.slots,
                       "lane_min_dofs": args.lane_dofs, "lane_workers": args.lane_workers,
                       "mem_budget_gib": args.budget_gib, "seed": args.seed, "tol": 1e-10},
            "arrival_rate_per_s": rate, "served_rate_under_load": R / el2,
            "latency_ms": {"p50": 1000.0 * float(np.percentile(lat, 50)),
                           "p99": 1000.0 * float(np.percentile(lat, 99)), "max": 1000.0 * float(lat.max())},
-           "one_at_a_time_fp64": {"solves_per_s": seq_rate, "method": "FP64 CG in HUTI order",
-                                  "latency_ms_p50": 1000.0 * float(np.percentile(seq_lat, 50)),
-                                  "latency_ms_p99": 1000.0 * float(np.percentile(seq_lat, 99))},
-           "speedup_solves_per_s": capacity / seq_rate,
+           "one_at_a_time": {**seq, "request": "GPU assembly + Dirichlet rows + FP64 CG solve"},
+           "speedup_solves_per_s": capacity / ref,
+           "speedup_vs_fp64_fast_one_at_a_time": capacity / seq["fp64_fast"]["solves_per_s"],
            "converged": int((infos == 1).sum() + (infos2 == 1).sum()), "requests_total": 2 * R,
            "failures": failures,
-           "res_true_max": float(max(rt.max(), rt2.max())), "max_live": max_live}
-    print(json.dumps(rec), flush=True)
-    for s in pool:
-        s.free()
-    return 0
+           "res_true_max": float(max(rt.max(), rt2.max())), "max_live": max_live,
+           "host_loop_s": {"capacity_run": host_times[0], "load_run": host_times[1]}}
+    for r in sysrep:
+        for sy in r:
+            sy.free()
+    return rec
 
 
 def form_ilu(args):
@@ -476,11 +558,12 @@ def main():
                     help="mixed CG: residual drop between FP64 replacements (0: the library default)")
     ap.add_argument("--systems", type=int, default=512, help="batch form: number of systems")
     ap.add_argument("--no-tune", action="store_true", help="batch form: no schedule search (default table)")
-    ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request systems")
+    ap.add_argument("--pool", type=int, default=64, help="serve form: distinct request meshes")
+    ap.add_argument("--variants", type=int, default=4, help="serve form: coefficient fields per mesh")
     ap.add_argument("--requests", type=int, default=1000, help="serve form: requests")
     ap.add_argument("--rate", type=float, default=0.0, help="serve form: arrivals per second (0: load x capacity)")
     ap.add_argument("--load", type=float, default=0.7, help="serve form: arrival rate / measured capacity")
-    ap.add_argument("--lane-dofs", type=int, default=30000, help="serve form: unknowns from which a request "
+    ap.add_argument("--lane-dofs", type=int, default=100000, help="serve form: unknowns from which a request "
                                                                    "takes the concurrent lane")
     ap.add_argument("--lane-workers", type=int, default=5, help="serve form: concurrent-lane workers (0: off)")
     ap.add_argument("--slots", type=int, default=64, help="serve form: server slots")
@@ -495,6 +578,8 @@ def main():
     ap.add_argument("--cpu-asm-n", type=int, default=64, help="CPU baseline: assembly sample mesh (n^3)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-forms", action="store_true", help="time only the headline form")
+    ap.add_argument("--no-summaries", action="store_true",
+                    help="headline: no short batching and serving runs after it")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -678,8 +763,30 @@ def main():
                    "sigma": args.sigma, "seed": args.seed, "tol": 1e-10, "method": "cg", "precond": "diagonal"},
         "roofline": roofline, "cpu_baseline": cpu, "forms": forms, "spmv": spmv, "assembly": assembly,
     }
-    print(json.dumps(rec), flush=True)
     sys_.free()
+    if not args.skip_forms and not args.no_summaries:
+        # short runs of the batching and serving forms (their own commands,
+        # --form batch / --form serve, run them at full size)
+        import copy
+        a = copy.copy(args)
+        a.systems, a.pool, a.requests = 64, 16, 200
+        log("batch summary (%d systems) ..." % a.systems)
+        b = run_batch(a)
+        f = b["forms"]
+        rec["batch_summary"] = {
+            "config": b["config"], "metric": b["metric"], "value": b["value"], "unit": b["unit"],
+            "wall_s": {k: f[k]["wall_s"] for k in f}, "converged": {k: f[k]["converged"] for k in f},
+            "res_true_max": {k: f[k]["res_true_max"] for k in f if "res_true_max" in f[k]},
+            "speedup_vs_fp64_fast_one_at_a_time": b["speedup_vs_fp64_fast_one_at_a_time"]}
+        log("serve summary (%d meshes, %d requests) ..." % (a.pool, a.requests))
+        v = run_serve(a)
+        rec["serve_summary"] = {
+            "config": v["config"], "metric": v["metric"], "capacity_solves_per_s": v["value"],
+            "latency_ms": v["latency_ms"], "arrival_rate_per_s": v["arrival_rate_per_s"],
+            "one_at_a_time": v["one_at_a_time"], "speedup_vs_fp64_faithful": v["speedup_solves_per_s"],
+            "speedup_vs_fp64_fast": v["speedup_vs_fp64_fast_one_at_a_time"],
+            "converged": v["converged"], "requests_total": v["requests_total"], "res_true_max": v["res_true_max"]}
+    print(json.dumps(rec), flush=True)
     return 0
 
 

@@ -993,6 +993,9 @@ __global__ void k_fill_x(int n, double *x, double v) {
 
 __global__ void k_inc(int *c) { *c += 1; }
 
+// a slot's initial state, written in stream order behind any replay queued
+__global__ void k_set_slot(BSys *dst, BSys v) { *dst = v; }
+
 __global__ void k_to_f32(int64_t nnz, const double *v, float *v32) {
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x)
     v32[j] = (float)v[j];
@@ -1029,6 +1032,7 @@ struct ehip_batch {
   double *p64 = nullptr;
   float *p32 = nullptr;
   int *active = nullptr;
+  double *bnorm = nullptr;       // per slot, ||b|| of the right-hand-side scaling (device)
   int *h_active = nullptr;       // pinned
   BSys *h_st = nullptr;          // pinned mirror
   // host view
@@ -1041,6 +1045,8 @@ struct ehip_batch {
     double bscale = 1.0;
     int64_t bytes = 0;
     int nb = 2048;               // chunk size (nonzeros) of this system
+    int64_t first_launch = 0;    // the first replay whose results include this system
+    bool asm_check = false;      // its assembly's degenerate-element count is read at retirement
   };
   std::vector<Slot> slot;
   bool graph_ok = false;
@@ -1051,6 +1057,12 @@ struct ehip_batch {
   int fast = 0;                  // FP64: recurrence residual with true-residual restarts
   int nbmax = 2048;              // largest chunk size in the work list (selects bk_run's capacity)
   int grid = 2048;               // workgroups of the multi-system launches
+  // serving: admissions only enqueue (the worklist is rebuilt at the next
+  // step), and a step launches the next replay without waiting for it
+  hipEvent_t ev_admit = nullptr;
+  bool dirty = false;            // the worklist no longer matches the slots
+  bool in_flight = false;        // a replay (and its state copy) is queued
+  int64_t nlaunch = 0;           // replays launched
   int tuning = 0;                // ehip_batch_set_tuning: 0 off, 1 untuned classes, 2 everything
   int tune_replays = 3;
   int64_t bytes = 0;
@@ -1298,6 +1310,7 @@ static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
   EHIP_HIP_TRY(hipMalloc(&b->slots, sizeof(SlotPtrs) * capacity));
   EHIP_HIP_TRY(hipMalloc(&b->st, sizeof(BSys) * capacity));
   EHIP_HIP_TRY(hipMalloc(&b->active, sizeof(int)));
+  EHIP_HIP_TRY(hipMalloc(&b->bnorm, sizeof(double) * capacity));
   EHIP_HIP_TRY(hipHostMalloc((void **)&b->h_active, sizeof(int), hipHostMallocDefault));
   EHIP_HIP_TRY(hipHostMalloc((void **)&b->h_st, sizeof(BSys) * capacity, hipHostMallocDefault));
   std::vector<BSys> zero(capacity);
@@ -1328,8 +1341,17 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
     set_error("batch: system has no values");
     return EHIP_ESTATE;
   }
-  if (int rc0 = assembly_check(s)) return rc0;
   auto &sl = b->slot[slot];
+  // an assembly still running is not waited for: its degenerate-element
+  // count is read when the system retires
+  bool asm_later = false;
+  if (s->asm_pending) {
+    if (hipStreamQuery(s->stream) == hipSuccess) {
+      if (int rc0 = assembly_check(s)) return rc0;
+    } else {
+      asm_later = true;
+    }
+  }
   const int n = s->n;
   int rc;
   if ((rc = ensure_state(s))) return rc;
@@ -1357,7 +1379,7 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   // x = 0 initial guess -> 1e-8 after scaling (IterSolve.src:358)
   EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
   if (b->p.scaling) {
-    if ((rc = scale_system(s, s->rhs, true, s->x, &sl.bscale, b->mixed != 0))) return rc;
+    if ((rc = scale_system(s, s->rhs, true, s->x, &sl.bscale, b->mixed != 0, b->bnorm + slot))) return rc;
   } else if (b->mixed) {
     if (!s->vals32)
       if ((rc = dmalloc(s, (void **)&s->vals32, sizeof(float) * s->nnz))) return rc;
@@ -1373,7 +1395,11 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   hipLaunchKernelGGL(bk_dg, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->diag, s->vals, dg64, dg32,
                      b->p.precond);
   EHIP_HIP_TRY(hipGetLastError());
-  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  // the engine's stream waits for this preparation on the system's stream
+  // (no host wait: a replay may be running)
+  if (!b->ev_admit) EHIP_HIP_TRY(hipEventCreateWithFlags(&b->ev_admit, hipEventDisableTiming));
+  EHIP_HIP_TRY(hipEventRecord(b->ev_admit, s->stream));
+  EHIP_HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_admit, 0));
   // slot state
   BSys z;
   std::memset(&z, 0, sizeof(z));
@@ -1386,10 +1412,12 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   z.inner_maxit = b->p.inner_maxit > 0 ? b->p.inner_maxit : 1000;
   z.phase = b->mixed ? PH_RES : PH_INIT;
   z.omega = 1.0;
-  EHIP_HIP_TRY(hipMemcpyAsync(&b->st[slot], &z, sizeof(BSys), hipMemcpyHostToDevice, b->stream));
-  EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+  hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, b->stream, &b->st[slot], z);
   sl.s = s;
   sl.used = true;
+  sl.asm_check = asm_later;
+  sl.first_launch = b->nlaunch + 1;
+  b->dirty = true;
   sl.nb = sched_nb(b->mixed, b->method == EHIP_CG ? 0 : 1, b->fast, size_class(s->nnz));
   sl.ck = make_chunks(s, sl.nb);
   b->bytes += sl.bytes;
@@ -1411,13 +1439,14 @@ static int engine_retire(ehip_batch *b, int slot, ehip_report *r) {
   rep.res_scaled = h.res;
   int rc;
   if (b->p.scaling)
-    if ((rc = backscale_system(s, s->rhs, s->x, sl.bscale))) return rc;
+    if ((rc = backscale_system(s, s->rhs, s->x, 0.0, b->bnorm + slot))) return rc;
   // unscaled FP64 residual and ComputeNorm
   double *b0 = sl.w64 + (size_t)NV64 * s->n;
   double rt = 0, nrm = 0;
   if ((rc = ehip_internal_report(s, b0, &rt, &nrm))) return rc;
   rep.res_true = rt;
   rep.norm = nrm;
+  if (sl.asm_check && assembly_check(s) != EHIP_OK) rep.info = EHIP_HUTI_DIVERGENCE;  // degenerate elements
   s->has_x = true;
   if (r) *r = rep;
   b->bytes -= sl.bytes;  // the workspace stays with the system for its next admission
@@ -1427,7 +1456,7 @@ static int engine_retire(ehip_batch *b, int slot, ehip_report *r) {
 
 // Replays the iteration graph once (graph_iters iterations of every slot),
 // or launches the same kernels one by one when graphs are off.
-static int engine_step(ehip_batch *b) {
+static int engine_launch(ehip_batch *b) {
   const int iters = b->p.check_every > 0 ? b->p.check_every : 8;
   if (graphs_enabled()) {
     if (!b->graph_ok) {
@@ -1441,6 +1470,13 @@ static int engine_step(ehip_batch *b) {
   }
   EHIP_HIP_TRY(hipMemcpyAsync(b->h_active, b->active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   EHIP_HIP_TRY(hipMemcpyAsync(b->h_st, b->st, sizeof(BSys) * b->capacity, hipMemcpyDeviceToHost, b->stream));
+  ++b->nlaunch;
+  return EHIP_OK;
+}
+
+static int engine_step(ehip_batch *b) {
+  int rc = engine_launch(b);
+  if (rc) return rc;
   EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
   return EHIP_OK;
 }
@@ -1608,11 +1644,10 @@ static void engine_free(ehip_batch *b) {
   if (!b) return;
   lane_stop(b);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  for (auto &sl : b->slot) {
-  }
+  if (b->ev_admit) (void)hipEventDestroy(b->ev_admit);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void *ptrs[] = {b->chunks, b->nchunks, b->slots, b->st, b->p64, b->p32, b->active};
+  void *ptrs[] = {b->chunks, b->nchunks, b->slots, b->st, b->p64, b->p32, b->active, b->bnorm};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (b->h_active) (void)hipHostFree(b->h_active);
@@ -1856,8 +1891,6 @@ extern "C" int ehip_server_submit(ehip_batch *b, ehip_system *s, int *slot_out) 
   }
   int slot = engine_admit(b, s);
   if (slot < 0) return slot;
-  int rc = upload_worklist(b);
-  if (rc) return rc;
   if (slot_out) *slot_out = slot;
   return EHIP_OK;
 }
@@ -1887,34 +1920,49 @@ static void lane_collect(ehip_batch *b, int *ndone, ehip_system **done_systems, 
   L->done.erase(L->done.begin(), L->done.begin() + (long)k);
 }
 
+// Pipelined: a step first takes the results of the replay the previous step
+// launched (waiting for it), retires the finished systems, rebuilds the
+// worklist if systems came or went, and launches the next replay without
+// waiting for it -- so the caller's admissions (assembly, Dirichlet rows,
+// scaling on each system's own stream) overlap the running replay.  A
+// system admitted while a replay runs joins the next one.
 extern "C" int ehip_server_step(ehip_batch *b, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
                                 int max_done) {
   if (!b || !ndone) return EHIP_EARG;
   *ndone = 0;
+  int rc;
+  bool changed = false;
+  if (b->in_flight) {
+    EHIP_HIP_TRY(hipStreamSynchronize(b->stream));
+    b->in_flight = false;
+    for (int i = 0; i < b->capacity; ++i) {
+      auto &sl = b->slot[i];
+      // results of replay nlaunch: only systems that were in it
+      if (!sl.used || sl.first_launch > b->nlaunch || !b->h_st[i].done) continue;
+      if (*ndone >= max_done) break;
+      ehip_system *s = sl.s;
+      ehip_report r{};
+      if ((rc = engine_retire(b, i, &r))) return rc;
+      if (done_systems) done_systems[*ndone] = s;
+      if (done_reports) done_reports[*ndone] = r;
+      (*ndone)++;
+      changed = true;
+    }
+  }
   int live = 0;
   for (auto &sl : b->slot) live += sl.used;
-  if (!live) {
+  if (changed || b->dirty) {
+    if ((rc = upload_worklist(b))) return rc;
+    b->dirty = false;
+  }
+  if (live) {
+    if ((rc = engine_launch(b))) return rc;
+    b->in_flight = true;
+    lane_collect(b, ndone, done_systems, done_reports, max_done, false, 0);
+  } else {
     // nothing batched: wait briefly for a concurrent-lane solve instead of spinning
-    lane_collect(b, ndone, done_systems, done_reports, max_done, true, 2);
-    return EHIP_OK;
+    lane_collect(b, ndone, done_systems, done_reports, max_done, *ndone == 0, 2);
   }
-  int rc = engine_step(b);
-  if (rc) return rc;
-  lane_collect(b, ndone, done_systems, done_reports, max_done, false, 0);
-  bool changed = false;
-  for (int i = 0; i < b->capacity; ++i) {
-    auto &sl = b->slot[i];
-    if (!sl.used || !b->h_st[i].done) continue;
-    if (*ndone >= max_done) break;
-    ehip_system *s = sl.s;
-    ehip_report r{};
-    if ((rc = engine_retire(b, i, &r))) return rc;
-    if (done_systems) done_systems[*ndone] = s;
-    if (done_reports) done_reports[*ndone] = r;
-    (*ndone)++;
-    changed = true;
-  }
-  if (changed) return upload_worklist(b);
   return EHIP_OK;
 }
 

@@ -253,9 +253,12 @@ int spmv_f64(ehip_system *s, const double *x, double *y);
 int spmv_f32(ehip_system *s, const float *x, float *y);
 
 // scaling (device): D from the diagonal, values *= D_i D_j, b *= D
+// bnorm_dev non-null (with rhs_scaling): ||b|| is written there and used on
+// the device, and nothing waits; bnorm_out is then not set.
 int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double *bnorm_out,
-                 bool make_f32);
-int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale);
+                 bool make_f32, double *bnorm_dev = nullptr);
+// rhs_scale_dev non-null: ||b|| is read from the device instead of rhs_scale
+int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale, const double *rhs_scale_dev = nullptr);
 
 int max_grid();
 

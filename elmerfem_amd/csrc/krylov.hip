@@ -122,7 +122,29 @@ __global__ void k_scale_rhs2(int n, double *b, double *D, double *x, double bnor
   }
 }
 
+// the same with ||b|| from device memory (sqrt of the finalized sum of squares)
+__global__ void k_scale_rhs2_dev(int n, double *b, double *D, double *x, const double *ss, double *bnorm_out) {
+  const double bnorm = sqrt(*ss);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *bnorm_out = bnorm;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double d = D[i] * bnorm;
+    D[i] = d;
+    b[i] = b[i] / bnorm;
+    if (x) x[i] = x[i] / d;
+  }
+}
+
 // BackScaleLinearSystem vector part: x *= D; D /= bnorm; b = b / D * bnorm
+__global__ void k_backscale_vec_dev(int n, double *b, double *D, double *x, const double *bn) {
+  const double bnorm = *bn;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (x) x[i] = x[i] * D[i];
+    double d = D[i] / bnorm;
+    D[i] = d;
+    b[i] = b[i] / d * bnorm;
+  }
+}
+
 __global__ void k_backscale_vec(int n, double *b, double *D, double *x, double bnorm) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     if (x) x[i] = x[i] * D[i];
@@ -132,7 +154,8 @@ __global__ void k_backscale_vec(int n, double *b, double *D, double *x, double b
   }
 }
 
-int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double *bnorm_out, bool make_f32) {
+int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double *bnorm_out, bool make_f32,
+                 double *bnorm_dev) {
   const int n = s->n;
   int rc;
   if ((rc = ensure_state(s))) return rc;
@@ -151,6 +174,13 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
     hipLaunchKernelGGL(k_scale_rhs, dim3(g), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, s->partials,
                        &s->st->ticket[0], dsum);
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(256), 0, s->stream, s->partials, g, dsum);
+    if (rhs_scaling && bnorm_dev) {
+      // ||b|| stays on the device: nothing waits here
+      hipLaunchKernelGGL(k_scale_rhs2_dev, dim3(g), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, x,
+                         (const double *)dsum, bnorm_dev);
+      EHIP_HIP_TRY(hipGetLastError());
+      return EHIP_OK;
+    }
     if (rhs_scaling) {
       double ss = 0;
       EHIP_HIP_TRY(hipMemcpyAsync(&ss, dsum, sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -164,9 +194,12 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
   return EHIP_OK;
 }
 
-int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale) {
+int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale, const double *rhs_scale_dev) {
   const int n = s->n;
-  if (b)
+  if (b && rhs_scale_dev)
+    hipLaunchKernelGGL(k_backscale_vec_dev, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, x,
+                       rhs_scale_dev);
+  else if (b)
     hipLaunchKernelGGL(k_backscale_vec, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, b, s->dscale, x,
                        rhs_scale);
   BackscaleValues f{s->vals, s->cols, s->dscale};
@@ -233,7 +266,6 @@ extern "C" int ehip_dirichlet(ehip_system *s, int nbc, const int *bc_rows, const
   if (!s || nbc < 0 || (nbc > 0 && !bc_rows)) return EHIP_EARG;
   const int n = s->n;
   int rc;
-  if ((rc = assembly_check(s))) return rc;
   // the rows and values go to the device once; a call with the same ones
   // (re-assembly with new coefficients on the same mesh) only enqueues kernels
   const bool same = s->bc_flag && (int)s->h_bc_rows.size() == nbc &&

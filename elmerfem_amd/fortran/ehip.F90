@@ -16,6 +16,8 @@ module ehip
   integer(c_int), parameter :: EHIP_CG = 0, EHIP_BICGSTAB = 1
   ! Linear System Preconditioning (IterSolve.src:372-411)
   integer(c_int), parameter :: EHIP_PRECOND_NONE = 0, EHIP_PRECOND_DIAGONAL = 1
+  ! ILU(k): EHIP_PRECOND_ILU0 + k, k = 0..9
+  integer(c_int), parameter :: EHIP_PRECOND_ILU0 = 2
   integer(c_int), parameter :: EHIP_FP64 = 0, EHIP_MIXED_IR = 1
   integer(c_int), parameter :: EHIP_FAITHFUL = 0, EHIP_FAST = 1
   ! HUTI_INFO codes (hutiter/src/huti_fdefs.h)

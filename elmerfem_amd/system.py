@@ -120,7 +120,7 @@ class System:
         coords None: the coordinates of the previous call (kept on the
         device).  wait False: the call only enqueues the work and returns 0;
         a degenerate element is then reported by the next call that waits on
-        the system (solve, dirichlet, values)."""
+        the system (solve, values, batch or server admission)."""
         mesh = getattr(self, "mesh", None)
         code = elem_code if elem_code is not None else mesh.elem_code
         cd = cdim if cdim is not None else (mesh.dim if mesh is not None else 3)

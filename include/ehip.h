@@ -192,8 +192,8 @@ int ehip_result_write(const char *path, const char *var, const char *equation, i
  * call waits and returns the device time; NULL: the work is only enqueued
  * on the system's stream (no allocation after the first call), and
  * degenerate elements are reported (EHIP_EARG) by the next call that waits
- * on the system (ehip_solve, ehip_dirichlet, ehip_system_get_values, a
- * batch or server admission, or ehip_assemble again). */
+ * on the system (ehip_solve, ehip_system_get_values, a batch or server
+ * admission, or ehip_assemble again). */
 int ehip_assemble(ehip_system *s, int physics, int elem_code, int cdim, const double *coords,
                   const double *elem_coef, const double *elem_conv, const double *elem_velo,
                   const double *elem_load, double poisson, double *t_ms);
@@ -372,7 +372,13 @@ int ehip_batch_get_solution(ehip_batch *b, int i, double *x);
  * that arrive while others run join at the next step. */
 typedef struct ehip_batch ehip_server;
 int ehip_server_create(const ehip_params *p, int max_slots, int64_t mem_budget, ehip_server **out);
-/* EHIP_ECAP: no free slot or over budget -- keep the request queued. */
+/* EHIP_ECAP: no free slot or over budget -- keep the request queued.
+ * Admission only enqueues work (the system's preparation on its own
+ * stream, which the server's stream then waits for); a system whose
+ * ehip_assemble without t_ms is still running is not waited for, and its
+ * degenerate-element check happens at retirement: a failure there gives
+ * the report info 3 (EHIP_HUTI_DIVERGENCE), with the count in
+ * ehip_last_error. */
 int ehip_server_submit(ehip_server *srv, ehip_system *s, int *slot);
 int ehip_server_step(ehip_server *srv, int *ndone, ehip_system **done_systems, ehip_report *done_reports,
                      int max_done);

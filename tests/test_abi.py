@@ -129,3 +129,13 @@ def test_schedule_table_host_logic():
     assert get(1 << 17) == (2048, 2048)
     p.method = 7
     assert L.ehip_batch_get_schedule(C.byref(p), 1, C.byref(nb), C.byref(grid)) == _lib.EARG
+
+
+@pytest.mark.skipif(not os.path.exists("/opt/rocm/lib/llvm/bin/flang"), reason="flang absent")
+def test_before_linsolve_shim_compiles():
+    """The Before-Linsolve shim builds against the ISO_C_BINDING module and
+    the stand-in for Elmer's DefUtils (it runs in the GPU tests)."""
+    import fortran_shim
+    with tempfile.TemporaryDirectory() as d:
+        exe = fortran_shim.build(d)
+        assert os.path.exists(exe)
