@@ -245,7 +245,7 @@ def run_serve(args):
     from elmerfem_amd import _lib, workloads as W
     from elmerfem_amd.batch import Server
     from elmerfem_amd.mesh import uniforms
-    import torch
+    from elmerfem_amd.system import pinned_empty
     _lib.init(0)
     P, V, REP = args.pool, max(1, args.variants), 2
     u, _ = uniforms(args.seed, np.arange(1, P + 1, dtype=np.uint64), stream=13)
@@ -264,10 +264,10 @@ def run_serve(args):
         # that do not hold the request loop
         vs = []
         for v in range(V):
-            buf = torch.empty(m.nelem, dtype=torch.float64, pin_memory=True).numpy()
+            buf = pinned_empty(m.nelem)
             buf[:] = np.exp(0.5 * rng.standard_normal(m.nelem))
             vs.append(buf)
-        load = torch.empty(m.nelem, dtype=torch.float64, pin_memory=True).numpy()
+        load = pinned_empty(m.nelem)
         load[:] = w["load"]
         meshes.append((w, load))
         sysrep.append(reps_i)

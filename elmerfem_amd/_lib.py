@@ -58,6 +58,8 @@ SPMV_SCHEDULES = [(256, 2048), (256, 1024), (512, 2048)]
 
 SIGNATURES = {
     "ehip_init": (C.c_int, [C.c_int]),
+    "ehip_host_alloc": (C.c_int, [_i64, C.POINTER(C.c_void_p)]),
+    "ehip_host_free": (C.c_int, [_vp]),
     "ehip_last_error": (C.c_char_p, []),
     "ehip_version": (C.c_int, []),
     "ehip_matvec": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -624,6 +624,23 @@ extern "C" {
 const char *ehip_last_error(void) { return ehip::g_err; }
 int ehip_version(void) { return 100; }
 
+int ehip_host_alloc(int64_t bytes, void **p) {
+  if (!p || bytes <= 0) return EHIP_EARG;
+  *p = nullptr;
+  hipError_t e = hipHostMalloc(p, (size_t)bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    set_error("hipHostMalloc(%lld bytes): %s", (long long)bytes, hipGetErrorString(e));
+    *p = nullptr;
+    return EHIP_ENOMEM;
+  }
+  return EHIP_OK;
+}
+
+int ehip_host_free(void *p) {
+  if (p) EHIP_HIP_TRY(hipHostFree(p));
+  return EHIP_OK;
+}
+
 int ehip_init(int device) {
   int count = 0;
   hipError_t e = hipGetDeviceCount(&count);

@@ -20,6 +20,21 @@ from . import _lib
 from .keywords import LinearSystemKeywords
 
 
+def pinned_empty(n, dtype=np.float64):
+    """A numpy array in pinned host memory (freed with the array): uploads
+    from it are DMA copies that do not hold the caller."""
+    L = _lib.init()
+    dt = np.dtype(dtype)
+    nbytes = max(int(n) * dt.itemsize, 1)
+    p = C.c_void_p()
+    _lib.check(L.ehip_host_alloc(nbytes, C.byref(p)), "ehip_host_alloc")
+    buf = (C.c_char * nbytes).from_address(p.value)
+    arr = np.frombuffer(buf, dtype=dt, count=int(n))
+    import weakref
+    weakref.finalize(buf, L.ehip_host_free, C.c_void_p(p.value))
+    return arr
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 

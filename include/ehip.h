@@ -105,6 +105,11 @@ typedef struct {
 
 /* ---- library ---- */
 int ehip_init(int device);
+/* Pinned host memory (hipHostMalloc) for arrays a caller uploads often --
+ * per-request coefficient fields, say: copies from it are DMA transfers
+ * that do not hold the calling thread.  No counterpart in the reference. */
+int ehip_host_alloc(int64_t bytes, void **p);
+int ehip_host_free(void *p);
 const char *ehip_last_error(void);
 int ehip_version(void);
 

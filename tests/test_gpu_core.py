@@ -418,8 +418,10 @@ def test_reassembly_without_waiting(ehip):
     s = System.from_mesh(m, dofs=1, optimize_bandwidth=False)
     s.assemble(0, m.coords, elem_coef=w["coef"], elem_load=w["load"])
     rng = np.random.default_rng(5)
+    from elmerfem_amd.system import pinned_empty
     for k in range(3):
-        coef = np.exp(0.5 * rng.standard_normal(m.nelem))
+        coef = pinned_empty(m.nelem) if k == 1 else np.empty(m.nelem)   # pinned or pageable uploads
+        coef[:] = np.exp(0.5 * rng.standard_normal(m.nelem))
         vals, rhs = O.assemble(0, m, pat, coef, load=w["load"])
         assert s.assemble(0, None, elem_coef=coef, elem_load=w["load"], wait=False) == 0.0
         gv, gb = s.values()
