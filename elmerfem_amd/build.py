@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libehip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
-HIP_SOURCES = ["system.hip", "krylov.hip", "assembly.hip", "batch.hip", "noalloc.hip", "ilu.hip"]
+HIP_SOURCES = ["system.hip", "krylov.hip", "assembly.hip", "batch.hip", "noalloc.hip", "ilu.hip", "pattern_gpu.hip"]
 CXX_SOURCES = ["pattern.cpp", "meshio.cpp"]
 
 # Packed FP32 instructions are disabled for every HIP translation unit.

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -727,12 +728,9 @@ static int dispatch_element(ehip_system *s, int physics, int code, int cdim, con
 
 using namespace ehip;
 
-extern "C" int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *elements, int nbnd, int npe_bnd,
-                                     const int *bnd_elements, int dofs, int optimize, ehip_system **out) {
-  if (!elements || !out || nnodes <= 0 || nelem <= 0 || npe <= 0 || npe > 27 || dofs < 1 || dofs > 3) {
-    set_error("ehip_system_from_mesh: bad arguments");
-    return EHIP_EARG;
-  }
+// The host build (pattern.cpp, OpenMP) -- used when EHIP_PATTERN_BUILD=host.
+static int from_mesh_host(int nnodes, int nelem, int npe, const int *elements, int nbnd, int npe_bnd,
+                          const int *bnd_elements, int dofs, int optimize, ehip_system **out) {
   HostPattern P;
   int prc = build_host_pattern(nnodes, nelem, npe, elements, bnd_elements ? nbnd : 0, npe_bnd, bnd_elements, dofs,
                                optimize, P);
@@ -793,6 +791,60 @@ extern "C" int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *
                          off_bytes, hipMemcpyHostToDevice));
   *out = s;
   return EHIP_OK;
+}
+
+// CreateMatrix on the device (pattern_gpu.hip): the same arrays as the host
+// build, bit for bit.
+static int from_mesh_device(int nnodes, int nelem, int npe, const int *elements, int nbnd, int npe_bnd,
+                            const int *bnd_elements, int dofs, int optimize, ehip_system **out) {
+  for (int64_t k = 0; k < (int64_t)nelem * npe; ++k)
+    if (elements[k] < 1 || elements[k] > nnodes) {
+      set_error("ehip_system_from_mesh: node id %d out of range", elements[k]);
+      return EHIP_EARG;
+    }
+  ehip_system *s = new ehip_system();
+  EHIP_HIP_TRY(hipGetDevice(&s->dev));
+  EHIP_HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  s->dofs = dofs;
+  s->nnodes = nnodes;
+  s->nelem = nelem;
+  s->npe = npe;
+  const int ld = npe * dofs;
+  int rc;
+  {
+    std::vector<int> el0((size_t)nelem * npe);
+    for (size_t k = 0; k < el0.size(); ++k) el0[k] = elements[k] - 1;
+    if ((rc = dmalloc(s, (void **)&s->elements, sizeof(int) * el0.size()))) {
+      ehip_system_free(s);
+      return rc;
+    }
+    EHIP_HIP_TRY(hipMemcpy(s->elements, el0.data(), sizeof(int) * el0.size(), hipMemcpyHostToDevice));
+  }
+  int max_len = 0;
+  rc = build_device_pattern(s, nnodes, nelem, npe, s->elements, bnd_elements ? nbnd : 0, npe_bnd, bnd_elements, dofs,
+                            optimize, &max_len);
+  if (rc) { ehip_system_free(s); return rc; }
+  s->pattern_on_device = true;
+  if ((rc = ehip_internal_upload_pattern(s)) ||
+      (rc = dmalloc(s, (void **)&s->emat, sizeof(double) * (size_t)nelem * ld * ld)) ||
+      (rc = dmalloc(s, (void **)&s->evec, sizeof(double) * (size_t)nelem * ld))) {
+    ehip_system_free(s);
+    return rc;
+  }
+  *out = s;
+  return EHIP_OK;
+}
+
+extern "C" int ehip_system_from_mesh(int nnodes, int nelem, int npe, const int *elements, int nbnd, int npe_bnd,
+                                     const int *bnd_elements, int dofs, int optimize, ehip_system **out) {
+  if (!elements || !out || nnodes <= 0 || nelem <= 0 || npe <= 0 || npe > 27 || dofs < 1 || dofs > 3) {
+    set_error("ehip_system_from_mesh: bad arguments");
+    return EHIP_EARG;
+  }
+  const char *how = std::getenv("EHIP_PATTERN_BUILD");
+  if (how && std::strcmp(how, "host") == 0)
+    return from_mesh_host(nnodes, nelem, npe, elements, nbnd, npe_bnd, bnd_elements, dofs, optimize, out);
+  return from_mesh_device(nnodes, nelem, npe, elements, nbnd, npe_bnd, bnd_elements, dofs, optimize, out);
 }
 
 namespace ehip {

@@ -180,6 +180,7 @@ struct ehip_system {
   std::vector<int> h_bc_rows;
   std::vector<double> h_bc_vals;
   int64_t bytes = 0;
+  bool pattern_on_device = false; // rows/cols/diag were built on the device (not uploaded)
   // sampled SpMV timing inside the Krylov loop
   int prof_every = 0, prof_count = 0, prof_n = 0;
   hipEvent_t prof_ev[2 * 512] = {};
@@ -261,6 +262,13 @@ int scale_system(ehip_system *s, double *b, bool rhs_scaling, double *x, double 
 int backscale_system(ehip_system *s, double *b, double *x, double rhs_scale, const double *rhs_scale_dev = nullptr);
 
 int max_grid();
+
+// CRS pattern and row-owner maps of a mesh built on the device
+// (pattern_gpu.hip): fills s->rows/cols/diag/rhs_start/rhs_contrib/row_off
+// on the device, s->n, s->nnz, the bandwidth fields and the 1-based host
+// copies h_rows/h_cols/h_diag/h_perm; el0_dev: 0-based connectivity.
+int build_device_pattern(ehip_system *s, int nnodes, int nelem, int npe, const int *el0_dev, int nbnd, int npe_b,
+                         const int *bel, int dofs, int optimize, int *max_len);
 
 // Reports the degenerate Gauss-point metrics of an ehip_assemble call made
 // without waiting (EHIP_EARG), once; waits for the system's stream.

@@ -190,6 +190,18 @@ void optimize_bandwidth(int n, const Adjacency &A, std::vector<int> &perm, int &
 
 }  // namespace
 
+void bandwidth_permutation(int n, const int64_t *start, const int *idx, std::vector<int> &perm, int &before,
+                           int &after, int &accepted) {
+  Adjacency A;
+  A.start.assign(start, start + n + 2);
+  A.idx.assign(idx, idx + start[n + 1]);
+  perm.resize((size_t)n + 1);
+  std::iota(perm.begin(), perm.end(), 0);
+  before = after = bandwidth(n, A, nullptr) + 1;
+  accepted = 0;
+  optimize_bandwidth(n, A, perm, before, after, accepted);
+}
+
 int build_host_pattern(int nnodes, int nelem, int npe, const int *el, int nbnd, int npe_b, const int *bel, int dofs,
                        int optimize, HostPattern &P) {
   if (nnodes <= 0 || nelem <= 0 || npe <= 0 || dofs <= 0) return -1;

@@ -20,6 +20,13 @@ struct HostPattern {
 int build_host_pattern(int nnodes, int nelem, int npe, const int *el, int nbnd, int npe_b, const int *bel,
                        int dofs, int optimize, HostPattern &out);
 
+// OptimizeBandwidth (BandwidthOptimize.src:108-355) over a given node
+// adjacency (node i, 1-based, at idx[start[i] .. start[i+1]), sorted, self
+// included): perm[1..n] (resized to n+1), the half bandwidths + 1 before
+// and after, and whether the new ordering was accepted (else identity).
+void bandwidth_permutation(int n, const int64_t *start, const int *idx, std::vector<int> &perm, int &before,
+                           int &after, int &accepted);
+
 // Row-owner gather maps for deterministic assembly.  For every matrix row
 // the elements touching it in ascending element order -- the order in which
 // CRS_GlueLocalMatrix adds their blocks (CRSMatrix.src:507-590) -- each as

@@ -710,21 +710,27 @@ int ehip_internal_upload_pattern(ehip_system *s) {
   const int n = s->n;
   const int64_t nnz = s->nnz;
   int rc;
-  if ((rc = dmalloc(s, (void **)&s->rows, sizeof(int) * (n + 1)))) return rc;
-  if ((rc = dmalloc(s, (void **)&s->cols, sizeof(int) * nnz))) return rc;
-  if ((rc = dmalloc(s, (void **)&s->diag, sizeof(int) * n))) return rc;
+  if (!s->pattern_on_device) {
+    if ((rc = dmalloc(s, (void **)&s->rows, sizeof(int) * (n + 1)))) return rc;
+    if ((rc = dmalloc(s, (void **)&s->cols, sizeof(int) * nnz))) return rc;
+    if ((rc = dmalloc(s, (void **)&s->diag, sizeof(int) * n))) return rc;
+  }
   if ((rc = dmalloc(s, (void **)&s->vals, sizeof(double) * nnz))) return rc;
   if ((rc = dmalloc(s, (void **)&s->rhs, sizeof(double) * n))) return rc;
   if ((rc = dmalloc(s, (void **)&s->x, sizeof(double) * n))) return rc;
   if ((rc = dmalloc(s, (void **)&s->dscale, sizeof(double) * n))) return rc;
   std::vector<int> tmp((size_t)std::max<int64_t>(nnz, n + 1));
-  for (int i = 0; i <= n; ++i) tmp[i] = s->h_rows[i] - 1;
-  EHIP_HIP_TRY(hipMemcpy(s->rows, tmp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  if (!s->pattern_on_device) {
+    for (int i = 0; i <= n; ++i) tmp[i] = s->h_rows[i] - 1;
+    EHIP_HIP_TRY(hipMemcpy(s->rows, tmp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  }
   for (int64_t j = 0; j < nnz; ++j) tmp[j] = s->h_cols[j] - 1;
-  EHIP_HIP_TRY(hipMemcpy(s->cols, tmp.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-  std::vector<int> d0(n);
-  for (int i = 0; i < n; ++i) d0[i] = s->h_diag[i] - 1;
-  EHIP_HIP_TRY(hipMemcpy(s->diag, d0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  if (!s->pattern_on_device) {
+    EHIP_HIP_TRY(hipMemcpy(s->cols, tmp.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    std::vector<int> d0(n);
+    for (int i = 0; i < n; ++i) d0[i] = s->h_diag[i] - 1;
+    EHIP_HIP_TRY(hipMemcpy(s->diag, d0.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  }
   EHIP_HIP_TRY(hipMemsetAsync(s->vals, 0, sizeof(double) * nnz, s->stream));
   EHIP_HIP_TRY(hipMemsetAsync(s->rhs, 0, sizeof(double) * n, s->stream));
   EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
