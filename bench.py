@@ -41,7 +41,10 @@ def index_bytes(nnz: int, bits: int, fmt: str = "csr", units: int = 0) -> int:
     """Column index bytes one SpMV streams (padding not counted): int32
     indices, or 16-bit offsets plus one int32 base per segment of 2048
     nonzeros (CSR stream) or per slice (SELL-64, which also reads one offset
-    per slice)."""
+    per slice); slice-shared offsets: one int32 offset per 64 nonzeros (the
+    slice's list, read once per wave) and the slice bounds."""
+    if fmt == "sell_shared":
+        return 4 * ((nnz + 63) // 64) + 8 * units
     if fmt == "sell":
         return (4 if bits == 32 else 2) * nnz + (4 if bits == 32 else 8) * units
     return 4 * nnz if bits == 32 else 2 * nnz + 4 * ((nnz + 2047) // 2048 + 2)
@@ -49,7 +52,10 @@ def index_bytes(nnz: int, bits: int, fmt: str = "csr", units: int = 0) -> int:
 
 def row_ptr_bytes(n: int, fp32: bool, fmt: str) -> int:
     """Row pointers, where the kernel reads them: the FP32 SELL-64 kernels
-    sum the zero padding instead of reading row lengths."""
+    sum the zero padding instead of reading row lengths; the slice-shared
+    layout reads a 4-byte entry mask per row in FP64 and nothing in FP32."""
+    if fmt == "sell_shared":
+        return 0 if fp32 else 4 * n
     return 0 if (fp32 and fmt == "sell") else 4 * (n + 1)
 
 
@@ -705,7 +711,9 @@ def main():
     # its launches sampled with HIP events on the system's stream in the timed solves
     c32 = sys_.spmv_config(1)
     bits = c32["index_bits"]
-    if c32["format"] == "sell":
+    if c32["format"] == "sell_shared":
+        kname = "k_spmv_epi_sho<float,256,%d,EpiCgFast1<float>>" % sys_.spmv_shared_layout()["lane_unroll"]
+    elif c32["format"] == "sell":
         lay = sys_.spmv_sell_layout()
         if lay["node_block"]:
             kname = "k_spmv_epi_sellb<float,256,%d,%d,EpiCgFast1<float>>" % (lay["lane_unroll"], lay["node_block"])
@@ -719,7 +727,8 @@ def main():
     # algorithmic bytes of one fused FP32 CG step: FP32 values, int32 column
     # indices, row pointers, z read, q and p read and written
     alg32 = 8 * nnz + 4 * (n + 1) + 20 * n
-    # what this layout reads for the same work (16-bit column offsets, no row pointers)
+    # what this layout reads for the same work (16-bit column offsets or slice-shared
+    # offsets, no row pointers)
     bytes32 = fused_cg32_bytes(nnz, n, bits, c32["format"], c32["chunks"])
     achieved = alg32 / (us * 1e-6) / 1e9
     traffic = measured_traffic(kname)

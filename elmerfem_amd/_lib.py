@@ -18,7 +18,7 @@ PRECOND_NONE, PRECOND_DIAGONAL, PRECOND_ILU0 = 0, 1, 2  # ILU(k) = PRECOND_ILU0 
 FP64, MIXED_IR = 0, 1
 FAITHFUL, FAST = 0, 1
 PHYS_MODELPDE, PHYS_ELASTIC = 0, 1
-SPMV_CSR, SPMV_SELL = 0, 1
+SPMV_CSR, SPMV_SELL, SPMV_SELL_SHARED = 0, 1, 2
 EARG, EHIP, ENOMEM, EUNSUP, ECAP, ESTATE = -1, -2, -3, -4, -5, -6
 NOALLOC_SCRATCH_BYTES = 96 << 20
 NOALLOC_MAX_N, NOALLOC_MAX_NNZ = 262144, 4194304
@@ -97,6 +97,7 @@ SIGNATURES = {
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ehip_spmv_set_format": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ehip_spmv_shared_layout": (C.c_int, [_vp, _ip, C.POINTER(C.c_int64)]),
     "ehip_spmv_sell_layout": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ehip_spmv_set_schedule": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ehip_copy_bench": (C.c_int, [_i64, C.c_int, _vp, _vp]),

@@ -1634,10 +1634,16 @@ static void lane_worker(Lane *L) {
 }
 
 // workspace a single-system mixed or FP64 solve keeps on the device
+// (plus the format copies of the values it has not allocated yet)
 static int64_t lane_bytes(const ehip_system *s, bool mixed) {
   const int64_t n = s->n, nnz = s->nnz;
-  if (!mixed) return n * 8 * 12;
-  return n * (4 * 8 + 8 * 2) + (s->vals32 ? 0 : 4 * nnz) + (s->sell_ok && !s->sell32 ? 4 * s->sell_nnz : 0);
+  int64_t copies = 0;
+  if (s->fmt64 == 2 && s->sho_ok && !s->sho64) copies += 8 * s->sho_nnz;
+  if (s->fmt64 == 1 && s->sell_ok && !s->sell64) copies += 8 * (s->sell2_ok ? s->sell2_nnz : s->sell_nnz);
+  if (!mixed) return n * 8 * 12 + copies;
+  if (s->fmt32 == 2 && s->sho_ok && !s->sho32) copies += 4 * s->sho_nnz;
+  if (s->fmt32 == 1 && s->sell_ok && !s->sell32) copies += 4 * s->sell_nnz;
+  return n * (4 * 8 + 8 * 2) + (s->vals32 ? 0 : 4 * nnz) + copies;
 }
 
 static void engine_free(ehip_batch *b) {

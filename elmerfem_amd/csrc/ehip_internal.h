@@ -129,13 +129,30 @@ struct ehip_system {
   double *sell64 = nullptr;
   float *sell32 = nullptr;
   bool sell_ok = false;
+  // SELL-64 with slice-shared column offsets (spmv_sho_body), built when
+  // every slice's rows take their columns from one short list of offsets
+  // from the row index (stencil-like patterns: meshes of regular topology
+  // in natural node order): slice s holds |list| entries per lane at
+  // [sho_off[s], sho_off[s+1]), the list itself at sho_tab[sho_off[s] / 64],
+  // an absent entry holding value 0; bit k of sho_mask[row] says entry k is
+  // one of the row's nonzeros (the row's nonzeros in CRS order)
+  bool sho_ok = false;
+  int sho_u = 8;                      // entries per lane in flight: 8 or 9
+  int64_t sho_nnz = 0;
+  int *sho_off = nullptr;             // nslice + 1
+  int *sho_tab = nullptr;             // sho_nnz / 64
+  unsigned *sho_mask = nullptr;       // n
+  double *sho64 = nullptr;
+  float *sho32 = nullptr;
+  uint64_t sho64_ver = 0, sho32_ver = 0;
   // workspace of the multi-system engine, kept across admissions (a
   // hipFree per retired request would synchronise the whole device)
   double *bw64 = nullptr;
   size_t bw64_n = 0;
   float *bw32 = nullptr;
   size_t bw32_n = 0;
-  int fmt64 = 0, fmt32 = 0;           // SpMV format per precision: 0 CSR stream, 1 SELL-64
+  int fmt64 = 0, fmt32 = 0;           // SpMV format per precision: 0 CSR stream, 1 SELL-64,
+                                      // 2 SELL-64 with shared offsets
   uint64_t vals_ver = 1, vals32_ver = 1, sell64_ver = 0, sell32_ver = 0;
   double *vals = nullptr, *rhs = nullptr, *x = nullptr;
   float *vals32 = nullptr;
@@ -225,6 +242,8 @@ inline void touch_vals(ehip_system *s, bool v64, bool v32) {
 // SELL copies of the current values (refreshed on the system's stream when stale)
 const double *sell_values64(ehip_system *s);
 const float *sell_values32(ehip_system *s);
+const double *sho_values64(ehip_system *s);
+const float *sho_values32(ehip_system *s);
 
 // SELL-64 launch shape: nonzeros per lane in flight (kSellU, or 9 when that
 // wastes fewer loads on the pattern's slice widths), and the grid (slice

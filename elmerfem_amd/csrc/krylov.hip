@@ -1308,6 +1308,15 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ ro
   spmv_sell_body<T, T, BS, U>(rows, n, rmap, sc, off, vals, xin, nslice, epi);
 }
 
+template <typename T, int BS, int U, class Epi>
+__global__ __launch_bounds__(BS) void k_spmv_epi_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
+                                                     const unsigned *__restrict__ mask, const T *__restrict__ vals,
+                                                     const T *__restrict__ xin, int nslice, Epi epi,
+                                                     const KStateT<T> *st) {
+  if (st && solve_done(st)) return;
+  spmv_sho_body<T, T, BS, U>(n, off, tab, mask, vals, xin, nslice, epi);
+}
+
 template <int BS, int U, class Epi>
 __global__ __launch_bounds__(BS) void k_spmv_epi_sell2(const int *__restrict__ rows, int n, const int *__restrict__ rmap,
                                                        const unsigned short *__restrict__ c16,
@@ -1375,11 +1384,29 @@ static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi
              s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
 }
 
+template <typename T, int U, class Epi>
+static void launch_epi_sho(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
+  const int g = epi_sell_grid<T>(s);
+  if (s->prof_start)
+    hipExtLaunchKernelGGL((k_spmv_epi_sho<T, 256, U, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
+                          s->prof_stop, 0, s->n, s->sho_off, s->sho_tab, s->sho_mask, sv, xin, s->nslice, e, st);
+  else
+    launch_k((k_spmv_epi_sho<T, 256, U, Epi>), dim3(g), dim3(256), s->stream, s->n, s->sho_off, s->sho_tab,
+             s->sho_mask, sv, xin, s->nslice, e, st);
+}
+
+// the active format of one precision: 0 CSR stream, 1 SELL-64, 2 SELL-64
+// with slice-shared offsets
+template <typename T>
+static int active_fmt(const ehip_system *s) {
+  const int f = sizeof(T) == 4 ? s->fmt32 : s->fmt64;
+  return f == 2 ? (s->sho_ok ? 2 : 0) : (f == 1 && s->sell_ok ? 1 : 0);
+}
+
 // Workgroups (and so published partials) of an epilogue SpMV launch
 template <typename T>
 static int epi_grid(const ehip_system *s, const SpmvPlan &p) {
-  constexpr bool F32 = sizeof(T) == 4;
-  return ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) ? epi_sell_grid<T>(s) : p.grid;
+  return active_fmt<T>(s) ? epi_sell_grid<T>(s) : p.grid;
 }
 
 // SpMV with epilogue on the system's active format (SELL-64 or CSR stream);
@@ -1389,10 +1416,25 @@ template <typename T, class Epi>
 static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T *xin, const Epi &e,
                        const KStateT<T> *st) {
   constexpr bool F32 = sizeof(T) == 4;
-  if ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) {
+  // a values copy that could not be allocated: no launch, the driver's
+  // take_launch_error() check ends the solve
+  auto missing = [](const void *sv) {
+    if (sv) return false;
+    if (tl_launch_err == hipSuccess) tl_launch_err = hipErrorOutOfMemory;
+    return true;
+  };
+  if (active_fmt<T>(s) == 2) {
+    const T *sv;
+    if constexpr (F32) sv = sho_values32(s); else sv = sho_values64(s);
+    if (missing(sv)) return;
+    if (s->sho_u == 9) launch_epi_sho<T, 9>(s, sv, xin, e, st);
+    else launch_epi_sho<T, 8>(s, sv, xin, e, st);
+    return;
+  }
+  if (active_fmt<T>(s) == 1) {
     const T *sv;
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
-    [[maybe_unused]] const int g = sell_grid(s);
+    if (missing(sv)) return;
     if (s->sellb_d) {  // node-blocked layout
       if (s->sellb_d == 3) launch_epi_sellb<T, 3>(s, sv, xin, e, st);
       else launch_epi_sellb<T, 2>(s, sv, xin, e, st);
@@ -1727,7 +1769,8 @@ static void cg32_iterations(ehip_system *s, const Vecs<float> &v, int count, boo
 // What a recorded block bakes in: buffers, SpMV format and schedule.
 static uint64_t cg32_key(const ehip_system *s) {
   const uint64_t parts[] = {(uint64_t)(uintptr_t)s->work32, (uint64_t)(uintptr_t)s->vals32,
-                            (uint64_t)(uintptr_t)s->sell32, (uint64_t)(uintptr_t)s->partials32,
+                            (uint64_t)(uintptr_t)s->sell32, (uint64_t)(uintptr_t)s->sho32,
+                            (uint64_t)(uintptr_t)s->partials32,
                             (uint64_t)(uintptr_t)s->st32, (uint64_t)(uintptr_t)s->plan32.chunk_row,
                             (uint64_t)s->fmt32, (uint64_t)s->plan32.block, (uint64_t)s->plan32.nb,
                             (uint64_t)s->plan32.grid, (uint64_t)s->n, (uint64_t)cg_fold_enabled()};
@@ -1821,7 +1864,8 @@ static int mixed_cg_ru(ehip_system *s, SolveCtx &c, double delta, int *inner_tot
   double rate = 0.0;  // log residual change per iteration (< 0)
   int batch = std::min(cap, 8);
   // the SELL copy of the FP32 values is refreshed here, outside any recording
-  if (s->fmt32 == 1 && s->sell_ok && !sell_values32(s)) return EHIP_ENOMEM;
+  if (active_fmt<float>(s) == 1 && !sell_values32(s)) return EHIP_ENOMEM;
+  if (active_fmt<float>(s) == 2 && !sho_values32(s)) return EHIP_ENOMEM;
   while (!s->h_st->done) {
     if (c.prof || !graphs_enabled()) {
       cg32_iterations(s, v, batch, c.prof);  // sampled launches need their own events

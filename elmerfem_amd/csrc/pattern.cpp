@@ -317,4 +317,60 @@ int build_row_maps(const HostPattern &P, int nelem, int npe, const int *el, std:
   return 0;
 }
 
+// Per slice, the row offsets are merged into the running union one row at a
+// time (rows of one slice mostly repeat the union, so a merge is a short
+// walk); a second walk per row sets its mask bits.
+bool slice_offsets(int n, const int *rows1, const int *c0, int max_len, std::vector<int> &len,
+                   std::vector<int> &tab, std::vector<unsigned> &mask) {
+  if (max_len > 32) max_len = 32;
+  const int S = (n + 63) / 64;
+  len.assign((size_t)S, 0);
+  tab.assign((size_t)S * max_len, 0);
+  mask.assign((size_t)n, 0u);
+  int fail = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : fail)
+  for (int sl = 0; sl < S; ++sl) {
+    if (fail) continue;
+    int u[33], t[33];
+    int nu = 0;
+    const int r0 = sl * 64, r1 = std::min(n, r0 + 64);
+    bool ok = true;
+    for (int r = r0; r < r1 && ok; ++r) {
+      const int j1 = rows1[r + 1] - 1;
+      int a = 0, b = rows1[r] - 1, nt = 0;
+      while (a < nu || b < j1) {
+        const int va = a < nu ? u[a] : INT32_MAX;
+        const int vb = b < j1 ? c0[b] - r : INT32_MAX;
+        const int v = std::min(va, vb);
+        a += va == v;
+        b += vb == v;
+        if (nt == max_len) {
+          ok = false;
+          break;
+        }
+        t[nt++] = v;
+      }
+      std::copy(t, t + nt, u);
+      nu = nt;
+    }
+    if (!ok) {
+      fail = 1;
+      continue;
+    }
+    len[sl] = nu;
+    std::copy(u, u + nu, tab.begin() + (size_t)sl * max_len);
+    for (int r = r0; r < r1; ++r) {
+      unsigned m = 0;
+      int k = 0;
+      for (int b = rows1[r] - 1; b < rows1[r + 1] - 1; ++b) {
+        const int v = c0[b] - r;
+        while (u[k] != v) ++k;  // v is in the union, which is sorted
+        m |= 1u << k;
+      }
+      mask[r] = m;
+    }
+  }
+  return !fail;
+}
+
 }  // namespace ehip

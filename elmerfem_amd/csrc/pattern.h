@@ -36,4 +36,13 @@ void bandwidth_permutation(int n, const int64_t *start, const int *idx, std::vec
 int build_row_maps(const HostPattern &P, int nelem, int npe, const int *el, std::vector<int> &row_start,
                    std::vector<int> &row_contrib, std::vector<uint16_t> &row_off, int *max_len);
 
+// Slice-shared column offsets of a pattern cut into slices of 64 rows: for
+// slice s, the sorted union of (column - row) over its rows at
+// tab[s * max_len ..] (len[s] entries), and for every row the bit mask of
+// the union entries that are its nonzeros (bit k: entry k).  rows1 is the
+// 1-based row pointer, c0 the 0-based columns, each row sorted.  Returns
+// false as soon as one slice's union exceeds max_len (<= 32) offsets.
+bool slice_offsets(int n, const int *rows1, const int *c0, int max_len, std::vector<int> &len,
+                   std::vector<int> &tab, std::vector<unsigned> &mask);
+
 }  // namespace ehip

@@ -448,4 +448,73 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
   if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
 }
 
+// SELL-64 with slice-shared column offsets, for stencil-like patterns
+// (meshes of regular topology in natural node order): entry k of every lane
+// of slice s multiplies x at row + tab[k], one offset list per slice (the
+// sorted union of its rows' column offsets), an entry that is not one of
+// the row's nonzeros holding value 0.  The column stream disappears -- a
+// wave reads one offset per entry, a wave-uniform (scalar) load, instead of
+// 64 indices -- and a wave's x loads are 64 consecutive elements.  Entries
+// are added in ascending offset order, i.e. the row's CRS order.  FP64 adds
+// only the row's own nonzeros (bit k of mask[row]), so results stay
+// bit-identical to the reference loop whatever x holds; FP32 adds the
+// absent entries too, as +0 times x at a column inside the matrix (clamped),
+// which leaves a finite sum unchanged (the same results as the other
+// layouts).
+template <typename V, typename A, int BS, int U, class Epi>
+__device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off, const int *__restrict__ tab,
+                                              const unsigned *__restrict__ mask, const V *__restrict__ vals,
+                                              const A *__restrict__ x, int nslice, Epi &epi) {
+  constexpr int WPB = BS / 64;
+  constexpr bool kF32 = sizeof(V) == 4;
+  constexpr int NV = Epi::NV;
+  constexpr int NA = NV > 0 ? NV : 1;
+  A acc[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) acc[k] = (A)0;
+  epi.init();
+  if (epi_skipped(epi)) return;
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const XcdWalk w((nslice + WPB - 1) / WPB);
+  for (int gidx = w.first; gidx < w.last; gidx += w.step) {
+    const int sl = gidx * WPB + wv;  // wave-uniform
+    if (sl >= nslice) continue;
+    const int r = sl * 64 + lane;  // >= n: a padding lane
+    const int rl = min(r, n - 1);
+    const typename Epi::Pre pre = epi.load(rl);
+    const unsigned m = kF32 || r >= n ? 0u : mask[rl];
+    const int p0 = off[sl];
+    const int wd = (off[sl + 1] - p0) >> 6;  // slice width
+    const int *__restrict__ t = tab + (p0 >> 6);
+    A sum = (A)0;
+    for (int k0 = 0; k0 < wd; k0 += U) {
+      V vv[U];
+      int cc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = min(k0 + u, wd - 1);  // clamped into the slice
+        vv[u] = ntload(vals + p0 + kk * 64 + lane);
+        cc[u] = min(max(rl + t[kk], 0), n - 1);
+      }
+      A xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = x[cc[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma clang fp contract(off)
+        const A tv = sum + (A)vv[u] * xv[u];
+        if constexpr (kF32) {
+          sum = (k0 + u < wd) ? tv : sum;  // only the clamped repeats are skipped
+        } else {
+          const int k = k0 + u;  // mask bits stop below wd <= 32
+          sum = (k < 32 && ((m >> (k & 31)) & 1u)) ? tv : sum;
+        }
+      }
+    }
+    if (r < n) epi.row(r, sum, pre, acc);
+  }
+  if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
+}
+
 }  // namespace ehip

@@ -11,6 +11,7 @@
 #include <mutex>
 
 #include "ehip_internal.h"
+#include "pattern.h"
 #include "spmv_core.h"
 
 namespace ehip {
@@ -170,6 +171,14 @@ __global__ __launch_bounds__(BS) void k_spmv_sellb(const int *__restrict__ rows,
   spmv_sellb_body<V, A, BS, U, D>(rows, n, rmap, cb, nbase, off, vals, x, nslice, epi);
 }
 
+template <typename V, typename A, int BS, int U>
+__global__ __launch_bounds__(BS) void k_spmv_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
+                                                 const unsigned *__restrict__ mask, const V *__restrict__ vals,
+                                                 const A *__restrict__ x, int nslice, A *y) {
+  EpiStore<A> epi{y};
+  spmv_sho_body<V, A, BS, U>(n, off, tab, mask, vals, x, nslice, epi);
+}
+
 template <typename V, typename A>
 static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A *x, A *y) {
   if (s->nnz == 0) {
@@ -177,6 +186,19 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
     return EHIP_OK;
   }
   constexpr bool F32 = sizeof(V) == 4;
+  if ((F32 ? s->fmt32 : s->fmt64) == 2 && s->sho_ok) {
+    const V *sv;
+    if constexpr (F32) sv = sho_values32(s); else sv = sho_values64(s);
+    if (!sv) return EHIP_ENOMEM;
+    const int g = sell_grid(s);
+    if (s->sho_u == 9)
+      hipLaunchKernelGGL((k_spmv_sho<V, A, 256, 9>), dim3(g), dim3(256), 0, s->stream, s->n, s->sho_off, s->sho_tab,
+                         s->sho_mask, sv, x, s->nslice, y);
+    else
+      hipLaunchKernelGGL((k_spmv_sho<V, A, 256, 8>), dim3(g), dim3(256), 0, s->stream, s->n, s->sho_off, s->sho_tab,
+                         s->sho_mask, sv, x, s->nslice, y);
+    return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
+  }
   if ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) {
     const V *sv;
     if constexpr (F32) sv = sell_values32(s); else sv = sell_values64(s);
@@ -337,6 +359,46 @@ static int64_t sell_slices(int n, const int *rows, const std::vector<int> *ord, 
   return tot;
 }
 
+// SELL-64 with slice-shared offsets (spmv_sho_body), made the default
+// format of both precisions when every slice's union of column offsets has
+// at most 32 entries and the layout holds at most 1/8 more entries than the
+// SELL-64 layout it replaces (sell_tot).
+static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_tot) {
+  const int n = s->n;
+  const int S = (n + 63) / 64;
+  constexpr int kMax = 32;
+  std::vector<int> len, tab;
+  std::vector<unsigned> mask;
+  if (!slice_offsets(n, s->h_rows.data(), c0.data(), kMax, len, tab, mask)) return EHIP_OK;
+  std::vector<int> off((size_t)S + 1);
+  int64_t tot = 0, w8 = 0, w9 = 0;
+  for (int sl = 0; sl < S; ++sl) {
+    off[sl] = (int)std::min<int64_t>(tot, INT32_MAX);
+    tot += 64 * (int64_t)len[sl];
+    w8 += (len[sl] + 7) / 8 * 8 - len[sl];
+    w9 += (len[sl] + 8) / 9 * 9 - len[sl];
+  }
+  if (tot > sell_tot + sell_tot / 8 || tot >= (int64_t)INT32_MAX || tot == 0) return EHIP_OK;
+  off[S] = (int)tot;
+  std::vector<int> ctab((size_t)(tot / 64));
+  for (int sl = 0; sl < S; ++sl)
+    std::copy(tab.begin() + (size_t)sl * kMax, tab.begin() + (size_t)sl * kMax + len[sl], ctab.begin() + off[sl] / 64);
+  int rc;
+  if ((rc = dmalloc(s, (void **)&s->sho_off, sizeof(int) * off.size())) ||
+      (rc = dmalloc(s, (void **)&s->sho_tab, sizeof(int) * ctab.size())) ||
+      (rc = dmalloc(s, (void **)&s->sho_mask, sizeof(unsigned) * (size_t)n)))
+    return rc;
+  EHIP_HIP_TRY(hipMemcpy(s->sho_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->sho_tab, ctab.data(), sizeof(int) * ctab.size(), hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemcpy(s->sho_mask, mask.data(), sizeof(unsigned) * (size_t)n, hipMemcpyHostToDevice));
+  // entries per lane in flight: 8 or 9, whichever loads fewer clamped repeats
+  s->sho_u = w9 < w8 ? 9 : 8;
+  s->sho_nnz = tot;
+  s->sho_ok = true;
+  s->fmt64 = s->fmt32 = 2;
+  return EHIP_OK;
+}
+
 static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   const int n = s->n;
   const int64_t nnz = s->nnz;
@@ -491,12 +553,12 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
     if ((rc = dmalloc(s, (void **)&s->sell_c32, sizeof(int) * tot))) return rc;
     EHIP_HIP_TRY(hipMemcpy(s->sell_c32, col.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
   }
-  if ((rc = dmalloc(s, (void **)&s->sell64, sizeof(double) * tot64))) return rc;
-  EHIP_HIP_TRY(hipMemset(s->sell64, 0, sizeof(double) * tot64));
+  (void)tot64;  // the FP64 values copy is allocated at first use (sell_values64)
   s->nslice = S;
   s->sell_nnz = tot;
   s->sell_ok = true;
   s->fmt64 = s->fmt32 = 1;
+  if (!mapped && !blk) return build_sho(s, c0, tot);
   return EHIP_OK;
 }
 
@@ -591,6 +653,10 @@ static void sell_gather(ehip_system *s, const int *off, const T *in, T *out) {
 }
 
 const double *sell_values64(ehip_system *s) {
+  if (!s->sell64) {
+    if (dmalloc(s, (void **)&s->sell64, sizeof(double) * (s->sell2_ok ? s->sell2_nnz : s->sell_nnz))) return nullptr;
+    s->sell64_ver = 0;
+  }
   if (s->sell64_ver != s->vals_ver) {
     if (s->sell2_ok)
       sell_gather<double, true>(s, s->sell2_off, s->vals, s->sell64);
@@ -611,6 +677,72 @@ const float *sell_values32(ehip_system *s) {
     s->sell32_ver = s->vals32_ver;
   }
   return s->sell32;
+}
+
+// Values into the slice-shared-offset layout, one wave per slice: the
+// slice's CRS range (at most 64 x 32 entries, every row being a subset of
+// the slice's offset list) is staged in LDS with coalesced loads, then lane
+// l writes entry k of its row -- its next nonzero when bit k of the row's
+// mask is set, else 0 -- so consecutive lanes store consecutive addresses.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sho_gather(int n, int nslice, const int *__restrict__ rows,
+                                                    const unsigned *__restrict__ mask, const int *__restrict__ off,
+                                                    const T *__restrict__ in, T *__restrict__ out) {
+  __shared__ T buf[4][64 * 32];
+  const int lane = (int)threadIdx.x & 63;
+  const int wv = (int)threadIdx.x >> 6;
+  const int ngroups = (nslice + 3) / 4;
+  for (int grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+    const int sl = grp * 4 + wv;
+    const bool live = sl < nslice;
+    const int a = live ? rows[sl * 64] : 0;
+    const int b = live ? rows[min(sl * 64 + 64, n)] : 0;
+    for (int j = lane; j < b - a; j += 64) buf[wv][j] = in[a + j];
+    __syncthreads();
+    if (live) {
+      const int r = sl * 64 + lane;
+      const unsigned m = r < n ? mask[r] : 0u;
+      int j = (r < n ? rows[r] : a) - a;
+      const int p0 = off[sl];
+      const int wd = (off[sl + 1] - p0) >> 6;
+      for (int k = 0; k < wd; ++k) {
+        const bool on = (m >> k) & 1u;
+        out[p0 + (int64_t)k * 64 + lane] = on ? buf[wv][j] : (T)0;
+        j += on;
+      }
+    }
+    __syncthreads();  // the next step's staging overwrites buf
+  }
+}
+
+template <typename T>
+static void sho_gather(ehip_system *s, const T *in, T *out) {
+  hipLaunchKernelGGL((k_sho_gather<T>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n, s->nslice, s->rows,
+                     s->sho_mask, s->sho_off, in, out);
+}
+
+const double *sho_values64(ehip_system *s) {
+  if (!s->sho64) {
+    if (dmalloc(s, (void **)&s->sho64, sizeof(double) * s->sho_nnz)) return nullptr;
+    s->sho64_ver = 0;
+  }
+  if (s->sho64_ver != s->vals_ver) {
+    sho_gather<double>(s, s->vals, s->sho64);
+    s->sho64_ver = s->vals_ver;
+  }
+  return s->sho64;
+}
+
+const float *sho_values32(ehip_system *s) {
+  if (!s->sho32) {
+    if (dmalloc(s, (void **)&s->sho32, sizeof(float) * s->sho_nnz)) return nullptr;
+    s->sho32_ver = 0;
+  }
+  if (s->sho32_ver != s->vals32_ver) {
+    sho_gather<float>(s, s->vals32, s->sho32);
+    s->sho32_ver = s->vals32_ver;
+  }
+  return s->sho32;
 }
 
 }  // namespace ehip
@@ -787,7 +919,8 @@ int ehip_system_free(ehip_system *s) {
                   s->cols16,     s->seg_base,    s->sell_off,  s->sell_c32,
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
                   s->sell2_c16,  s->sell_row,    s->sellb_c16, s->sellb_base, s->bw64,
-                  s->bw32};
+                  s->bw32,       s->sho_off,     s->sho_tab,   s->sho_mask,   s->sho64,
+                  s->sho32};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
@@ -949,9 +1082,15 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
                      int *index_bits) {
   if (!s) return EHIP_EARG;
   const SpmvPlan &p = precision == 0 ? s->plan64 : s->plan32;
-  const int fmt = (precision == 0 ? s->fmt64 : s->fmt32) == 1 && s->sell_ok ? EHIP_SPMV_SELL : EHIP_SPMV_CSR;
+  const int f = precision == 0 ? s->fmt64 : s->fmt32;
+  const int fmt = f == 2 && s->sho_ok ? EHIP_SPMV_SELL_SHARED : (f == 1 && s->sell_ok ? EHIP_SPMV_SELL : EHIP_SPMV_CSR);
   if (format) *format = fmt;
-  if (fmt == EHIP_SPMV_SELL) {
+  if (fmt == EHIP_SPMV_SELL_SHARED) {
+    if (block) *block = 256;
+    if (nnz_per_block) *nnz_per_block = 0;
+    if (chunks) *chunks = s->nslice;
+    if (index_bits) *index_bits = 0;
+  } else if (fmt == EHIP_SPMV_SELL) {
     if (block) *block = 256;
     if (nnz_per_block) *nnz_per_block = 0;
     if (chunks) *chunks = s->nslice;
@@ -979,13 +1118,30 @@ int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int
   return EHIP_OK;
 }
 
+int ehip_spmv_shared_layout(ehip_system *s, int *lane_unroll, int64_t *entries) {
+  if (!s) return EHIP_EARG;
+  if (!s->sho_ok) {
+    set_error("ehip_spmv_shared_layout: this pattern has no slice-shared offset layout");
+    return EHIP_EUNSUP;
+  }
+  if (lane_unroll) *lane_unroll = s->sho_u;
+  if (entries) *entries = s->sho_nnz;
+  return EHIP_OK;
+}
+
 int ehip_spmv_set_format(ehip_system *s, int precision, int format) {
-  if (!s || (format != EHIP_SPMV_CSR && format != EHIP_SPMV_SELL)) return EHIP_EARG;
+  if (!s || (format != EHIP_SPMV_CSR && format != EHIP_SPMV_SELL && format != EHIP_SPMV_SELL_SHARED))
+    return EHIP_EARG;
   if (format == EHIP_SPMV_SELL && !s->sell_ok) {
     set_error("ehip_spmv_set_format: this pattern has no SELL-64 layout (padding above the limit)");
     return EHIP_EUNSUP;
   }
-  (precision == 0 ? s->fmt64 : s->fmt32) = format == EHIP_SPMV_SELL ? 1 : 0;
+  if (format == EHIP_SPMV_SELL_SHARED && !s->sho_ok) {
+    set_error("ehip_spmv_set_format: this pattern has no slice-shared offset layout (a slice's rows use more "
+              "than 32 distinct column offsets)");
+    return EHIP_EUNSUP;
+  }
+  (precision == 0 ? s->fmt64 : s->fmt32) = format;
   return EHIP_OK;
 }
 
@@ -1025,11 +1181,14 @@ int ehip_spmv_tune(ehip_system *s, int precision) {
     if (us < best) { best = us; bi = i; }
   }
   if ((rc = plan_spmv(s, plan, cfg[bi][0], cfg[bi][1]))) return rc;
-  if (s->sell_ok) {
-    fmt = 1;
+  for (int f = 1; f <= 2; ++f) {
+    if (!(f == 1 ? s->sell_ok : s->sho_ok)) continue;
+    const int prev = fmt;
+    fmt = f;
     double us = 0;
     if ((rc = ehip_spmv_bench(s, precision, reps, &us))) return rc;
-    if (!(us < best)) fmt = 0;
+    if (us < best) best = us;
+    else fmt = prev;
   }
   if (precision == 0) s->tuned64 = true; else s->tuned32 = true;
   return EHIP_OK;

@@ -259,11 +259,19 @@ class System:
                    "ehip_spmv_sell_layout")
         return {"sort_window": w.value, "node_block": b.value, "lane_unroll": u.value, "pair_unroll": pu.value}
 
+    def spmv_shared_layout(self):
+        """{"lane_unroll": entries each lane loads at once, "entries": stored entries}
+        of the slice-shared offset layout (EhipError when the pattern has none)."""
+        u, e = C.c_int(), C.c_int64()
+        _lib.check(_lib.load().ehip_spmv_shared_layout(self._h, C.byref(u), C.byref(e)), "ehip_spmv_shared_layout")
+        return {"lane_unroll": u.value, "entries": e.value}
+
     def spmv_config(self, precision=0):
         f, b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _lib.check(_lib.load().ehip_spmv_config(self._h, precision, C.byref(f), C.byref(b), C.byref(nb),
                                                 C.byref(ch), C.byref(ib)), "ehip_spmv_config")
-        return {"format": "sell" if f.value == _lib.SPMV_SELL else "csr", "block": b.value,
+        name = {_lib.SPMV_SELL: "sell", _lib.SPMV_SELL_SHARED: "sell_shared"}.get(f.value, "csr")
+        return {"format": name, "block": b.value,
                 "nnz_per_block": nb.value, "chunks": ch.value, "index_bits": ib.value}
 
 

@@ -286,14 +286,27 @@ int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
  *  EHIP_SPMV_CSR : CRS streamed in chunks of nonzeros, products staged in LDS;
  *  EHIP_SPMV_SELL: SELL-64 copy (slices of 64 rows, column-major, padded to
  *                  the slice's longest row), one lane per row -- built when
- *                  padding adds at most 25% to the nonzeros. */
+ *                  padding adds at most 25% to the nonzeros;
+ *  EHIP_SPMV_SELL_SHARED: SELL-64 whose slices share one list of column
+ *                  offsets from the row index (entry k of every row of a
+ *                  slice at column row + offset k, absent entries holding
+ *                  0, a per-row bit mask naming the row's nonzeros), so no
+ *                  column indices are streamed -- built, and made the
+ *                  default of both precisions, when every slice's rows use
+ *                  at most 32 distinct offsets and the layout holds at most
+ *                  1/8 more entries than SELL-64 (stencil-like patterns:
+ *                  meshes of regular topology in natural node order).
+ * Every format sums each row in CRS order; FP64 results are the same bits in
+ * all of them. */
 #define EHIP_SPMV_CSR 0
 #define EHIP_SPMV_SELL 1
+#define EHIP_SPMV_SELL_SHARED 2
 /* SpMV configuration of one precision (0 = FP64 values, 1 = FP32): format,
  * workgroup size, nonzeros per chunk (CSR; 0 for SELL), number of chunks
  * (CSR) or slices (SELL), and the column index width the kernels read (16:
  * offsets from a per-segment or per-slice base when every span is < 65536
- * columns; 32: int32 indices).  Any pointer may be NULL. */
+ * columns; 32: int32 indices; 0: none, EHIP_SPMV_SELL_SHARED).  Any pointer
+ * may be NULL. */
 int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int *nnz_per_block, int *chunks,
                      int *index_bits);
 /* Shape of the SELL-64 layout (EHIP_EUNSUP when the pattern has none):
@@ -312,9 +325,17 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
  * 3515-3640, reads CRS directly). */
 int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll,
                           int *pair_unroll);
-/* Selects the format for one precision (EHIP_EUNSUP: no SELL layout). */
+/* Shape of the slice-shared offset layout (EHIP_EUNSUP when the pattern has
+ * none): lane_unroll = entries each lane loads before using them (8, or 9
+ * when that wastes fewer loads on the slice widths); entries = stored
+ * entries, absent ones and the padding lanes of the last slice included
+ * (64 x the slices' offset-list lengths).  Any pointer may be NULL.  No
+ * counterpart in the reference. */
+int ehip_spmv_shared_layout(ehip_system *s, int *lane_unroll, int64_t *entries);
+/* Selects the format for one precision (EHIP_EUNSUP: the pattern has no
+ * layout of that format). */
 int ehip_spmv_set_format(ehip_system *s, int precision, int format);
-/* Autotuner: times every compiled CSR schedule and the SELL format on this
+/* Autotuner: times every compiled CSR schedule and the SELL formats on this
  * matrix, keeps the fastest.  Every choice sums each row in the same order. */
 int ehip_spmv_tune(ehip_system *s, int precision);
 /* Sets a schedule explicitly (EHIP_EUNSUP if it is not compiled in).  Every

@@ -28,23 +28,31 @@ def row_rel_err(pat, a, b):
     return err
 
 
-def has_sell(s):
+FORMAT_NAMES = {_lib.SPMV_CSR: "csr", _lib.SPMV_SELL: "sell", _lib.SPMV_SELL_SHARED: "sell_shared"}
+
+
+def has_format(s, fmt):
     try:
-        s.spmv_set_format(_lib.SPMV_SELL, 0)
+        s.spmv_set_format(fmt, 0)
         return True
     except _lib.EhipError:
         return False
 
 
+def has_sell(s):
+    return has_format(s, _lib.SPMV_SELL)
+
+
 def set_format(s, fmt):
     """Both precisions on one SpMV format; skips when the pattern has no
-    SELL-64 layout (small multi-DOF meshes pad their slices too much)."""
-    if fmt == _lib.SPMV_SELL and not has_sell(s):
+    layout of that format (small multi-DOF meshes pad their SELL-64 slices
+    too much; slice-shared offsets need a stencil-like pattern)."""
+    if fmt != _lib.SPMV_CSR and not has_format(s, fmt):
         s.free()
-        pytest.skip("no SELL-64 layout for this pattern (padding above the limit)")
+        pytest.skip("no %s layout for this pattern" % FORMAT_NAMES[fmt])
     s.spmv_set_format(fmt, 0)
     s.spmv_set_format(fmt, 1)
-    assert s.spmv_config(0)["format"] == ("sell" if fmt == _lib.SPMV_SELL else "csr")
+    assert s.spmv_config(0)["format"] == FORMAT_NAMES[fmt]
 
 
 def dense(rows, cols, vals):
@@ -79,7 +87,8 @@ def test_spmv_bit_exact(ehip, case):
         ref = np.zeros(pat.n)
         O.lib().or_spmv(pat.n, O._p(pat.rows), O._p(pat.cols), O._p(vals), O._p(u), O._p(ref))
         assert np.array_equal(s.spmv(u), ref)
-    for fmt in ((_lib.SPMV_SELL,) if has_sell(s) else ()) + (_lib.SPMV_CSR,):  # same row sums in every format
+    fmts = [f for f in (_lib.SPMV_SELL_SHARED, _lib.SPMV_SELL) if has_format(s, f)] + [_lib.SPMV_CSR]
+    for fmt in fmts:  # same row sums in every format
         s.spmv_set_format(fmt)
         assert np.array_equal(s.spmv(u), ref), fmt
     s.spmv_tune(0)
@@ -483,7 +492,7 @@ def test_dirichlet_bit_exact(ehip, symmetric):
 
 @pytest.mark.parametrize("case", ["cg_poisson8", "cg_lognormal8", "cg_poisson32", "bicgstab_convdiff16",
                                   "bicgstab_poisson8", "cg_elastic3", "cg_tet4_4", "cg_none_lognormal8"])
-@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL, _lib.SPMV_CSR])
+@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL_SHARED, _lib.SPMV_SELL, _lib.SPMV_CSR])
 def test_faithful_solve_matches_reference(ehip, case, fmt):
     spec = CASES[case]
     w, pat, vals, rhs, rows = assembled_case(spec)
@@ -515,7 +524,7 @@ def test_faithful_solve_matches_reference(ehip, case, fmt):
     s.free()
 
 
-@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL, _lib.SPMV_CSR])
+@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL_SHARED, _lib.SPMV_SELL, _lib.SPMV_CSR])
 @pytest.mark.parametrize("precision,mode", [(_lib.FP64, _lib.FAST), (_lib.MIXED_IR, _lib.FAST)])
 @pytest.mark.parametrize("case", ["cg_lognormal8", "cg_poisson32", "bicgstab_convdiff16", "cg_elastic3",
                                   "cg_tet4_4"])
@@ -651,7 +660,7 @@ def test_mixed_graph_replay_equals_direct_launches(ehip, case):
     s.free()
 
 
-@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL, _lib.SPMV_CSR])
+@pytest.mark.parametrize("fmt", [_lib.SPMV_SELL_SHARED, _lib.SPMV_SELL, _lib.SPMV_CSR])
 @pytest.mark.parametrize("case", ["cg_poisson32", "cg_lognormal8", "cg_tet4_4", "cg_elastic3"])
 def test_mixed_folded_steps_equal_unfolded(ehip, case, fmt, monkeypatch):
     """The mixed CG's folded FP32 steps (two launches per iteration: the next
