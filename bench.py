@@ -711,18 +711,21 @@ def main():
     # its launches sampled with HIP events on the system's stream in the timed solves
     c32 = sys_.spmv_config(1)
     bits = c32["index_bits"]
+    # the SpMV step's epilogue: folded (the previous vector step's partials summed
+    # in every workgroup, two launches per iteration) unless EHIP_CG_FOLD=0
+    epi = "EpiCgFast1<float>" if os.environ.get("EHIP_CG_FOLD", "1")[:1] == "0" else "EpiCgFold<float>"
     if c32["format"] == "sell_shared":
-        kname = "k_spmv_epi_sho<float,256,%d,EpiCgFast1<float>>" % sys_.spmv_shared_layout()["lane_unroll"]
+        kname = "k_spmv_epi_sho<float,256,%d,%s>" % (sys_.spmv_shared_layout()["lane_unroll"], epi)
     elif c32["format"] == "sell":
         lay = sys_.spmv_sell_layout()
         if lay["node_block"]:
-            kname = "k_spmv_epi_sellb<float,256,%d,%d,EpiCgFast1<float>>" % (lay["lane_unroll"], lay["node_block"])
+            kname = "k_spmv_epi_sellb<float,256,%d,%d,%s>" % (lay["lane_unroll"], lay["node_block"], epi)
         else:
-            kname = "k_spmv_epi_sell<float,256,%d,%s,EpiCgFast1<float>>" % (lay["lane_unroll"],
-                                                                           "true" if bits == 16 else "false")
+            kname = "k_spmv_epi_sell<float,256,%d,%s,%s>" % (lay["lane_unroll"], "true" if bits == 16 else "false",
+                                                            epi)
     else:
-        kname = "k_spmv_epi<float,%d,%d,%s,EpiCgFast1<float>>" % (c32["block"], c32["nnz_per_block"],
-                                                                  "ColsU16" if bits == 16 else "ColsI32")
+        kname = "k_spmv_epi<float,%d,%d,%s,%s>" % (c32["block"], c32["nnz_per_block"],
+                                                  "ColsU16" if bits == 16 else "ColsI32", epi)
     us = float(np.mean([r["spmv_us"] for r in sampled if r["spmv_samples"] > 0]))
     # algorithmic bytes of one fused FP32 CG step: FP32 values, int32 column
     # indices, row pointers, z read, q and p read and written
