@@ -571,7 +571,7 @@ def main():
     ap.add_argument("--load", type=float, default=0.7, help="serve form: arrival rate / measured capacity")
     ap.add_argument("--lane-dofs", type=int, default=100000, help="serve form: unknowns from which a request "
                                                                    "takes the concurrent lane")
-    ap.add_argument("--lane-workers", type=int, default=5, help="serve form: concurrent-lane workers (0: off)")
+    ap.add_argument("--lane-workers", type=int, default=4, help="serve form: concurrent-lane workers (0: off)")
     ap.add_argument("--slots", type=int, default=64, help="serve form: server slots")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="serve form: HBM admission budget")
     ap.add_argument("--gpus", type=int, default=1)

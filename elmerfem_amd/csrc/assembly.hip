@@ -277,8 +277,9 @@ struct ElemArgs {
   const double *gs;     // [ngp]
   const double *coef, *conv, *velo, *load;  // per element (may be null)
   double poisson;
-  double *emat;         // nelem blocks: ld x ld row-major, or packed upper triangles
-  double *evec;         // nelem * ld
+  double *emat;         // element blocks: k_element, ld x ld row-major one after another; k_element_reg,
+                        // row-major or packed upper triangles, component-major (entry k at k * nelem + e)
+  double *evec;         // nelem * ld, laid out as emat
   int *bad;             // degenerate element counter
 };
 
@@ -457,13 +458,23 @@ struct RegTables {
 template <int NPE, int NGP, int DIM, int CDIM, bool SYM>
 __global__ __launch_bounds__(256) void k_element_reg(ElemArgs a, RegTables<NPE, NGP> t) {
   constexpr int NK = SYM ? NPE * (NPE + 1) / 2 : NPE * NPE;
+  // the element's node coordinates, loaded once into this thread's own LDS
+  // column (no barrier: no thread reads another's) and read again at every
+  // Gauss point -- held in registers they would spill the block, the
+  // gradients and the metric; 48 KB for Hex8, within what the register-
+  // limited occupancy leaves per workgroup
+  __shared__ double xs[NPE * 3][256];
+  const int tid = (int)threadIdx.x;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < a.nelem; e += gridDim.x * blockDim.x) {
-    // node ids stay in registers; the coordinates are read again at every
-    // Gauss point (cache hits) rather than held, to keep the block, the
-    // gradients and the metric in registers without spilling
-    int gid[NPE];
+    {
+      int gid[NPE];
 #pragma unroll
-    for (int nd = 0; nd < NPE; nd++) gid[nd] = a.el[(size_t)e * NPE + nd];
+      for (int nd = 0; nd < NPE; nd++) gid[nd] = a.el[(size_t)e * NPE + nd];
+#pragma unroll
+      for (int nd = 0; nd < NPE; nd++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) xs[nd * 3 + c][tid] = a.xyz[3 * (size_t)gid[nd] + c];
+    }
     const double c0 = a.coef ? a.coef[e] : 0.0;
     const double c1 = (!SYM && a.conv) ? a.conv[e] : 0.0;
     const double v0 = (!SYM && a.velo) ? a.velo[3 * (size_t)e] : 0.0;
@@ -482,7 +493,7 @@ __global__ __launch_bounds__(256) void k_element_reg(ElemArgs a, RegTables<NPE, 
       for (int c = 0; c < 3; c++) {
         double X[NPE];
 #pragma unroll
-        for (int nd = 0; nd < NPE; nd++) X[nd] = a.xyz[3 * (size_t)gid[nd] + c];
+        for (int nd = 0; nd < NPE; nd++) X[nd] = xs[nd * 3 + c][tid];
 #pragma unroll
         for (int i = 0; i < DIM; i++) {
           double sdx = 0.0;
@@ -539,14 +550,17 @@ __global__ __launch_bounds__(256) void k_element_reg(ElemArgs a, RegTables<NPE, 
         F[p] = F[p] + W * sb[5] * t.basis[gp * NPE + p];
       }
     }
-    // packed (SYM) or row-major; the full row-major block with the lower
+    // packed (SYM) or row-major blocks, stored component-major: entry k of
+    // element e at k * nelem + e, so each store of a wave covers consecutive
+    // addresses (and so do the gather's loads when neighbouring rows take
+    // neighbouring elements).  The full row-major block with the lower
     // triangle mirrored measured the same in total (element kernel +31%,
-    // gather -19% at 128^3 Hex8) with 470 MB more traffic
-    double *out = a.emat + (size_t)e * NK;
+    // gather -19% at 128^3 Hex8) with 470 MB more traffic.
+    const size_t ne = (size_t)a.nelem;
 #pragma unroll
-    for (int k = 0; k < NK; k++) out[k] = K[k];
+    for (int k = 0; k < NK; k++) a.emat[k * ne + e] = K[k];
 #pragma unroll
-    for (int p = 0; p < NPE; p++) a.evec[(size_t)e * NPE + p] = F[p];
+    for (int p = 0; p < NPE; p++) a.evec[p * ne + e] = F[p];
   }
 }
 
@@ -559,10 +573,14 @@ __global__ __launch_bounds__(256) void k_element_reg(ElemArgs a, RegTables<NPE, 
 // BS rows have consecutive slots: when they fit in CAP doubles they are
 // accumulated in LDS (zeroed, added, written out with coalesced stores),
 // otherwise in place in Values.
-template <int LD, bool SYM, typename OffT>
+// SOA: blocks stored component-major (entry k of element e at k * ne + e,
+// force entry i at i * ne + e: k_element_reg), else block after block
+// (k_element).
+template <int LD, bool SYM, bool SOA, typename OffT>
 __device__ __forceinline__ double row_accumulate(int r, double *dst, const int *__restrict__ rstart,
                                                  const int *__restrict__ rcontrib, const OffT *__restrict__ roff,
-                                                 const double *__restrict__ emat, const double *__restrict__ evec) {
+                                                 const double *__restrict__ emat, const double *__restrict__ evec,
+                                                 size_t ne) {
   constexpr int NK = SYM ? LD * (LD + 1) / 2 : LD * LD;
   // the block row in pieces of CH columns, and the entries of small blocks
   // in groups of G: a piece's loads (of every entry in the group) go out
@@ -582,14 +600,22 @@ __device__ __forceinline__ double row_accumulate(int r, double *dst, const int *
 #pragma unroll
       for (int g = 0; g < G; ++g) {
         const int e = code[g] / LD, li = code[g] - e * LD;
-        const double *blk = emat + (size_t)e * NK;
         const OffT *off = roff + (size_t)min(kg + g, k1 - 1) * LD;
+        // one word for the entry's column positions when they are bytes of
+        // a whole aligned word (the map is 256-byte aligned, an entry LD bytes)
+        constexpr bool kWord = sizeof(OffT) == 1 && CH == LD && (LD == 4 || LD == 8);
+        uint64_t ow = 0;
+        if constexpr (kWord) {
+          if constexpr (LD == 8) ow = *reinterpret_cast<const uint64_t *>(off);
+          else ow = *reinterpret_cast<const uint32_t *>(off);
+        }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
           const int lj = l0 + u;
           const int idx = SYM ? (li <= lj ? tri_index<LD>(li, lj) : tri_index<LD>(lj, li)) : li * LD + lj;
-          v[g][u] = blk[idx];
-          o[g][u] = (int)off[lj];
+          v[g][u] = SOA ? emat[(size_t)idx * ne + e] : emat[(size_t)e * NK + idx];
+          if constexpr (kWord) o[g][u] = (int)((ow >> (8 * u)) & 0xff);
+          else o[g][u] = (int)off[lj];
         }
       }
 #pragma unroll
@@ -600,16 +626,21 @@ __device__ __forceinline__ double row_accumulate(int r, double *dst, const int *
     }
 #pragma unroll
     for (int g = 0; g < G; ++g)
-      if (kg + g < k1) f = f + evec[code[g]];
+      if (kg + g < k1) {
+        const int e = code[g] / LD, li = code[g] - e * LD;
+        f = f + (SOA ? evec[(size_t)li * ne + e] : evec[code[g]]);
+      }
   }
   return f;
 }
 
-template <int LD, bool SYM, typename OffT, int BS, int CAP>
+template <int LD, bool SYM, bool SOA, typename OffT, int BS, int CAP>
 __global__ __launch_bounds__(BS) void k_row_gather(int n, const int *__restrict__ rows,
                                                     const int *__restrict__ rstart, const int *__restrict__ rcontrib,
                                                     const OffT *__restrict__ roff, const double *__restrict__ emat,
-                                                    const double *__restrict__ evec, double *vals, double *rhs) {
+                                                    const double *__restrict__ evec, int nelem, double *vals,
+                                                    double *rhs) {
+  const size_t ne = (size_t)nelem;
   __shared__ double acc[CAP];
   const XcdWalk w((n + BS - 1) / BS);
   for (int rb = w.first; rb < w.last; rb += w.step) {
@@ -621,13 +652,14 @@ __global__ __launch_bounds__(BS) void k_row_gather(int n, const int *__restrict_
       for (int j = threadIdx.x; j < len; j += BS) acc[j] = 0.0;
       __syncthreads();
       if (r < r1)
-        rhs[r] = row_accumulate<LD, SYM, OffT>(r, acc + (rows[r] - base), rstart, rcontrib, roff, emat, evec);
+        rhs[r] = row_accumulate<LD, SYM, SOA, OffT>(r, acc + (rows[r] - base), rstart, rcontrib, roff, emat, evec,
+                                                    ne);
       __syncthreads();
       for (int j = threadIdx.x; j < len; j += BS) vals[base + j] = acc[j];
       __syncthreads();
     } else if (r < r1) {
       for (int j = rows[r]; j < rows[r + 1]; ++j) vals[j] = 0.0;
-      rhs[r] = row_accumulate<LD, SYM, OffT>(r, vals + rows[r], rstart, rcontrib, roff, emat, evec);
+      rhs[r] = row_accumulate<LD, SYM, SOA, OffT>(r, vals + rows[r], rstart, rcontrib, roff, emat, evec, ne);
     }
   }
 }
@@ -642,40 +674,42 @@ __global__ void k_zero_int(int *p) { *p = 0; }
 // 8 rows (rows of up to 32 entries on average stay in LDS), 64 rows in 64 KB
 // for blocks of up to 30 rows, 16 rows in 64 KB for the 81-row Hex27
 // elasticity blocks (rows of up to 375 entries)
-template <int LD, bool SYM>
+template <int LD, bool SYM, bool SOA>
 static void row_gather_as(ehip_system *s) {
   constexpr int BS = LD <= 8 ? 128 : (LD <= 30 ? 64 : 16);
   constexpr int CAP = LD <= 8 ? 4096 : 8192;
   const int grid = (std::min((s->n + BS - 1) / BS, 32768) + 7) / 8 * 8;  // a multiple of 8 (XcdWalk)
   if (s->row_off_bytes == 1)
-    hipLaunchKernelGGL((k_row_gather<LD, SYM, uint8_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
-                       s->rows, s->rhs_start, s->rhs_contrib, (const uint8_t *)s->row_off, s->emat, s->evec, s->vals,
-                       s->rhs);
+    hipLaunchKernelGGL((k_row_gather<LD, SYM, SOA, uint8_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
+                       s->rows, s->rhs_start, s->rhs_contrib, (const uint8_t *)s->row_off, s->emat, s->evec,
+                       s->nelem, s->vals, s->rhs);
   else
-    hipLaunchKernelGGL((k_row_gather<LD, SYM, uint16_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
+    hipLaunchKernelGGL((k_row_gather<LD, SYM, SOA, uint16_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
                        s->rows, s->rhs_start, s->rhs_contrib, (const uint16_t *)s->row_off, s->emat, s->evec,
-                       s->vals, s->rhs);
+                       s->nelem, s->vals, s->rhs);
 }
 
 // The gathers compiled: block row lengths of the supported element types
-// (Tri3, Quad4/Tet4, Hex8, Tet10, Hex27 scalar; Hex8, Tet10, Hex27 with 3 dofs)
+// (Tri3, Quad4/Tet4, Hex8 scalar from k_element_reg, component-major;
+// Tet10, Hex27 scalar and Hex8, Tet10, Hex27 with 3 dofs from k_element,
+// block after block)
 static int launch_row_gather(ehip_system *s, int ld, bool packed) {
   if (packed) {
     switch (ld) {
-      case 3: row_gather_as<3, true>(s); return EHIP_OK;
-      case 4: row_gather_as<4, true>(s); return EHIP_OK;
-      case 8: row_gather_as<8, true>(s); return EHIP_OK;
+      case 3: row_gather_as<3, true, true>(s); return EHIP_OK;
+      case 4: row_gather_as<4, true, true>(s); return EHIP_OK;
+      case 8: row_gather_as<8, true, true>(s); return EHIP_OK;
     }
   } else {
     switch (ld) {
-      case 3: row_gather_as<3, false>(s); return EHIP_OK;
-      case 4: row_gather_as<4, false>(s); return EHIP_OK;
-      case 8: row_gather_as<8, false>(s); return EHIP_OK;
-      case 10: row_gather_as<10, false>(s); return EHIP_OK;
-      case 24: row_gather_as<24, false>(s); return EHIP_OK;
-      case 27: row_gather_as<27, false>(s); return EHIP_OK;
-      case 30: row_gather_as<30, false>(s); return EHIP_OK;
-      case 81: row_gather_as<81, false>(s); return EHIP_OK;
+      case 3: row_gather_as<3, false, true>(s); return EHIP_OK;
+      case 4: row_gather_as<4, false, true>(s); return EHIP_OK;
+      case 8: row_gather_as<8, false, true>(s); return EHIP_OK;
+      case 10: row_gather_as<10, false, false>(s); return EHIP_OK;
+      case 24: row_gather_as<24, false, false>(s); return EHIP_OK;
+      case 27: row_gather_as<27, false, false>(s); return EHIP_OK;
+      case 30: row_gather_as<30, false, false>(s); return EHIP_OK;
+      case 81: row_gather_as<81, false, false>(s); return EHIP_OK;
     }
   }
   set_error("ehip_assemble: no gather for element blocks of %d rows", ld);
