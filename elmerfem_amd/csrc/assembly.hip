@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_element_reg(ElemArgs a, RegTables<NPE, 
 // force entry i at i * ne + e: k_element_reg), else block after block
 // (k_element).
 template <int LD, bool SYM, bool SOA, typename OffT>
-__device__ __forceinline__ double row_accumulate(int r, double *dst, const int *__restrict__ rstart,
+__device__ __forceinline__ double row_accumulate(int r, double *dst, const int *__restrict__ mbase,
                                                  const int *__restrict__ rcontrib, const OffT *__restrict__ roff,
                                                  const double *__restrict__ emat, const double *__restrict__ evec,
                                                  size_t ne) {
@@ -588,22 +588,26 @@ __device__ __forceinline__ double row_accumulate(int r, double *dst, const int *
   // column order
   constexpr int CH = LD <= 16 ? LD : (LD % 9 == 0 ? 9 : (LD % 10 == 0 ? 10 : (LD % 8 == 0 ? 8 : 1)));
   constexpr int G = LD <= 8 ? 4 : 1;
+  // one word for an entry's column positions when they are the bytes of a
+  // whole aligned word (the map is 256-byte aligned, an entry LD bytes)
+  constexpr bool kWord = sizeof(OffT) == 1 && CH == LD && (LD == 4 || LD == 8);
   double f = 0.0;
-  const int k0 = rstart[r], k1 = rstart[r + 1];
-  for (int kg = k0; kg < k1; kg += G) {
+  // slice-major maps: entry k of slice row l at b + 64 k + l; the slice's
+  // entry count is the same for its 64 rows (shorter rows padded with -1)
+  const int sl = r >> 6, l = r & 63;
+  const int b = mbase[sl], cnt = (mbase[sl + 1] - b) >> 6;
+  for (int kg = 0; kg < cnt; kg += G) {
     int code[G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) code[g] = rcontrib[min(kg + g, k1 - 1)];
+    for (int g = 0; g < G; ++g) code[g] = kg + g < cnt ? rcontrib[b + (kg + g) * 64 + l] : -1;
     for (int l0 = 0; l0 < LD; l0 += CH) {
       double v[G][CH];
       int o[G][CH];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int e = code[g] / LD, li = code[g] - e * LD;
-        const OffT *off = roff + (size_t)min(kg + g, k1 - 1) * LD;
-        // one word for the entry's column positions when they are bytes of
-        // a whole aligned word (the map is 256-byte aligned, an entry LD bytes)
-        constexpr bool kWord = sizeof(OffT) == 1 && CH == LD && (LD == 4 || LD == 8);
+        const int cg = code[g] >= 0 ? code[g] : 0;  // a padding entry loads element 0 and adds nothing
+        const int e = cg / LD, li = cg - e * LD;
+        const OffT *off = roff + ((size_t)b + (size_t)min(kg + g, cnt - 1) * 64 + l) * LD;
         uint64_t ow = 0;
         if constexpr (kWord) {
           if constexpr (LD == 8) ow = *reinterpret_cast<const uint64_t *>(off);
@@ -620,13 +624,13 @@ __device__ __forceinline__ double row_accumulate(int r, double *dst, const int *
       }
 #pragma unroll
       for (int g = 0; g < G; ++g)
-        if (kg + g < k1)
+        if (code[g] >= 0)
 #pragma unroll
           for (int u = 0; u < CH; ++u) dst[o[g][u]] += v[g][u];
     }
 #pragma unroll
     for (int g = 0; g < G; ++g)
-      if (kg + g < k1) {
+      if (code[g] >= 0) {
         const int e = code[g] / LD, li = code[g] - e * LD;
         f = f + (SOA ? evec[(size_t)li * ne + e] : evec[code[g]]);
       }
@@ -636,7 +640,7 @@ __device__ __forceinline__ double row_accumulate(int r, double *dst, const int *
 
 template <int LD, bool SYM, bool SOA, typename OffT, int BS, int CAP>
 __global__ __launch_bounds__(BS) void k_row_gather(int n, const int *__restrict__ rows,
-                                                    const int *__restrict__ rstart, const int *__restrict__ rcontrib,
+                                                    const int *__restrict__ mbase, const int *__restrict__ rcontrib,
                                                     const OffT *__restrict__ roff, const double *__restrict__ emat,
                                                     const double *__restrict__ evec, int nelem, double *vals,
                                                     double *rhs) {
@@ -652,16 +656,97 @@ __global__ __launch_bounds__(BS) void k_row_gather(int n, const int *__restrict_
       for (int j = threadIdx.x; j < len; j += BS) acc[j] = 0.0;
       __syncthreads();
       if (r < r1)
-        rhs[r] = row_accumulate<LD, SYM, SOA, OffT>(r, acc + (rows[r] - base), rstart, rcontrib, roff, emat, evec,
+        rhs[r] = row_accumulate<LD, SYM, SOA, OffT>(r, acc + (rows[r] - base), mbase, rcontrib, roff, emat, evec,
                                                     ne);
       __syncthreads();
       for (int j = threadIdx.x; j < len; j += BS) vals[base + j] = acc[j];
       __syncthreads();
     } else if (r < r1) {
       for (int j = rows[r]; j < rows[r + 1]; ++j) vals[j] = 0.0;
-      rhs[r] = row_accumulate<LD, SYM, SOA, OffT>(r, vals + rows[r], rstart, rcontrib, roff, emat, evec, ne);
+      rhs[r] = row_accumulate<LD, SYM, SOA, OffT>(r, vals + rows[r], mbase, rcontrib, roff, emat, evec, ne);
     }
   }
+}
+
+// Slice-major gather maps from the row-major ones: one thread per slice
+// row (padding rows of the last slice included), entry k of slice row l
+// written at base[s] + 64 k + l, rows shorter than the slice's longest
+// padded with code -1 and zero positions.
+__global__ void k_slice_count(int n, int nslice, const int *__restrict__ rs, int *cnt) {
+  for (int sl = blockIdx.x * blockDim.x + threadIdx.x; sl < nslice; sl += gridDim.x * blockDim.x) {
+    int m = 0;
+    for (int r = sl * 64; r < min(n, sl * 64 + 64); ++r) m = max(m, rs[r + 1] - rs[r]);
+    cnt[sl] = m;
+  }
+}
+
+template <typename OffT>
+__global__ void k_slice_maps(int n, int nslice, int ld, const int *__restrict__ rs, const int *__restrict__ rc,
+                             const OffT *__restrict__ ro, const int *__restrict__ base, int *c2, OffT *o2) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nslice * 64; i += gridDim.x * blockDim.x) {
+    const int sl = i >> 6, l = i & 63;
+    const int b = base[sl], cnt = (base[sl + 1] - b) >> 6;
+    const int k0 = i < n ? rs[i] : 0, len = i < n ? rs[i + 1] - k0 : 0;
+    for (int k = 0; k < cnt; ++k) {
+      const int64_t d = (int64_t)b + (int64_t)k * 64 + l;
+      c2[d] = k < len ? rc[k0 + k] : -1;
+      for (int j = 0; j < ld; ++j) o2[d * ld + j] = k < len ? ro[(int64_t)(k0 + k) * ld + j] : (OffT)0;
+    }
+  }
+}
+
+int slice_maps(ehip_system *s) {
+  const int n = s->n, S = (n + 63) / 64, ld = s->npe * s->dofs;
+  const size_t ob = (size_t)s->row_off_bytes;
+  int nent = 0;
+  EHIP_HIP_TRY(hipMemcpy(&nent, s->rhs_start + n, sizeof(int), hipMemcpyDeviceToHost));
+  int *cnt = nullptr;
+  int rc;
+  if ((rc = dmalloc(s, (void **)&cnt, sizeof(int) * S))) return rc;
+  hipLaunchKernelGGL(k_slice_count, dim3(std::min((S + 255) / 256, 4096)), dim3(256), 0, s->stream, n, S,
+                     s->rhs_start, cnt);
+  std::vector<int> h((size_t)S), base((size_t)S + 1);
+  EHIP_HIP_TRY(hipMemcpyAsync(h.data(), cnt, sizeof(int) * S, hipMemcpyDeviceToHost, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  dfree(s, cnt, sizeof(int) * S);
+  int64_t tot = 0;
+  for (int sl = 0; sl < S; ++sl) {
+    base[sl] = (int)tot;
+    tot += 64 * (int64_t)h[sl];
+    if (tot * ld >= (int64_t)INT32_MAX) {
+      set_error("ehip_system_from_mesh: padded gather maps exceed 2^31 entries");
+      return EHIP_ECAP;
+    }
+  }
+  base[S] = (int)tot;
+  int *mb = nullptr, *c2 = nullptr;
+  void *o2 = nullptr;
+  if ((rc = dmalloc(s, (void **)&mb, sizeof(int) * (S + 1))) ||
+      (rc = dmalloc(s, (void **)&c2, sizeof(int) * std::max<int64_t>(tot, 1))) ||
+      (rc = dmalloc(s, &o2, std::max<size_t>((size_t)tot * ld * ob, 8)))) {
+    dfree(s, mb, sizeof(int) * (S + 1));
+    dfree(s, c2, sizeof(int) * std::max<int64_t>(tot, 1));
+    return rc;
+  }
+  EHIP_HIP_TRY(hipMemcpyAsync(mb, base.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice, s->stream));
+  const int grid = std::min((S * 64 + 255) / 256, 16384);
+  if (ob == 1)
+    hipLaunchKernelGGL(k_slice_maps<uint8_t>, dim3(grid), dim3(256), 0, s->stream, n, S, ld, s->rhs_start,
+                       s->rhs_contrib, (const uint8_t *)s->row_off, mb, c2, (uint8_t *)o2);
+  else
+    hipLaunchKernelGGL(k_slice_maps<uint16_t>, dim3(grid), dim3(256), 0, s->stream, n, S, ld, s->rhs_start,
+                       s->rhs_contrib, (const uint16_t *)s->row_off, mb, c2, (uint16_t *)o2);
+  EHIP_HIP_TRY(hipGetLastError());
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  dfree(s, s->rhs_start, sizeof(int) * ((size_t)n + 1));
+  dfree(s, s->rhs_contrib, sizeof(int) * std::max(nent, 1));
+  dfree(s, s->row_off, (size_t)nent * ld * ob);
+  s->rhs_start = nullptr;
+  s->map_base = mb;
+  s->rhs_contrib = c2;
+  s->row_off = o2;
+  s->map_entries = tot;
+  return EHIP_OK;
 }
 
 struct DevTables {
@@ -681,11 +766,11 @@ static void row_gather_as(ehip_system *s) {
   const int grid = (std::min((s->n + BS - 1) / BS, 32768) + 7) / 8 * 8;  // a multiple of 8 (XcdWalk)
   if (s->row_off_bytes == 1)
     hipLaunchKernelGGL((k_row_gather<LD, SYM, SOA, uint8_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
-                       s->rows, s->rhs_start, s->rhs_contrib, (const uint8_t *)s->row_off, s->emat, s->evec,
+                       s->rows, s->map_base, s->rhs_contrib, (const uint8_t *)s->row_off, s->emat, s->evec,
                        s->nelem, s->vals, s->rhs);
   else
     hipLaunchKernelGGL((k_row_gather<LD, SYM, SOA, uint16_t, BS, CAP>), dim3(grid), dim3(BS), 0, s->stream, s->n,
-                       s->rows, s->rhs_start, s->rhs_contrib, (const uint16_t *)s->row_off, s->emat, s->evec,
+                       s->rows, s->map_base, s->rhs_contrib, (const uint16_t *)s->row_off, s->emat, s->evec,
                        s->nelem, s->vals, s->rhs);
 }
 
@@ -823,6 +908,10 @@ static int from_mesh_host(int nnodes, int nelem, int npe, const int *elements, i
   EHIP_HIP_TRY(hipMemcpy(s->rhs_contrib, rcontrib.data(), sizeof(int) * rcontrib.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->row_off, s->row_off_bytes == 1 ? (const void *)roff8.data() : (const void *)roff.data(),
                          off_bytes, hipMemcpyHostToDevice));
+  if ((rc = slice_maps(s))) {
+    ehip_system_free(s);
+    return rc;
+  }
   *out = s;
   return EHIP_OK;
 }
@@ -859,7 +948,7 @@ static int from_mesh_device(int nnodes, int nelem, int npe, const int *elements,
                             optimize, &max_len);
   if (rc) { ehip_system_free(s); return rc; }
   s->pattern_on_device = true;
-  if ((rc = ehip_internal_upload_pattern(s)) ||
+  if ((rc = ehip_internal_upload_pattern(s)) || (rc = slice_maps(s)) ||
       (rc = dmalloc(s, (void **)&s->emat, sizeof(double) * (size_t)nelem * ld * ld)) ||
       (rc = dmalloc(s, (void **)&s->evec, sizeof(double) * (size_t)nelem * ld))) {
     ehip_system_free(s);

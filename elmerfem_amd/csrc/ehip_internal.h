@@ -176,10 +176,16 @@ struct ehip_system {
   // row-owner gather maps (from ehip_system_from_mesh)
   int nelem = 0, npe = 0;
   int *elements = nullptr;        // nelem*npe, 0-based node ids (device)
-  int *rhs_start = nullptr;       // n+1: each row's entries in rhs_contrib
-  int *rhs_contrib = nullptr;     // nelem*ld: e*ld + the row's local index, element order
+  // Built row-major (rhs_start: each row's entries, in element order), then
+  // made slice-major for the gather (slice_maps): entry k of row 64 s + l
+  // at map_base[s] + 64 k + l, each slice padded to its longest row with
+  // code -1; rhs_start is freed then.
+  int *rhs_start = nullptr;       // n+1 (row-major build only)
+  int *map_base = nullptr;        // nslice+1 (slice-major)
+  int *rhs_contrib = nullptr;     // per entry: e*ld + the row's local index (-1: padding)
   void *row_off = nullptr;        // per entry, ld column positions in the row (uint8 or uint16)
   int row_off_bytes = 0;          // 1 or 2
+  int64_t map_entries = 0;        // entries of rhs_contrib
   double *emat = nullptr;         // nelem*ld*ld element matrices (row-major, or packed upper triangles)
   double *evec = nullptr;         // nelem*ld element force vectors
   // assembly inputs kept on the device between calls (no allocation, free or
@@ -286,6 +292,9 @@ int max_grid();
 // (pattern_gpu.hip): fills s->rows/cols/diag/rhs_start/rhs_contrib/row_off
 // on the device, s->n, s->nnz, the bandwidth fields and the 1-based host
 // copies h_rows/h_cols/h_diag/h_perm; el0_dev: 0-based connectivity.
+// Row-major gather maps (rhs_start, rhs_contrib, row_off) into the
+// slice-major layout the gather reads (map_base, rhs_contrib, row_off).
+int slice_maps(ehip_system *s);
 int build_device_pattern(ehip_system *s, int nnodes, int nelem, int npe, const int *el0_dev, int nbnd, int npe_b,
                          const int *bel, int dofs, int optimize, int *max_len);
 

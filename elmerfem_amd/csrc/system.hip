@@ -920,7 +920,7 @@ int ehip_system_free(ehip_system *s) {
                   s->sell_c16,   s->sell_base,   s->sell64,    s->sell32,     s->sell2_off,
                   s->sell2_c16,  s->sell_row,    s->sellb_c16, s->sellb_base, s->bw64,
                   s->bw32,       s->sho_off,     s->sho_tab,   s->sho_mask,   s->sho64,
-                  s->sho32};
+                  s->sho32,      s->map_base};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : s->prof_ev)
