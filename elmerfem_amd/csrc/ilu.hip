@@ -77,19 +77,47 @@ __global__ void k_ilu_invdiag(int n, const int *__restrict__ diag, double *ilu) 
   }
 }
 
+// s - sum over k of a(k) x(cols(k)) for the row part [j0, j1), subtracted
+// in order k = j0, j0+1, ... (the reference's loop): the part's values and
+// column indices are loaded kLuU at a time before any x gather, and the x
+// gathers before any subtraction, so a level costs a few memory latencies
+// instead of two per entry (indices past the part are clamped into it and
+// their terms skipped).
+constexpr int kLuU = 16;
+__device__ __forceinline__ double lu_part(double s, int j0, int j1, const int *__restrict__ cols,
+                                          const double *__restrict__ ilu, const double *x) {
+#pragma clang fp contract(off)
+  const int len = j1 - j0;
+  for (int k0 = 0; k0 < len; k0 += kLuU) {
+    double a[kLuU];
+    int c[kLuU];
+#pragma unroll
+    for (int u = 0; u < kLuU; ++u) {
+      const int j = j0 + min(k0 + u, len - 1);
+      a[u] = ilu[j];
+      c[u] = cols[j];
+    }
+    double xv[kLuU];
+#pragma unroll
+    for (int u = 0; u < kLuU; ++u) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < kLuU; ++u) {
+      const double t = s - a[u] * xv[u];
+      s = k0 + u < len ? t : s;
+    }
+  }
+  return s;
+}
+
 // Forward level: dst(i) = src(i) - sum over the lower part of L(i,j) dst(j)
 __global__ __launch_bounds__(kIluBlock) void k_lu_fwd(const int *__restrict__ rows, const int *__restrict__ cols,
                                                     const int *__restrict__ diag, const double *__restrict__ ilu,
                                                     const double *src, double *dst, const int *__restrict__ order,
                                                     int begin, int end) {
-#pragma clang fp contract(off)
   const int t = begin + blockIdx.x * kIluBlock + threadIdx.x;
   if (t >= end) return;
   const int i = order[t];
-  double s = src[i];
-  const int di = diag[i];
-  for (int j = rows[i]; j < di; ++j) s = s - ilu[j] * dst[cols[j]];
-  dst[i] = s;
+  dst[i] = lu_part(src[i], rows[i], diag[i], cols, ilu, dst);
 }
 
 // Backward level: dst(i) = U(i,i)^-1 (dst(i) - sum over the upper part of U(i,j) dst(j))
@@ -100,10 +128,8 @@ __global__ __launch_bounds__(kIluBlock) void k_lu_bwd(const int *__restrict__ ro
   const int t = begin + blockIdx.x * kIluBlock + threadIdx.x;
   if (t >= end) return;
   const int i = order[t];
-  double s = dst[i];
-  const int di = diag[i], r1 = rows[i + 1];
-  for (int j = di + 1; j < r1; ++j) s = s - ilu[j] * dst[cols[j]];
-  dst[i] = ilu[di] * s;
+  const int di = diag[i];
+  dst[i] = ilu[di] * lu_part(dst[i], di + 1, rows[i + 1], cols, ilu, dst);
 }
 
 // ---------------------------------------------------------------- host: pattern and levels
