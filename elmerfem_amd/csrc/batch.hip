@@ -78,6 +78,13 @@ struct SlotPtrs {
   const double *du;   // the scaling's D when the unscaled residual is tested, else null
   int n, ck0, nck, method;
   int nb;             // the system's chunk size (nonzeros); a longer chunk is one long row
+  // the system's slice-shared offset layout (spmv_sho_body) when it has one
+  // and uses it: SpMV ops then take one thread per row instead of staging
+  // the chunk's products (null: the CRS stream)
+  const int *sho_off, *sho_tab;
+  const unsigned *sho_mask;
+  const double *sv64;
+  const float *sv32;
 };
 
 struct BSys {
@@ -175,6 +182,57 @@ __device__ __forceinline__ void chunk_spmv(const Op &op, typename Op::Ctx &cx, c
   lds_barrier();
 }
 
+// SpMV of a chunk on the slice-shared offset layout: thread t takes rows
+// r0 + t, r0 + t + BS, ... -- the rows, and so the partial sums, that the
+// staged path gives it -- and sums each row in CRS order as spmv_sho_body
+// does (FP64: the row's own nonzeros by its mask; FP32: the absent entries
+// as +0 times x), so the results are the staged path's bits.
+template <class Op, int BS>
+__device__ __forceinline__ void chunk_spmv_sho(const Op &op, typename Op::Ctx &cx, const SlotPtrs &sp,
+                                               const Chunk &ck, typename Op::T (&acc)[Op::NV > 0 ? Op::NV : 1]) {
+  using T = typename Op::T;
+  using VT = typename std::remove_cv<typename std::remove_pointer<decltype(op.vals(sp))>::type>::type;
+  constexpr bool kF32 = sizeof(VT) == 4;
+  constexpr int U = 9;
+  const VT *vals;
+  if constexpr (kF32) vals = sp.sv32; else vals = sp.sv64;
+  const T *xin = op.xin(cx);
+  const int n = sp.n;
+  for (int r = ck.r0 + (int)threadIdx.x; r < ck.r1; r += BS) {
+    const int sl = r >> 6, l = r & 63;
+    const unsigned m = kF32 ? 0u : sp.sho_mask[r];
+    const int p0 = sp.sho_off[sl];
+    const int wd = (sp.sho_off[sl + 1] - p0) >> 6;
+    const int *t = sp.sho_tab + (p0 >> 6);
+    T sum = (T)0;
+    for (int k0 = 0; k0 < wd; k0 += U) {
+      VT vv[U];
+      int cc[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = min(k0 + u, wd - 1);
+        vv[u] = ntload(vals + p0 + kk * 64 + l);
+        cc[u] = min(max(r + t[kk], 0), n - 1);
+      }
+      T xv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) xv[u] = xin[cc[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma clang fp contract(off)
+        const T tv = sum + (T)vv[u] * xv[u];
+        if constexpr (kF32) {
+          sum = (k0 + u < wd) ? tv : sum;
+        } else {
+          const int k = k0 + u;
+          sum = (k < 32 && ((m >> (k & 31)) & 1u)) ? tv : sum;
+        }
+      }
+    }
+    op.row(cx, r, sum, acc);
+  }
+}
+
 // One generic kernel: every work item whose slot the op accepts is handled by
 // one workgroup -- SpMV ops stage the chunk's products in LDS and give each
 // row its left-to-right sum, vector ops visit the rows -- and the op's finish
@@ -226,7 +284,9 @@ __global__ __launch_bounds__(BS) void bk_run(BatchArgs a, Op op) {
         // the branch-free stream's depth follows the chunk's length, not the
         // kernel's capacity NB: every depth gives each row the same
         // left-to-right sum, so a system's results depend on its chunking only
-        if (len > sp.nb) {
+        if (sp.sho_off) {
+          chunk_spmv_sho<Op, BS>(op, cx, sp, ck, acc);
+        } else if (len > sp.nb) {
           // one row longer than the system's chunk size: strided partial sums
           for (int r = ck.r0; r < ck.r1; ++r) {
             T part = (T)0;
@@ -1047,6 +1107,7 @@ struct ehip_batch {
     int nb = 2048;               // chunk size (nonzeros) of this system
     int64_t first_launch = 0;    // the first replay whose results include this system
     bool asm_check = false;      // its assembly's degenerate-element count is read at retirement
+    bool sho = false;            // SpMV on the system's slice-shared offset layout
   };
   std::vector<Slot> slot;
   bool graph_ok = false;
@@ -1262,6 +1323,10 @@ static int upload_worklist(ehip_batch *b) {
       sp.du = (b->p.true_residual_stop && b->p.scaling) ? s->dscale : nullptr;
       sp.n = s->n; sp.method = b->method;
       sp.nb = sl.nb;
+      if (sl.sho) {
+        sp.sho_off = s->sho_off; sp.sho_tab = s->sho_tab; sp.sho_mask = s->sho_mask;
+        sp.sv64 = s->sho64; sp.sv32 = s->sho32;
+      }
       nbmax = std::max(nbmax, sl.nb);
       sp.ck0 = (int)all.size();
       sp.nck = (int)sl.ck.size();
@@ -1324,7 +1389,11 @@ static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
 }
 
 static int64_t slot_bytes(const ehip_system *s, bool mixed) {
-  return (int64_t)s->n * (8 * (NV64 + 1) + (mixed ? 4 * (NV32 + 1) : 0)) + (mixed && !s->vals32 ? 4 * s->nnz : 0);
+  int64_t copies = 0;  // slice-shared offset value copies not allocated yet
+  if (s->sho_ok && s->fmt64 == 2 && (!mixed || s->fmt32 == 2))
+    copies = (s->sho64 ? 0 : 8 * s->sho_nnz) + (mixed && !s->sho32 ? 4 * s->sho_nnz : 0);
+  return (int64_t)s->n * (8 * (NV64 + 1) + (mixed ? 4 * (NV32 + 1) : 0)) + (mixed && !s->vals32 ? 4 * s->nnz : 0) +
+         copies;
 }
 
 // Scales the system, prepares its vectors and state, and puts it into a free
@@ -1387,6 +1456,10 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
     touch_vals(s, false, true);
   }
   hipLaunchKernelGGL(k_fill_x, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, 1.0e-8);
+  // the slice-shared offset copies of the scaled values, when the system
+  // uses that layout (gathered on the system's stream, as the scaling)
+  sl.sho = s->sho_ok && s->fmt64 == 2 && (!b->mixed || s->fmt32 == 2);
+  if (sl.sho && (!sho_values64(s) || (b->mixed && !sho_values32(s)))) return EHIP_ENOMEM;
   // Jacobi divisors
   const int nvi = b->method == EHIP_CG ? 6 : 10;
   double *dg64 = b->mixed ? sl.w64 + (size_t)n : sl.w64 + (size_t)(b->method == EHIP_CG ? 5 : 8) * n;

@@ -361,8 +361,8 @@ static int64_t sell_slices(int n, const int *rows, const std::vector<int> *ord, 
 
 // SELL-64 with slice-shared offsets (spmv_sho_body), made the default
 // format of both precisions when every slice's union of column offsets has
-// at most 32 entries and the layout holds at most 1/8 more entries than the
-// SELL-64 layout it replaces (sell_tot).
+// at most 32 entries and the layout holds at most 1/4 more entries than the
+// nonzeros or at most 1/8 more than the SELL-64 layout (sell_tot, 0: none).
 static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_tot) {
   const int n = s->n;
   const int S = (n + 63) / 64;
@@ -378,7 +378,8 @@ static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_to
     w8 += (len[sl] + 7) / 8 * 8 - len[sl];
     w9 += (len[sl] + 8) / 9 * 9 - len[sl];
   }
-  if (tot > sell_tot + sell_tot / 8 || tot >= (int64_t)INT32_MAX || tot == 0) return EHIP_OK;
+  const int64_t limit = std::max(s->nnz + s->nnz / 4 + 64, sell_tot + sell_tot / 8);
+  if (tot > limit || tot >= (int64_t)INT32_MAX || tot == 0) return EHIP_OK;
   off[S] = (int)tot;
   std::vector<int> ctab((size_t)(tot / 64));
   for (int sl = 0; sl < S; ++sl)
@@ -394,6 +395,7 @@ static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_to
   // entries per lane in flight: 8 or 9, whichever loads fewer clamped repeats
   s->sho_u = w9 < w8 ? 9 : 8;
   s->sho_nnz = tot;
+  s->nslice = S;
   s->sho_ok = true;
   s->fmt64 = s->fmt32 = 2;
   return EHIP_OK;
@@ -558,7 +560,6 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   s->sell_nnz = tot;
   s->sell_ok = true;
   s->fmt64 = s->fmt32 = 1;
-  if (!mapped && !blk) return build_sho(s, c0, tot);
   return EHIP_OK;
 }
 
@@ -870,6 +871,9 @@ int ehip_internal_upload_pattern(ehip_system *s) {
   if ((rc = plan_spmv(s, s->plan32, 256, 2048))) return rc;
   if ((rc = build_cols16(s, tmp))) return rc;
   if ((rc = build_sell(s, tmp))) return rc;
+  // node-blocked patterns keep the node-blocked SELL-64 (their rows use
+  // D x (neighbour nodes) offsets, beyond a 32-entry list)
+  if (!s->sellb_d && (rc = build_sho(s, tmp, s->sell_ok ? s->sell_nnz : 0))) return rc;
   return ensure_state(s);
 }
 
