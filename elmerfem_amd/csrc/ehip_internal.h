@@ -137,6 +137,9 @@ struct ehip_system {
   // an absent entry holding value 0; bit k of sho_mask[row] says entry k is
   // one of the row's nonzeros (the row's nonzeros in CRS order)
   bool sho_ok = false;
+  // the SELL-64 and 16-bit CRS column layouts are built when first selected,
+  // tuned or queried when the shared-offset layout was built (the default)
+  bool layouts_deferred = false;
   int sho_u = 8;                      // entries per lane in flight: 8 or 9
   int64_t sho_nnz = 0;
   int *sho_off = nullptr;             // nslice + 1

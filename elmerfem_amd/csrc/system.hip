@@ -378,7 +378,7 @@ static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_to
     w8 += (len[sl] + 7) / 8 * 8 - len[sl];
     w9 += (len[sl] + 8) / 9 * 9 - len[sl];
   }
-  const int64_t limit = std::max(s->nnz + s->nnz / 4 + 64, sell_tot + sell_tot / 8);
+  const int64_t limit = std::max<int64_t>(s->nnz + s->nnz / 4 + 64, sell_tot + sell_tot / 8);
   if (tot > limit || tot >= (int64_t)INT32_MAX || tot == 0) return EHIP_OK;
   off[S] = (int)tot;
   std::vector<int> ctab((size_t)(tot / 64));
@@ -869,12 +869,34 @@ int ehip_internal_upload_pattern(ehip_system *s) {
   EHIP_HIP_TRY(hipMemsetAsync(s->x, 0, sizeof(double) * n, s->stream));
   if ((rc = plan_spmv(s, s->plan64, 256, 2048))) return rc;
   if ((rc = plan_spmv(s, s->plan32, 256, 2048))) return rc;
+  // the slice-shared offset layout first; when it exists the other SpMV
+  // layouts are built only if a caller asks for them (ensure_layouts).
+  // Multi-DOF patterns rarely qualify (a 3-DOF row alone has 81 offsets)
+  // and keep the node-blocked SELL-64.
+  if ((rc = build_sho(s, tmp, 0))) return rc;
+  if (s->sho_ok) {
+    s->layouts_deferred = true;
+  } else {
+    if ((rc = build_cols16(s, tmp))) return rc;
+    if ((rc = build_sell(s, tmp))) return rc;
+  }
+  return ensure_state(s);
+}
+
+// Builds the SELL-64 and 16-bit column layouts deferred at pattern upload;
+// the formats in use stay as they are.
+static int ensure_layouts(ehip_system *s) {
+  if (!s->layouts_deferred) return EHIP_OK;
+  s->layouts_deferred = false;
+  std::vector<int> tmp((size_t)std::max<int64_t>(s->nnz, s->n + 1));
+  for (int64_t j = 0; j < s->nnz; ++j) tmp[j] = s->h_cols[j] - 1;
+  const int f64 = s->fmt64, f32 = s->fmt32;
+  int rc;
   if ((rc = build_cols16(s, tmp))) return rc;
   if ((rc = build_sell(s, tmp))) return rc;
-  // node-blocked patterns keep the node-blocked SELL-64 (their rows use
-  // D x (neighbour nodes) offsets, beyond a 32-entry list)
-  if (!s->sellb_d && (rc = build_sho(s, tmp, s->sell_ok ? s->sell_nnz : 0))) return rc;
-  return ensure_state(s);
+  s->fmt64 = f64;
+  s->fmt32 = f32;
+  return EHIP_OK;
 }
 
 int ehip_system_create(int n, int64_t nnz, const int *rows, const int *cols, const int *diag,
@@ -1111,6 +1133,7 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
 int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll,
                           int *pair_unroll) {
   if (!s) return EHIP_EARG;
+  if (int rc = ensure_layouts(s)) return rc;
   if (!s->sell_ok) {
     set_error("ehip_spmv_sell_layout: this pattern has no SELL-64 layout (padding above the limit)");
     return EHIP_EUNSUP;
@@ -1136,6 +1159,8 @@ int ehip_spmv_shared_layout(ehip_system *s, int *lane_unroll, int64_t *entries) 
 int ehip_spmv_set_format(ehip_system *s, int precision, int format) {
   if (!s || (format != EHIP_SPMV_CSR && format != EHIP_SPMV_SELL && format != EHIP_SPMV_SELL_SHARED))
     return EHIP_EARG;
+  if (format != EHIP_SPMV_SELL_SHARED)
+    if (int rc = ensure_layouts(s)) return rc;
   if (format == EHIP_SPMV_SELL && !s->sell_ok) {
     set_error("ehip_spmv_set_format: this pattern has no SELL-64 layout (padding above the limit)");
     return EHIP_EUNSUP;
@@ -1151,6 +1176,7 @@ int ehip_spmv_set_format(ehip_system *s, int precision, int format) {
 
 int ehip_spmv_set_schedule(ehip_system *s, int precision, int block, int nnz_per_block) {
   if (!s) return EHIP_EARG;
+  if (int rc = ensure_layouts(s)) return rc;
   bool known = false;
 #define EHIP_CFG_KNOWN(BS, NB) known = known || (block == BS && nnz_per_block == NB);
   EHIP_SPMV_CONFIGS(EHIP_CFG_KNOWN)
@@ -1172,6 +1198,7 @@ int ehip_spmv_tune(ehip_system *s, int precision) {
   static const int cfg[][2] = {EHIP_SPMV_CONFIGS(EHIP_CFG_ENTRY)};
 #undef EHIP_CFG_ENTRY
   const int ncfg = (int)(sizeof(cfg) / sizeof(cfg[0]));
+  if (int rc = ensure_layouts(s)) return rc;
   SpmvPlan &plan = precision == 0 ? s->plan64 : s->plan32;
   int &fmt = precision == 0 ? s->fmt64 : s->fmt32;
   double best = 1e30;
