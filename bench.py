@@ -314,7 +314,7 @@ def run_serve(args):
                      "latency_ms_p99": 1000.0 * float(np.percentile(lat_s, 99))}
         log("one-at-a-time %s: %.1f solves/s" % (name, seq[name]["solves_per_s"]))
     srv = Server(kw, precision=_lib.MIXED_IR, max_slots=args.slots, mem_budget=int(args.budget_gib * (1 << 30)),
-                 true_residual_stop=True)
+                 check_every=args.check_every, true_residual_stop=True)
     if args.lane_workers > 0:
         srv.set_lane(args.lane_dofs, args.lane_workers)
     R = args.requests
@@ -413,6 +413,7 @@ def run_serve(args):
                       "n_min": int(min(sizes)), "n_max": int(max(sizes)),
                       "dofs_mean": float(np.mean([r[0].n for r in sysrep])), "slots": args.slots,
                       "lane_min_dofs": args.lane_dofs, "lane_workers": args.lane_workers,
+                      "check_every": args.check_every,
                       "mem_budget_gib": args.budget_gib, "seed": args.seed, "tol": 1e-10},
            "arrival_rate_per_s": rate, "served_rate_under_load": R / el2,
            "latency_ms": {"p50": 1000.0 * float(np.percentile(lat, 50)),
@@ -573,6 +574,8 @@ def main():
                                                                    "takes the concurrent lane")
     ap.add_argument("--lane-workers", type=int, default=4, help="serve form: concurrent-lane workers (0: off)")
     ap.add_argument("--slots", type=int, default=64, help="serve form: server slots")
+    ap.add_argument("--check-every", type=int, default=0,
+                    help="serve form: iterations per engine replay (0: the library's 8)")
     ap.add_argument("--budget-gib", type=float, default=64.0, help="serve form: HBM admission budget")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

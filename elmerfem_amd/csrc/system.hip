@@ -362,7 +362,8 @@ static int64_t sell_slices(int n, const int *rows, const std::vector<int> *ord, 
 // SELL-64 with slice-shared offsets (spmv_sho_body), made the default
 // format of both precisions when every slice's union of column offsets has
 // at most 32 entries and the layout holds at most 1/4 more entries than the
-// nonzeros or at most 1/8 more than the SELL-64 layout (sell_tot, 0: none).
+// nonzeros (or at most 1/8 more than a SELL-64 layout of sell_tot entries,
+// when one is given).
 static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_tot) {
   const int n = s->n;
   const int S = (n + 63) / 64;

@@ -294,8 +294,11 @@ int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
  *                  column indices are streamed -- built, and made the
  *                  default of both precisions, when every slice's rows use
  *                  at most 32 distinct offsets and the layout holds at most
- *                  1/8 more entries than SELL-64 (stencil-like patterns:
- *                  meshes of regular topology in natural node order).
+ *                  1/4 more entries than the nonzeros (stencil-like
+ *                  patterns: meshes of regular topology in natural node
+ *                  order); the SELL-64 and CSR-stream layouts of such a
+ *                  pattern are then built when first selected, tuned or
+ *                  queried.
  * Every format sums each row in CRS order; FP64 results are the same bits in
  * all of them. */
 #define EHIP_SPMV_CSR 0
