@@ -718,7 +718,8 @@ def main():
     # in every workgroup, two launches per iteration) unless EHIP_CG_FOLD=0
     epi = "EpiCgFast1<float>" if os.environ.get("EHIP_CG_FOLD", "1")[:1] == "0" else "EpiCgFold<float>"
     if c32["format"] == "sell_shared":
-        kname = "k_spmv_epi_sho<float,256,%d,%s>" % (sys_.spmv_shared_layout()["lane_unroll"], epi)
+        lay = sys_.spmv_shared_layout()
+        kname = "k_spmv_epi_sho<float,256,%d,%d,%s>" % (lay["lane_unroll"], max(1, lay["node_block"]), epi)
     elif c32["format"] == "sell":
         lay = sys_.spmv_sell_layout()
         if lay["node_block"]:

@@ -97,7 +97,7 @@ SIGNATURES = {
     "ehip_spmv_bench": (C.c_int, [_vp, C.c_int, C.c_int, _vp]),
     "ehip_spmv_config": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ehip_spmv_set_format": (C.c_int, [_vp, C.c_int, C.c_int]),
-    "ehip_spmv_shared_layout": (C.c_int, [_vp, _ip, C.POINTER(C.c_int64)]),
+    "ehip_spmv_shared_layout": (C.c_int, [_vp, _ip, _ip, C.POINTER(C.c_int64)]),
     "ehip_spmv_sell_layout": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     "ehip_spmv_set_schedule": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ehip_copy_bench": (C.c_int, [_i64, C.c_int, _vp, _vp]),

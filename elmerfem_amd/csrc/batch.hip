@@ -1390,7 +1390,7 @@ static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
 
 static int64_t slot_bytes(const ehip_system *s, bool mixed) {
   int64_t copies = 0;  // slice-shared offset value copies not allocated yet
-  if (s->sho_ok && s->fmt64 == 2 && (!mixed || s->fmt32 == 2))
+  if (s->sho_ok && s->sho_d == 1 && s->fmt64 == 2 && (!mixed || s->fmt32 == 2))
     copies = (s->sho64 ? 0 : 8 * s->sho_nnz) + (mixed && !s->sho32 ? 4 * s->sho_nnz : 0);
   return (int64_t)s->n * (8 * (NV64 + 1) + (mixed ? 4 * (NV32 + 1) : 0)) + (mixed && !s->vals32 ? 4 * s->nnz : 0) +
          copies;
@@ -1458,7 +1458,7 @@ static int engine_admit(ehip_batch *b, ehip_system *s) {
   hipLaunchKernelGGL(k_fill_x, dim3(vec_grid(n)), dim3(kVecBlock), 0, s->stream, n, s->x, 1.0e-8);
   // the slice-shared offset copies of the scaled values, when the system
   // uses that layout (gathered on the system's stream, as the scaling)
-  sl.sho = s->sho_ok && s->fmt64 == 2 && (!b->mixed || s->fmt32 == 2);
+  sl.sho = s->sho_ok && s->sho_d == 1 && s->fmt64 == 2 && (!b->mixed || s->fmt32 == 2);
   if (sl.sho && (!sho_values64(s) || (b->mixed && !sho_values32(s)))) return EHIP_ENOMEM;
   // Jacobi divisors
   const int nvi = b->method == EHIP_CG ? 6 : 10;

@@ -140,7 +140,11 @@ struct ehip_system {
   // the SELL-64 and 16-bit CRS column layouts are built when first selected,
   // tuned or queried when the shared-offset layout was built (the default)
   bool layouts_deferred = false;
-  int sho_u = 8;                      // entries per lane in flight: 8 or 9
+  int sho_u = 8;                      // entries (node blocks) per lane in flight: 8 or 9
+  // node-block size: 1, or D when the pattern is made of D x D node blocks
+  // (offsets and mask bits per block; entry (k, d) of a lane at
+  // sho_off[s] + 64 (k D + d) + lane, the list at sho_tab[sho_off[s] / (64 D)])
+  int sho_d = 1;
   int64_t sho_nnz = 0;
   int *sho_off = nullptr;             // nslice + 1
   int *sho_tab = nullptr;             // sho_nnz / 64

@@ -1308,13 +1308,13 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ ro
   spmv_sell_body<T, T, BS, U>(rows, n, rmap, sc, off, vals, xin, nslice, epi);
 }
 
-template <typename T, int BS, int U, class Epi>
+template <typename T, int BS, int U, int D, class Epi>
 __global__ __launch_bounds__(BS) void k_spmv_epi_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                                      const unsigned *__restrict__ mask, const T *__restrict__ vals,
                                                      const T *__restrict__ xin, int nslice, Epi epi,
                                                      const KStateT<T> *st) {
   if (st && solve_done(st)) return;
-  spmv_sho_body<T, T, BS, U>(n, off, tab, mask, vals, xin, nslice, epi);
+  spmv_sho_body<T, T, BS, U, D>(n, off, tab, mask, vals, xin, nslice, epi);
 }
 
 template <int BS, int U, class Epi>
@@ -1384,14 +1384,14 @@ static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi
              s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
 }
 
-template <typename T, int U, class Epi>
+template <typename T, int U, int D, class Epi>
 static void launch_epi_sho(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
   const int g = epi_sell_grid<T>(s);
   if (s->prof_start)
-    hipExtLaunchKernelGGL((k_spmv_epi_sho<T, 256, U, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
+    hipExtLaunchKernelGGL((k_spmv_epi_sho<T, 256, U, D, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
                           s->prof_stop, 0, s->n, s->sho_off, s->sho_tab, s->sho_mask, sv, xin, s->nslice, e, st);
   else
-    launch_k((k_spmv_epi_sho<T, 256, U, Epi>), dim3(g), dim3(256), s->stream, s->n, s->sho_off, s->sho_tab,
+    launch_k((k_spmv_epi_sho<T, 256, U, D, Epi>), dim3(g), dim3(256), s->stream, s->n, s->sho_off, s->sho_tab,
              s->sho_mask, sv, xin, s->nslice, e, st);
 }
 
@@ -1427,8 +1427,10 @@ static void launch_epi(ehip_system *s, const SpmvPlan &p, const T *vals, const T
     const T *sv;
     if constexpr (F32) sv = sho_values32(s); else sv = sho_values64(s);
     if (missing(sv)) return;
-    if (s->sho_u == 9) launch_epi_sho<T, 9>(s, sv, xin, e, st);
-    else launch_epi_sho<T, 8>(s, sv, xin, e, st);
+    if (s->sho_d == 3) launch_epi_sho<T, 9, 3>(s, sv, xin, e, st);
+    else if (s->sho_d == 2) launch_epi_sho<T, 9, 2>(s, sv, xin, e, st);
+    else if (s->sho_u == 9) launch_epi_sho<T, 9, 1>(s, sv, xin, e, st);
+    else launch_epi_sho<T, 8, 1>(s, sv, xin, e, st);
     return;
   }
   if (active_fmt<T>(s) == 1) {

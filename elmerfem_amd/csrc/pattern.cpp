@@ -319,9 +319,11 @@ int build_row_maps(const HostPattern &P, int nelem, int npe, const int *el, std:
 
 // Per slice, the row offsets are merged into the running union one row at a
 // time (rows of one slice mostly repeat the union, so a merge is a short
-// walk); a second walk per row sets its mask bits.
+// walk); a second walk per row sets its mask bits.  With D > 1 a row's
+// entries are taken D at a time (whole D-column node blocks) and the
+// offsets are node offsets: column / D - row / D.
 bool slice_offsets(int n, const int *rows1, const int *c0, int max_len, std::vector<int> &len,
-                   std::vector<int> &tab, std::vector<unsigned> &mask) {
+                   std::vector<int> &tab, std::vector<unsigned> &mask, int D) {
   if (max_len > 32) max_len = 32;
   const int S = (n + 63) / 64;
   len.assign((size_t)S, 0);
@@ -336,14 +338,14 @@ bool slice_offsets(int n, const int *rows1, const int *c0, int max_len, std::vec
     const int r0 = sl * 64, r1 = std::min(n, r0 + 64);
     bool ok = true;
     for (int r = r0; r < r1 && ok; ++r) {
-      const int j1 = rows1[r + 1] - 1;
+      const int j1 = rows1[r + 1] - 1, node = r / D;
       int a = 0, b = rows1[r] - 1, nt = 0;
       while (a < nu || b < j1) {
         const int va = a < nu ? u[a] : INT32_MAX;
-        const int vb = b < j1 ? c0[b] - r : INT32_MAX;
+        const int vb = b < j1 ? c0[b] / D - node : INT32_MAX;
         const int v = std::min(va, vb);
         a += va == v;
-        b += vb == v;
+        b += vb == v ? D : 0;
         if (nt == max_len) {
           ok = false;
           break;
@@ -362,8 +364,9 @@ bool slice_offsets(int n, const int *rows1, const int *c0, int max_len, std::vec
     for (int r = r0; r < r1; ++r) {
       unsigned m = 0;
       int k = 0;
-      for (int b = rows1[r] - 1; b < rows1[r + 1] - 1; ++b) {
-        const int v = c0[b] - r;
+      const int node = r / D;
+      for (int b = rows1[r] - 1; b < rows1[r + 1] - 1; b += D) {
+        const int v = c0[b] / D - node;
         while (u[k] != v) ++k;  // v is in the union, which is sorted
         m |= 1u << k;
       }

@@ -40,9 +40,11 @@ int build_row_maps(const HostPattern &P, int nelem, int npe, const int *el, std:
 // slice s, the sorted union of (column - row) over its rows at
 // tab[s * max_len ..] (len[s] entries), and for every row the bit mask of
 // the union entries that are its nonzeros (bit k: entry k).  rows1 is the
-// 1-based row pointer, c0 the 0-based columns, each row sorted.  Returns
-// false as soon as one slice's union exceeds max_len (<= 32) offsets.
+// 1-based row pointer, c0 the 0-based columns, each row sorted.  With D > 1
+// (rows made of whole D-column node blocks, D unknowns per node) the
+// offsets and mask bits are per block: column / D - row / D.  Returns false
+// as soon as one slice's union exceeds max_len (<= 32) offsets.
 bool slice_offsets(int n, const int *rows1, const int *c0, int max_len, std::vector<int> &len,
-                   std::vector<int> &tab, std::vector<unsigned> &mask);
+                   std::vector<int> &tab, std::vector<unsigned> &mask, int D = 1);
 
 }  // namespace ehip

@@ -460,8 +460,11 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
 // bit-identical to the reference loop whatever x holds; FP32 adds the
 // absent entries too, as +0 times x at a column inside the matrix (clamped),
 // which leaves a finite sum unchanged (the same results as the other
-// layouts).
-template <typename V, typename A, int BS, int U, class Epi>
+// layouts).  D > 1: rows made of whole D x D node blocks (D unknowns per
+// node, interleaved); the list holds node offsets, block k of row r covers
+// columns D (r / D + tab[k]) + d, d < D, its values at off[s] + 64 (k D + d)
+// + lane, and mask bit k stands for the whole block.
+template <typename V, typename A, int BS, int U, int D, class Epi>
 __device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                               const unsigned *__restrict__ mask, const V *__restrict__ vals,
                                               const A *__restrict__ x, int nslice, Epi &epi) {
@@ -484,31 +487,35 @@ __device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off
     const int rl = min(r, n - 1);
     const typename Epi::Pre pre = epi.load(rl);
     const unsigned m = kF32 || r >= n ? 0u : mask[rl];
+    const int node = rl / D;
     const int p0 = off[sl];
-    const int wd = (off[sl + 1] - p0) >> 6;  // slice width
-    const int *__restrict__ t = tab + (p0 >> 6);
+    const int wd = (off[sl + 1] - p0) / (64 * D);  // slice width in blocks
+    const int *__restrict__ t = tab + p0 / (64 * D);
     A sum = (A)0;
     for (int k0 = 0; k0 < wd; k0 += U) {
-      V vv[U];
+      V vv[U][D];
       int cc[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int kk = min(k0 + u, wd - 1);  // clamped into the slice
-        vv[u] = ntload(vals + p0 + kk * 64 + lane);
-        cc[u] = min(max(rl + t[kk], 0), n - 1);
-      }
-      A xv[U];
+        cc[u] = min(max(D * (node + t[kk]), 0), n - D);
 #pragma unroll
-      for (int u = 0; u < U; ++u) xv[u] = x[cc[u]];
+        for (int d = 0; d < D; ++d) vv[u][d] = ntload(vals + p0 + (kk * D + d) * 64 + lane);
+      }
+      A xv[U][D];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int d = 0; d < D; ++d) xv[u][d] = x[cc[u] + d];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
+        const int k = k0 + u;  // mask bits stop below wd <= 32
+        const bool on = kF32 ? k < wd : (k < 32 && ((m >> (k & 31)) & 1u));
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
 #pragma clang fp contract(off)
-        const A tv = sum + (A)vv[u] * xv[u];
-        if constexpr (kF32) {
-          sum = (k0 + u < wd) ? tv : sum;  // only the clamped repeats are skipped
-        } else {
-          const int k = k0 + u;  // mask bits stop below wd <= 32
-          sum = (k < 32 && ((m >> (k & 31)) & 1u)) ? tv : sum;
+          const A tv = sum + (A)vv[u][d] * xv[u][d];
+          sum = on ? tv : sum;
         }
       }
     }

@@ -171,12 +171,18 @@ __global__ __launch_bounds__(BS) void k_spmv_sellb(const int *__restrict__ rows,
   spmv_sellb_body<V, A, BS, U, D>(rows, n, rmap, cb, nbase, off, vals, x, nslice, epi);
 }
 
-template <typename V, typename A, int BS, int U>
+template <typename V, typename A, int BS, int U, int D>
 __global__ __launch_bounds__(BS) void k_spmv_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                                  const unsigned *__restrict__ mask, const V *__restrict__ vals,
                                                  const A *__restrict__ x, int nslice, A *y) {
   EpiStore<A> epi{y};
-  spmv_sho_body<V, A, BS, U>(n, off, tab, mask, vals, x, nslice, epi);
+  spmv_sho_body<V, A, BS, U, D>(n, off, tab, mask, vals, x, nslice, epi);
+}
+
+template <typename V, typename A, int U, int D>
+static void launch_spmv_sho(ehip_system *s, const V *sv, const A *x, A *y) {
+  hipLaunchKernelGGL((k_spmv_sho<V, A, 256, U, D>), dim3(sell_grid(s)), dim3(256), 0, s->stream, s->n, s->sho_off,
+                     s->sho_tab, s->sho_mask, sv, x, s->nslice, y);
 }
 
 template <typename V, typename A>
@@ -190,13 +196,10 @@ static int launch_spmv(ehip_system *s, const SpmvPlan &p, const V *vals, const A
     const V *sv;
     if constexpr (F32) sv = sho_values32(s); else sv = sho_values64(s);
     if (!sv) return EHIP_ENOMEM;
-    const int g = sell_grid(s);
-    if (s->sho_u == 9)
-      hipLaunchKernelGGL((k_spmv_sho<V, A, 256, 9>), dim3(g), dim3(256), 0, s->stream, s->n, s->sho_off, s->sho_tab,
-                         s->sho_mask, sv, x, s->nslice, y);
-    else
-      hipLaunchKernelGGL((k_spmv_sho<V, A, 256, 8>), dim3(g), dim3(256), 0, s->stream, s->n, s->sho_off, s->sho_tab,
-                         s->sho_mask, sv, x, s->nslice, y);
+    if (s->sho_d == 3) launch_spmv_sho<V, A, 9, 3>(s, sv, x, y);
+    else if (s->sho_d == 2) launch_spmv_sho<V, A, 9, 2>(s, sv, x, y);
+    else if (s->sho_u == 9) launch_spmv_sho<V, A, 9, 1>(s, sv, x, y);
+    else launch_spmv_sho<V, A, 8, 1>(s, sv, x, y);
     return hipGetLastError() == hipSuccess ? EHIP_OK : EHIP_EHIP;
   }
   if ((F32 ? s->fmt32 : s->fmt64) == 1 && s->sell_ok) {
@@ -364,27 +367,29 @@ static int64_t sell_slices(int n, const int *rows, const std::vector<int> *ord, 
 // at most 32 entries and the layout holds at most 1/4 more entries than the
 // nonzeros (or at most 1/8 more than a SELL-64 layout of sell_tot entries,
 // when one is given).
-static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_tot) {
+static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_tot, int D = 1) {
   const int n = s->n;
   const int S = (n + 63) / 64;
   constexpr int kMax = 32;
   std::vector<int> len, tab;
   std::vector<unsigned> mask;
-  if (!slice_offsets(n, s->h_rows.data(), c0.data(), kMax, len, tab, mask)) return EHIP_OK;
+  if (!slice_offsets(n, s->h_rows.data(), c0.data(), kMax, len, tab, mask, D)) return EHIP_OK;
+  // slice s: 64 lanes x len[s] blocks of D entries
   std::vector<int> off((size_t)S + 1);
   int64_t tot = 0, w8 = 0, w9 = 0;
   for (int sl = 0; sl < S; ++sl) {
     off[sl] = (int)std::min<int64_t>(tot, INT32_MAX);
-    tot += 64 * (int64_t)len[sl];
+    tot += 64 * (int64_t)D * len[sl];
     w8 += (len[sl] + 7) / 8 * 8 - len[sl];
     w9 += (len[sl] + 8) / 9 * 9 - len[sl];
   }
   const int64_t limit = std::max<int64_t>(s->nnz + s->nnz / 4 + 64, sell_tot + sell_tot / 8);
   if (tot > limit || tot >= (int64_t)INT32_MAX || tot == 0) return EHIP_OK;
   off[S] = (int)tot;
-  std::vector<int> ctab((size_t)(tot / 64));
+  std::vector<int> ctab((size_t)(tot / (64 * D)));
   for (int sl = 0; sl < S; ++sl)
-    std::copy(tab.begin() + (size_t)sl * kMax, tab.begin() + (size_t)sl * kMax + len[sl], ctab.begin() + off[sl] / 64);
+    std::copy(tab.begin() + (size_t)sl * kMax, tab.begin() + (size_t)sl * kMax + len[sl],
+              ctab.begin() + off[sl] / (64 * D));
   int rc;
   if ((rc = dmalloc(s, (void **)&s->sho_off, sizeof(int) * off.size())) ||
       (rc = dmalloc(s, (void **)&s->sho_tab, sizeof(int) * ctab.size())) ||
@@ -393,8 +398,9 @@ static int build_sho(ehip_system *s, const std::vector<int> &c0, int64_t sell_to
   EHIP_HIP_TRY(hipMemcpy(s->sho_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->sho_tab, ctab.data(), sizeof(int) * ctab.size(), hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->sho_mask, mask.data(), sizeof(unsigned) * (size_t)n, hipMemcpyHostToDevice));
-  // entries per lane in flight: 8 or 9, whichever loads fewer clamped repeats
+  // entries (blocks) per lane in flight: 8 or 9, whichever loads fewer clamped repeats
   s->sho_u = w9 < w8 ? 9 : 8;
+  s->sho_d = D;
   s->sho_nnz = tot;
   s->nslice = S;
   s->sho_ok = true;
@@ -717,10 +723,36 @@ __global__ __launch_bounds__(256) void k_sho_gather(int n, int nslice, const int
   }
 }
 
+// The node-blocked form (D > 1): a slice's CRS range is up to D times larger,
+// so lanes read their rows' blocks from global memory directly.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sho_gather_blocks(int n, int nslice, int D, const int *__restrict__ rows,
+                                                           const unsigned *__restrict__ mask,
+                                                           const int *__restrict__ off, const T *__restrict__ in,
+                                                           T *__restrict__ out) {
+  const int lane = (int)threadIdx.x & 63;
+  for (int sl = blockIdx.x * 4 + ((int)threadIdx.x >> 6); sl < nslice; sl += gridDim.x * 4) {
+    const int r = sl * 64 + lane;
+    const unsigned m = r < n ? mask[r] : 0u;
+    int j = r < n ? rows[r] : 0;
+    const int p0 = off[sl];
+    const int wb = (off[sl + 1] - p0) / (64 * D);
+    for (int k = 0; k < wb; ++k) {
+      const bool on = (m >> k) & 1u;
+      for (int d = 0; d < D; ++d) out[p0 + ((int64_t)k * D + d) * 64 + lane] = on ? in[j + d] : (T)0;
+      j += on ? D : 0;
+    }
+  }
+}
+
 template <typename T>
 static void sho_gather(ehip_system *s, const T *in, T *out) {
-  hipLaunchKernelGGL((k_sho_gather<T>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n, s->nslice, s->rows,
-                     s->sho_mask, s->sho_off, in, out);
+  if (s->sho_d > 1)
+    hipLaunchKernelGGL((k_sho_gather_blocks<T>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n, s->nslice,
+                       s->sho_d, s->rows, s->sho_mask, s->sho_off, in, out);
+  else
+    hipLaunchKernelGGL((k_sho_gather<T>), dim3(gather_grid(s)), dim3(256), 0, s->stream, s->n, s->nslice, s->rows,
+                       s->sho_mask, s->sho_off, in, out);
 }
 
 const double *sho_values64(ehip_system *s) {
@@ -880,6 +912,8 @@ int ehip_internal_upload_pattern(ehip_system *s) {
   } else {
     if ((rc = build_cols16(s, tmp))) return rc;
     if ((rc = build_sell(s, tmp))) return rc;
+    // node-interleaved D-DOF patterns: the same with one offset per node block
+    if (s->sellb_d > 1 && (rc = build_sho(s, tmp, s->sell_nnz, s->sellb_d))) return rc;
   }
   return ensure_state(s);
 }
@@ -1146,13 +1180,14 @@ int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int
   return EHIP_OK;
 }
 
-int ehip_spmv_shared_layout(ehip_system *s, int *lane_unroll, int64_t *entries) {
+int ehip_spmv_shared_layout(ehip_system *s, int *node_block, int *lane_unroll, int64_t *entries) {
   if (!s) return EHIP_EARG;
   if (!s->sho_ok) {
     set_error("ehip_spmv_shared_layout: this pattern has no slice-shared offset layout");
     return EHIP_EUNSUP;
   }
-  if (lane_unroll) *lane_unroll = s->sho_u;
+  if (node_block) *node_block = s->sho_d > 1 ? s->sho_d : 0;
+  if (lane_unroll) *lane_unroll = s->sho_d > 1 ? 9 : s->sho_u;
   if (entries) *entries = s->sho_nnz;
   return EHIP_OK;
 }

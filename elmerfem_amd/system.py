@@ -260,11 +260,13 @@ class System:
         return {"sort_window": w.value, "node_block": b.value, "lane_unroll": u.value, "pair_unroll": pu.value}
 
     def spmv_shared_layout(self):
-        """{"lane_unroll": entries each lane loads at once, "entries": stored entries}
-        of the slice-shared offset layout (EhipError when the pattern has none)."""
-        u, e = C.c_int(), C.c_int64()
-        _lib.check(_lib.load().ehip_spmv_shared_layout(self._h, C.byref(u), C.byref(e)), "ehip_spmv_shared_layout")
-        return {"lane_unroll": u.value, "entries": e.value}
+        """{"node_block": 0 or DOFs per node block, "lane_unroll": entries (blocks) each lane
+        loads at once, "entries": stored values} of the slice-shared offset layout
+        (EhipError when the pattern has none)."""
+        b, u, e = C.c_int(), C.c_int(), C.c_int64()
+        _lib.check(_lib.load().ehip_spmv_shared_layout(self._h, C.byref(b), C.byref(u), C.byref(e)),
+                   "ehip_spmv_shared_layout")
+        return {"node_block": b.value, "lane_unroll": u.value, "entries": e.value}
 
     def spmv_config(self, precision=0):
         f, b, nb, ch, ib = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()

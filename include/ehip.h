@@ -298,7 +298,9 @@ int ehip_copy_bench(int64_t bytes, int reps, double *gbps, double *us_per_call);
  *                  patterns: meshes of regular topology in natural node
  *                  order); the SELL-64 and CSR-stream layouts of such a
  *                  pattern are then built when first selected, tuned or
- *                  queried.
+ *                  queried.  Node-interleaved 2- or 3-DOF patterns get
+ *                  the same with one node offset and one mask bit per
+ *                  D x D block (ehip_spmv_shared_layout's node_block).
  * Every format sums each row in CRS order; FP64 results are the same bits in
  * all of them. */
 #define EHIP_SPMV_CSR 0
@@ -329,12 +331,14 @@ int ehip_spmv_config(ehip_system *s, int precision, int *format, int *block, int
 int ehip_spmv_sell_layout(ehip_system *s, int *sort_window, int *node_block, int *lane_unroll,
                           int *pair_unroll);
 /* Shape of the slice-shared offset layout (EHIP_EUNSUP when the pattern has
- * none): lane_unroll = entries each lane loads before using them (8, or 9
- * when that wastes fewer loads on the slice widths); entries = stored
- * entries, absent ones and the padding lanes of the last slice included
- * (64 x the slices' offset-list lengths).  Any pointer may be NULL.  No
- * counterpart in the reference. */
-int ehip_spmv_shared_layout(ehip_system *s, int *lane_unroll, int64_t *entries);
+ * none): node_block = D when the pattern is made of whole D x D blocks of
+ * node-interleaved unknowns (D = 2, 3) and the lists hold node offsets,
+ * one mask bit per block, else 0; lane_unroll = entries (blocks, with
+ * node_block) each lane loads before using them (8, or 9 when that wastes
+ * fewer loads on the slice widths; 9 blocks with node_block); entries =
+ * stored values, absent ones and the padding lanes of the last slice
+ * included.  Any pointer may be NULL.  No counterpart in the reference. */
+int ehip_spmv_shared_layout(ehip_system *s, int *node_block, int *lane_unroll, int64_t *entries);
 /* Selects the format for one precision (EHIP_EUNSUP: the pattern has no
  * layout of that format). */
 int ehip_spmv_set_format(ehip_system *s, int precision, int format);

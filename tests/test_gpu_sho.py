@@ -145,18 +145,18 @@ def test_fp32_products_identical_to_sell(ehip):
     s.free()
 
 
-@pytest.mark.parametrize("kind", ["scrambled", "elastic"])
+@pytest.mark.parametrize("kind", ["scrambled", "hex27_elastic"])
 def test_absent_without_a_stencil(ehip, kind):
     """Randomly relabelled nodes (a slice's rows then use far more than 32
-    offsets) and 3-DOF elasticity (81 offsets per row; node-blocked SELL-64
-    instead) have no shared-offset layout."""
+    offsets) and Hex27 elasticity (up to 125 neighbour nodes per row;
+    node-blocked SELL-64 instead) have no shared-offset layout."""
     if kind == "scrambled":
         m = M.hex8_cube(6)
         relabel = (np.random.default_rng(9).permutation(m.nnodes) + 1).astype(np.int32)
         m.elements, m.bnd_elements = relabel[m.elements - 1], relabel[m.bnd_elements - 1]
         s = System.from_mesh(m, dofs=1, optimize_bandwidth=False)
     else:
-        s, _ = W.build_system(W.elasticity_hex(4))
+        s, _ = W.build_system(W.elasticity_hex(3, quadratic=True))
     assert s.spmv_config(0)["format"] in ("sell", "csr")
     with pytest.raises(_lib.EhipError):
         s.spmv_set_format(SHARED)
@@ -194,3 +194,44 @@ def test_batch_engine_identical_to_crs_stream(ehip, precision, mode, method):
         assert np.array_equal(a, b)
     for s in systems:
         s.free()
+
+
+def test_node_blocks_default_and_bit_exact(ehip):
+    """3-DOF Hex8 elasticity in natural node order: the shared-offset layout
+    with one node offset and one mask bit per 3 x 3 block is the default,
+    and its FP64 products are the reference loop's bits, as are those of the
+    node-blocked SELL-64 and the CSR stream."""
+    w = W.elasticity_hex(16)
+    s, _ = W.build_system(w)
+    assert s.spmv_config(0)["format"] == "sell_shared"
+    assert s.spmv_shared_layout()["node_block"] == 3
+    _, rows, cols, _ = s.pattern()
+    vals, rhs = s.values()
+    u = np.random.default_rng(5).standard_normal(s.n)
+    ref = oracle_spmv(rows, cols, vals, u)
+    for fmt in (SHARED, SELL, CSR):
+        s.spmv_set_format(fmt)
+        assert np.array_equal(s.spmv(u), ref), fmt
+    assert s.spmv_sell_layout()["node_block"] == 3
+    s.free()
+
+
+@pytest.mark.parametrize("precision,mode", [(_lib.FP64, _lib.FAST), (_lib.MIXED_IR, _lib.FAST)])
+def test_node_block_solves_identical_to_sell(ehip, precision, mode):
+    """Same row sums, same workgroups as the node-blocked SELL-64: the
+    elasticity solve gives the same iterations and solution bits."""
+    w = W.elasticity_hex(16)
+    s, _ = W.build_system(w)
+    _, rows, cols, _ = s.pattern()
+    vals, rhs = s.values()
+    out = []
+    for fmt in (SHARED, SELL):
+        s.set_values(vals, rhs)
+        both(s, fmt)
+        x, rep = s.solve(w["kw"], precision=precision, mode=mode, true_residual_stop=True)
+        assert rep["info"] == 1 and rep["res_true"] <= 1e-10, rep
+        ax = oracle_spmv(rows, cols, vals, x)
+        assert np.linalg.norm(rhs - ax) / np.linalg.norm(rhs) <= 1e-10
+        out.append((x, rep["iters"], rep["outer_iters"]))
+    assert out[0][1:] == out[1][1:] and np.array_equal(out[0][0], out[1][0])
+    s.free()
