@@ -15,11 +15,11 @@ are timed beside it, as are GPU assembly and the SpMV kernels.
     python bench.py [--steps K] [--warmup W] [--n 128]
 
 Prints one JSON line.  value = DOFs x inner iterations / solve time.  After
-the headline (outside its timed steps) short runs of the batching form (64
-convection-diffusion systems) and the serving form (16 meshes, 200
-requests) add `batch_summary` and `serve_summary` to the same line
-(`--no-summaries` leaves them out; `--form batch` / `--form serve` run
-those forms at full size).
+the headline (outside its timed steps) a short run of the batching form (64
+convection-diffusion systems) and the serving form at its full size (64
+meshes, 1000 requests) add `batch_summary` and `serve_summary` to the same
+line (`--no-summaries` leaves them out; `--form batch` runs the batching
+form at its full size, 512 systems).
 """
 from __future__ import annotations
 
@@ -781,11 +781,11 @@ def main():
     }
     sys_.free()
     if not args.skip_forms and not args.no_summaries:
-        # short runs of the batching and serving forms (their own commands,
-        # --form batch / --form serve, run them at full size)
+        # a short run of the batching form (--form batch runs it at full
+        # size) and the serving form at the size of --form serve
         import copy
         a = copy.copy(args)
-        a.systems, a.pool, a.requests = 64, 16, 200
+        a.systems = 64
         log("batch summary (%d systems) ..." % a.systems)
         b = run_batch(a)
         f = b["forms"]
