@@ -494,7 +494,6 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
   }
   if ((rc = dmalloc(s, (void **)&s->sell_off, sizeof(int) * off.size()))) return rc;
   EHIP_HIP_TRY(hipMemcpy(s->sell_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
-  int64_t tot64 = tot;  // FP64 values: unpacked, or pair-packed below
   if (blk) {
     // one node offset per block: block k of slice row 64 sl + l at off[sl]/D + 64 k + l
     std::vector<unsigned short> cb((size_t)(tot / D));
@@ -556,13 +555,13 @@ static int build_sell(ehip_system *s, const std::vector<int> &c0) {
         w7 += (np + 6) / 7 * 7 - np;
       }
       s->sell2_u = w7 < w4 ? 7 : kSell2U;
-      tot64 = tot2;
     }
   } else {
     if ((rc = dmalloc(s, (void **)&s->sell_c32, sizeof(int) * tot))) return rc;
     EHIP_HIP_TRY(hipMemcpy(s->sell_c32, col.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
   }
-  (void)tot64;  // the FP64 values copy is allocated at first use (sell_values64)
+  // the values copies (FP64 plain or pair-packed, FP32) are allocated at
+  // first use (sell_values64 / sell_values32)
   s->nslice = S;
   s->sell_nnz = tot;
   s->sell_ok = true;
