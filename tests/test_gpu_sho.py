@@ -235,3 +235,56 @@ def test_node_block_solves_identical_to_sell(ehip, precision, mode):
         out.append((x, rep["iters"], rep["outer_iters"]))
     assert out[0][1:] == out[1][1:] and np.array_equal(out[0][0], out[1][0])
     s.free()
+
+
+def _kron_five_point(nx, ny, B):
+    """The 5-point Laplacian of an nx x ny grid (natural order) times a D x D
+    SPD block B, as 1-based CRS with node-interleaved unknowns: SPD."""
+    D = B.shape[0]
+    rows, cols, vals = [1], [], []
+    for j in range(ny):
+        for i in range(nx):
+            node = j * nx + i
+            nb = [(node - nx, -1.0)] * (j > 0) + [(node - 1, -1.0)] * (i > 0) + [(node, 4.2)] + \
+                 [(node + 1, -1.0)] * (i < nx - 1) + [(node + nx, -1.0)] * (j < ny - 1)
+            for d in range(D):
+                for m, lv in nb:
+                    for e in range(D):
+                        cols.append(D * m + e + 1)
+                        vals.append(lv * B[d, e])
+                rows.append(len(cols) + 1)
+    rows = np.array(rows, np.int32)
+    cols = np.array(cols, np.int32)
+    n = nx * ny * D
+    diag = np.array([rows[r] + np.nonzero(cols[rows[r] - 1:rows[r + 1] - 1] == r + 1)[0][0] for r in range(n)],
+                    np.int32)
+    return rows, cols, diag, np.array(vals)
+
+
+def test_two_dof_node_blocks(ehip):
+    """2-DOF node blocks (D = 2): the shared-offset layout per block gives the
+    reference loop's FP64 bits, and a mixed CG solve is bit-identical to the
+    one on the node-blocked SELL-64."""
+    B = np.array([[2.0, 0.5], [0.5, 1.0]])
+    rows, cols, diag, vals = _kron_five_point(60, 45, B)
+    n = len(rows) - 1
+    s = System.from_csr(rows, cols, diag)
+    s.set_values(vals, np.ones(n))
+    assert s.spmv_config(0)["format"] == "sell_shared"
+    assert s.spmv_shared_layout()["node_block"] == 2
+    u = np.random.default_rng(8).standard_normal(n)
+    ref = oracle_spmv(rows, cols, vals, u)
+    for fmt in (SHARED, SELL, CSR):
+        s.spmv_set_format(fmt)
+        assert np.array_equal(s.spmv(u), ref), fmt
+    assert s.spmv_sell_layout()["node_block"] == 2
+    kw = _lib.Params(0, 1, 1e-10, 5000, 1, _lib.MIXED_IR, _lib.FAST, 0, 0, 0, 0, 0, 0, 1)
+    out = []
+    for fmt in (SHARED, SELL):
+        s.set_values(vals, np.ones(n))
+        both(s, fmt)
+        x, rep = s.solve(kw)
+        assert rep["info"] == 1 and rep["res_true"] <= 1e-10, rep
+        out.append((x, rep["iters"], rep["outer_iters"]))
+    assert out[0][1:] == out[1][1:] and np.array_equal(out[0][0], out[1][0])
+    s.free()
