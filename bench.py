@@ -719,7 +719,9 @@ def main():
     epi = "EpiCgFast1<float>" if os.environ.get("EHIP_CG_FOLD", "1")[:1] == "0" else "EpiCgFold<float>"
     if c32["format"] == "sell_shared":
         lay = sys_.spmv_shared_layout()
-        kname = "k_spmv_epi_sho<float,256,%d,%d,%s>" % (lay["lane_unroll"], max(1, lay["node_block"]), epi)
+        # values streamed non-temporally only when the FP32 copy exceeds the 256 MiB Infinity Cache
+        nt = "true" if 4 * lay["entries"] > (256 << 20) else "false"
+        kname = "k_spmv_epi_sho<float,256,%d,%d,%s,%s>" % (lay["lane_unroll"], max(1, lay["node_block"]), nt, epi)
     elif c32["format"] == "sell":
         lay = sys_.spmv_sell_layout()
         if lay["node_block"]:

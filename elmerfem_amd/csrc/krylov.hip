@@ -1308,13 +1308,13 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ ro
   spmv_sell_body<T, T, BS, U>(rows, n, rmap, sc, off, vals, xin, nslice, epi);
 }
 
-template <typename T, int BS, int U, int D, class Epi>
+template <typename T, int BS, int U, int D, bool NT, class Epi>
 __global__ __launch_bounds__(BS) void k_spmv_epi_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                                      const unsigned *__restrict__ mask, const T *__restrict__ vals,
                                                      const T *__restrict__ xin, int nslice, Epi epi,
                                                      const KStateT<T> *st) {
   if (st && solve_done(st)) return;
-  spmv_sho_body<T, T, BS, U, D>(n, off, tab, mask, vals, xin, nslice, epi);
+  spmv_sho_body<T, T, BS, U, D, NT>(n, off, tab, mask, vals, xin, nslice, epi);
 }
 
 template <int BS, int U, class Epi>
@@ -1384,15 +1384,21 @@ static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi
              s->sell_c32, s->sell_c16, s->sell_base, s->sell_off, sv, xin, s->nslice, e, st);
 }
 
-template <typename T, int U, int D, class Epi>
-static void launch_epi_sho(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
+template <typename T, int U, int D, bool NT, class Epi>
+static void launch_epi_sho_as(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
   const int g = epi_sell_grid<T>(s);
   if (s->prof_start)
-    hipExtLaunchKernelGGL((k_spmv_epi_sho<T, 256, U, D, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
+    hipExtLaunchKernelGGL((k_spmv_epi_sho<T, 256, U, D, NT, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
                           s->prof_stop, 0, s->n, s->sho_off, s->sho_tab, s->sho_mask, sv, xin, s->nslice, e, st);
   else
-    launch_k((k_spmv_epi_sho<T, 256, U, D, Epi>), dim3(g), dim3(256), s->stream, s->n, s->sho_off, s->sho_tab,
+    launch_k((k_spmv_epi_sho<T, 256, U, D, NT, Epi>), dim3(g), dim3(256), s->stream, s->n, s->sho_off, s->sho_tab,
              s->sho_mask, sv, xin, s->nslice, e, st);
+}
+
+template <typename T, int U, int D, class Epi>
+static void launch_epi_sho(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
+  if (values_nt((int64_t)sizeof(T) * s->sho_nnz)) launch_epi_sho_as<T, U, D, true>(s, sv, xin, e, st);
+  else launch_epi_sho_as<T, U, D, false>(s, sv, xin, e, st);
 }
 
 // the active format of one precision: 0 CSR stream, 1 SELL-64, 2 SELL-64

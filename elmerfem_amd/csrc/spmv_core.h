@@ -463,8 +463,11 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
 // layouts).  D > 1: rows made of whole D x D node blocks (D unknowns per
 // node, interleaved); the list holds node offsets, block k of row r covers
 // columns D (r / D + tab[k]) + d, d < D, its values at off[s] + 64 (k D + d)
-// + lane, and mask bit k stands for the whole block.
-template <typename V, typename A, int BS, int U, int D, class Epi>
+// + lane, and mask bit k stands for the whole block.  NT: the values are
+// streamed with non-temporal loads; without, they are cached, which keeps a
+// values copy smaller than the Infinity Cache resident there from one SpMV
+// to the next (values_nt picks per system).
+template <typename V, typename A, int BS, int U, int D, bool NT, class Epi>
 __device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                               const unsigned *__restrict__ mask, const V *__restrict__ vals,
                                               const A *__restrict__ x, int nslice, Epi &epi) {
@@ -500,7 +503,7 @@ __device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off
         const int kk = min(k0 + u, wd - 1);  // clamped into the slice
         cc[u] = min(max(D * (node + t[kk]), 0), n - D);
 #pragma unroll
-        for (int d = 0; d < D; ++d) vv[u][d] = ntload(vals + p0 + (kk * D + d) * 64 + lane);
+        for (int d = 0; d < D; ++d) vv[u][d] = NT ? ntload(vals + p0 + (kk * D + d) * 64 + lane) : vals[p0 + (kk * D + d) * 64 + lane];
       }
       A xv[U][D];
 #pragma unroll

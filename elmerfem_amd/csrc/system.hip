@@ -171,18 +171,22 @@ __global__ __launch_bounds__(BS) void k_spmv_sellb(const int *__restrict__ rows,
   spmv_sellb_body<V, A, BS, U, D>(rows, n, rmap, cb, nbase, off, vals, x, nslice, epi);
 }
 
-template <typename V, typename A, int BS, int U, int D>
+template <typename V, typename A, int BS, int U, int D, bool NT>
 __global__ __launch_bounds__(BS) void k_spmv_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                                  const unsigned *__restrict__ mask, const V *__restrict__ vals,
                                                  const A *__restrict__ x, int nslice, A *y) {
   EpiStore<A> epi{y};
-  spmv_sho_body<V, A, BS, U, D>(n, off, tab, mask, vals, x, nslice, epi);
+  spmv_sho_body<V, A, BS, U, D, NT>(n, off, tab, mask, vals, x, nslice, epi);
 }
 
 template <typename V, typename A, int U, int D>
 static void launch_spmv_sho(ehip_system *s, const V *sv, const A *x, A *y) {
-  hipLaunchKernelGGL((k_spmv_sho<V, A, 256, U, D>), dim3(sell_grid(s)), dim3(256), 0, s->stream, s->n, s->sho_off,
-                     s->sho_tab, s->sho_mask, sv, x, s->nslice, y);
+  if (values_nt((int64_t)sizeof(V) * s->sho_nnz))
+    hipLaunchKernelGGL((k_spmv_sho<V, A, 256, U, D, true>), dim3(sell_grid(s)), dim3(256), 0, s->stream, s->n,
+                       s->sho_off, s->sho_tab, s->sho_mask, sv, x, s->nslice, y);
+  else
+    hipLaunchKernelGGL((k_spmv_sho<V, A, 256, U, D, false>), dim3(sell_grid(s)), dim3(256), 0, s->stream, s->n,
+                       s->sho_off, s->sho_tab, s->sho_mask, sv, x, s->nslice, y);
 }
 
 template <typename V, typename A>
