@@ -32,6 +32,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <condition_variable>
@@ -1116,6 +1117,7 @@ struct ehip_batch {
   int graph_iters = 0;
   int mixed = 0, method = 0;
   int fast = 0;                  // FP64: recurrence residual with true-residual restarts
+  int refine_every = 1;          // mixed: the refinement ops run every refine_every-th iteration
   int nbmax = 2048;              // largest chunk size in the work list (selects bk_run's capacity)
   int grid = 2048;               // workgroups of the multi-system launches
   // serving: admissions only enqueue (the worklist is rebuilt at the next
@@ -1172,6 +1174,7 @@ constexpr int kChunkSizes[3] = {1024, 2048, 4096};
 constexpr int kGrids[3] = {1024, 2048, 4096};
 constexpr int kDefaultNB = 2048;
 constexpr int kSizeClasses = 6;
+constexpr int kRefineEvery = 2;
 
 // Size class of a system by its nonzeros: < 2^15, 2^17, 2^19, 2^21, 2^23, more.
 static int size_class(int64_t nnz) {
@@ -1243,8 +1246,13 @@ static void launch_op(ehip_batch *b, const Op &op) {
   launch_k((bk_fin<Op>), dim3(fgrid), dim3(256), b->stream, args_of(b), op, b->capacity);
 }
 
-// one iteration of every active slot
-static void enqueue_iteration(ehip_batch *b) {
+// one iteration of every active slot.  Mixed precision: the refinement ops
+// (correction, FP64 residual, FP32 restart) are enqueued only when `refine`
+// is set; a slot whose inner solve ended in between waits in PH_CORR,
+// doing nothing, until the next iteration that has them.  Every op acts on
+// a slot only in its own phase and idle iterations count nowhere, so a
+// slot's arithmetic -- and its bits -- do not depend on where they fall.
+static void enqueue_iteration(ehip_batch *b, bool refine) {
   if (!b->mixed) {
     launch_op(b, OpInit{});
     launch_op(b, OpStart{{}, b->method});
@@ -1271,9 +1279,11 @@ static void enqueue_iteration(ehip_batch *b) {
       }
     }
   } else {
-    launch_op(b, OpCorr{});
-    launch_op(b, OpRes{});
-    launch_op(b, OpPrep{{}, b->method});
+    if (refine) {
+      launch_op(b, OpCorr{});
+      launch_op(b, OpRes{});
+      launch_op(b, OpPrep{{}, b->method});
+    }
     if (b->method == EHIP_CG) {
       launch_op(b, OpCgF1{});
       launch_op(b, OpCgF2{});
@@ -1299,7 +1309,7 @@ static int build_graph(ehip_batch *b, int iters) {
   EHIP_HIP_TRY(hipGraphCreate(&rec.graph, 0));
   b->graph = rec.graph;
   tl_graph_rec = &rec;
-  for (int k = 0; k < iters; ++k) enqueue_iteration(b);
+  for (int k = 0; k < iters; ++k) enqueue_iteration(b, k % b->refine_every == 0);
   tl_graph_rec = nullptr;
   EHIP_HIP_TRY(rec.err);
   EHIP_HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -1360,6 +1370,14 @@ static int upload_worklist(ehip_batch *b) {
   return EHIP_OK;
 }
 
+// Iterations between the mixed form's refinement points in the multi-system
+// launches (EHIP_BATCH_REFINE_EVERY, 1 to 8).
+static int refine_every() {
+  const char *v = std::getenv("EHIP_BATCH_REFINE_EVERY");
+  const int k = v ? std::atoi(v) : kRefineEvery;
+  return k < 1 ? 1 : (k > 8 ? 8 : k);
+}
+
 static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
   ehip_batch *b = new ehip_batch();
   EHIP_HIP_TRY(hipGetDevice(&b->dev));
@@ -1371,6 +1389,7 @@ static int engine_create(int capacity, const ehip_params *p, ehip_batch **out) {
   b->method = p->method;
   b->fast = !b->mixed && p->mode == EHIP_FAST;
   b->grid = sched_grid(b->mixed, b->method == EHIP_CG ? 0 : 1, b->fast);
+  b->refine_every = refine_every();
   EHIP_HIP_TRY(hipMalloc(&b->nchunks, sizeof(int)));
   EHIP_HIP_TRY(hipMalloc(&b->slots, sizeof(SlotPtrs) * capacity));
   EHIP_HIP_TRY(hipMalloc(&b->st, sizeof(BSys) * capacity));
@@ -1538,7 +1557,7 @@ static int engine_launch(ehip_batch *b) {
     }
     EHIP_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   } else {
-    for (int k = 0; k < iters; ++k) enqueue_iteration(b);
+    for (int k = 0; k < iters; ++k) enqueue_iteration(b, k % b->refine_every == 0);
     EHIP_HIP_TRY(take_launch_error());
   }
   EHIP_HIP_TRY(hipMemcpyAsync(b->h_active, b->active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
@@ -1820,7 +1839,9 @@ extern "C" int ehip_batch_solve(ehip_batch *bh, const ehip_params *p, ehip_repor
   if (!eng_idx.empty()) {
     // every replay runs graph_iters iterations of every live slot
     const int gi = p->check_every > 0 ? p->check_every : 8;
-    const int64_t cap = (int64_t)p->maxit * (p->precision == EHIP_MIXED_IR ? 2 : 1) / gi + 64;
+    // mixed: up to maxit inner steps, one refinement per step at worst, each
+    // waiting up to refine_every - 1 idle iterations
+    const int64_t cap = (int64_t)p->maxit * (p->precision == EHIP_MIXED_IR ? 1 + e->refine_every : 1) / gi + 64;
     int64_t replays = 0;
     for (;;) {
       if ((rc = engine_step(e))) { engine_free(e); return rc; }

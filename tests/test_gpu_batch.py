@@ -102,6 +102,28 @@ def test_alone_equals_in_mix(ehip):
         assert ra["iters"] == r["iters"] and ra["outer_iters"] == r["outer_iters"]
 
 
+@pytest.mark.parametrize("kind", ["tet4", "convdiff"])  # CG, BiCGStab
+def test_refine_period_keeps_bits(ehip, monkeypatch, kind):
+    """Mixed precision in the multi-system launches: the refinement ops run
+    every k-th iteration (EHIP_BATCH_REFINE_EVERY); a slot whose inner solve
+    ends in between only waits, so every k gives the same solutions, bit for
+    bit, and the same inner and outer iteration counts."""
+    ws = [make(kind, i) for i in range(5)]
+    kw = ws[0]["kw"]
+    runs = []
+    for k in ("1", "3"):
+        monkeypatch.setenv("EHIP_BATCH_REFINE_EVERY", k)
+        systems = [W.build_system(w, optimize_bandwidth=True)[0] for w in ws]
+        xs, reps, _ = solve_batch(systems, kw, precision=_lib.MIXED_IR, mode=_lib.FAST, true_residual_stop=True)
+        assert all(r["info"] == 1 for r in reps)
+        runs.append((xs, reps))
+    (xa, ra), (xb, rb) = runs
+    for w, x1, x3, r1, r3 in zip(ws, xa, xb, ra, rb):
+        assert np.array_equal(x1, x3)
+        assert r1["iters"] == r3["iters"] and r1["outer_iters"] == r3["outer_iters"]
+        assert host_residual(w, x3) <= 1e-10
+
+
 @pytest.mark.parametrize("precision,method", [(_lib.MIXED_IR, _lib.CG), (_lib.FP64, _lib.BICGSTAB)])
 def test_batch_concurrent_lane(ehip, precision, method):
     """A batch whose larger systems take the concurrent lane: those results
