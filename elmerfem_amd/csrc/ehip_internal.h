@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -272,6 +273,29 @@ constexpr int kSellBU = 4;  // node blocks per lane in flight (node-blocked)
 // 460 MB FP64 copy: 106 us streamed, 112 us cached).
 constexpr int64_t kMallBytes = 256ll << 20;
 inline bool values_nt(int64_t bytes) { return bytes > kMallBytes; }
+// Of a larger copy, the slices before sho_nt_from are read with cached loads
+// -- about kMallWorkingBytes less the solver's vectors (six of n entries),
+// proportionally to the slice count -- and the rest streamed.  Measured
+// optimum of that budget: 128 MB for the FP64 copy at 128^3 (FP64 fast CG
+// 59.9 -> 54.2 ms), 96 MB for FP64 and >= 160 MB for FP32 on 96^3 Hex8
+// elasticity: each where cached values and vectors come to about 228 MB.
+// EHIP_MALL_CACHED_MB=m (read at every launch) sets the budget to m MB for
+// every copy, smaller ones included.
+constexpr int64_t kMallWorkingBytes = 228ll << 20;
+inline int64_t mall_cached_override() {
+  const char *v = std::getenv("EHIP_MALL_CACHED_MB");
+  return v && *v ? (int64_t)std::atoll(v) << 20 : (int64_t)-1;
+}
+// whether a copy of `bytes` takes the streamed-loads kernel
+inline bool sho_streamed(int64_t bytes) { return values_nt(bytes) || mall_cached_override() >= 0; }
+inline int sho_nt_from(int nslice, int64_t bytes, int64_t n, int elem) {
+  const int64_t o = mall_cached_override();
+  if (o < 0 && !values_nt(bytes)) return nslice;
+  const int64_t c = o >= 0 ? o : kMallWorkingBytes - 6 * n * elem;
+  if (c <= 0) return 0;
+  if (c >= bytes) return nslice;
+  return (int)((double)nslice * (double)c / (double)bytes);
+}
 
 inline int sell_grid(const ehip_system *s) {
   int g = (s->nslice + 3) / 4;

@@ -1311,10 +1311,10 @@ __global__ __launch_bounds__(BS) void k_spmv_epi_sell(const int *__restrict__ ro
 template <typename T, int BS, int U, int D, bool NT, class Epi>
 __global__ __launch_bounds__(BS) void k_spmv_epi_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                                      const unsigned *__restrict__ mask, const T *__restrict__ vals,
-                                                     const T *__restrict__ xin, int nslice, Epi epi,
-                                                     const KStateT<T> *st) {
+                                                     const T *__restrict__ xin, int nslice, int nt_from,
+                                                     Epi epi, const KStateT<T> *st) {
   if (st && solve_done(st)) return;
-  spmv_sho_body<T, T, BS, U, D, NT>(n, off, tab, mask, vals, xin, nslice, epi);
+  spmv_sho_body<T, T, BS, U, D, NT>(n, off, tab, mask, vals, xin, nslice, nt_from, epi);
 }
 
 template <int BS, int U, class Epi>
@@ -1387,17 +1387,19 @@ static void launch_epi_sell(ehip_system *s, const T *sv, const T *xin, const Epi
 template <typename T, int U, int D, bool NT, class Epi>
 static void launch_epi_sho_as(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
   const int g = epi_sell_grid<T>(s);
+  const int nt_from = sho_nt_from(s->nslice, (int64_t)sizeof(T) * s->sho_nnz, s->n, (int)sizeof(T));
   if (s->prof_start)
     hipExtLaunchKernelGGL((k_spmv_epi_sho<T, 256, U, D, NT, Epi>), dim3(g), dim3(256), 0, s->stream, s->prof_start,
-                          s->prof_stop, 0, s->n, s->sho_off, s->sho_tab, s->sho_mask, sv, xin, s->nslice, e, st);
+                          s->prof_stop, 0, s->n, s->sho_off, s->sho_tab, s->sho_mask, sv, xin, s->nslice, nt_from, e,
+                          st);
   else
     launch_k((k_spmv_epi_sho<T, 256, U, D, NT, Epi>), dim3(g), dim3(256), s->stream, s->n, s->sho_off, s->sho_tab,
-             s->sho_mask, sv, xin, s->nslice, e, st);
+             s->sho_mask, sv, xin, s->nslice, nt_from, e, st);
 }
 
 template <typename T, int U, int D, class Epi>
 static void launch_epi_sho(ehip_system *s, const T *sv, const T *xin, const Epi &e, const KStateT<T> *st) {
-  if (values_nt((int64_t)sizeof(T) * s->sho_nnz)) launch_epi_sho_as<T, U, D, true>(s, sv, xin, e, st);
+  if (sho_streamed((int64_t)sizeof(T) * s->sho_nnz)) launch_epi_sho_as<T, U, D, true>(s, sv, xin, e, st);
   else launch_epi_sho_as<T, U, D, false>(s, sv, xin, e, st);
 }
 

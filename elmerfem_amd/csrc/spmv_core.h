@@ -463,14 +463,16 @@ __device__ __forceinline__ void spmv_sellb_body(const int *__restrict__ rows, in
 // layouts).  D > 1: rows made of whole D x D node blocks (D unknowns per
 // node, interleaved); the list holds node offsets, block k of row r covers
 // columns D (r / D + tab[k]) + d, d < D, its values at off[s] + 64 (k D + d)
-// + lane, and mask bit k stands for the whole block.  NT: the values are
-// streamed with non-temporal loads; without, they are cached, which keeps a
-// values copy smaller than the Infinity Cache resident there from one SpMV
-// to the next (values_nt picks per system).
+// + lane, and mask bit k stands for the whole block.  NT: the values of
+// slices from nt_from on are streamed with non-temporal loads and those
+// before it read with cached loads; without NT every value load is cached.
+// Cached values stay in the Infinity Cache from one SpMV to the next, so a
+// values copy smaller than that cache is read cached whole, and a larger
+// one cached up to a budget and streamed after it (sho_nt_from picks).
 template <typename V, typename A, int BS, int U, int D, bool NT, class Epi>
 __device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                               const unsigned *__restrict__ mask, const V *__restrict__ vals,
-                                              const A *__restrict__ x, int nslice, Epi &epi) {
+                                              const A *__restrict__ x, int nslice, int nt_from, Epi &epi) {
   constexpr int WPB = BS / 64;
   constexpr bool kF32 = sizeof(V) == 4;
   constexpr int NV = Epi::NV;
@@ -483,46 +485,58 @@ __device__ __forceinline__ void spmv_sho_body(int n, const int *__restrict__ off
   const int lane = (int)threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const XcdWalk w((nslice + WPB - 1) / WPB);
-  for (int gidx = w.first; gidx < w.last; gidx += w.step) {
-    const int sl = gidx * WPB + wv;  // wave-uniform
-    if (sl >= nslice) continue;
-    const int r = sl * 64 + lane;  // >= n: a padding lane
-    const int rl = min(r, n - 1);
-    const typename Epi::Pre pre = epi.load(rl);
-    const unsigned m = kF32 || r >= n ? 0u : mask[rl];
-    const int node = rl / D;
-    const int p0 = off[sl];
-    const int wd = (off[sl + 1] - p0) / (64 * D);  // slice width in blocks
-    const int *__restrict__ t = tab + p0 / (64 * D);
-    A sum = (A)0;
-    for (int k0 = 0; k0 < wd; k0 += U) {
-      V vv[U][D];
-      int cc[U];
+  // the wave's slices in [s0, s1), values read with streamed (true) or
+  // cached loads; called once per kind, so each loop has registers of its own
+  auto walk = [&](auto streamed, int s0, int s1) {
+    constexpr bool kNt = decltype(streamed)::value;
+    for (int gidx = w.first; gidx < w.last; gidx += w.step) {
+      const int sl = gidx * WPB + wv;  // wave-uniform
+      if (sl < s0 || sl >= s1) continue;
+      const int r = sl * 64 + lane;  // >= n: a padding lane
+      const int rl = min(r, n - 1);
+      const typename Epi::Pre pre = epi.load(rl);
+      const unsigned m = kF32 || r >= n ? 0u : mask[rl];
+      const int node = rl / D;
+      const int p0 = off[sl];
+      const int wd = (off[sl + 1] - p0) / (64 * D);  // slice width in blocks
+      const int *__restrict__ t = tab + p0 / (64 * D);
+      A sum = (A)0;
+      for (int k0 = 0; k0 < wd; k0 += U) {
+        V vv[U][D];
+        int cc[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int kk = min(k0 + u, wd - 1);  // clamped into the slice
-        cc[u] = min(max(D * (node + t[kk]), 0), n - D);
+        for (int u = 0; u < U; ++u) {
+          const int kk = min(k0 + u, wd - 1);  // clamped into the slice
+          cc[u] = min(max(D * (node + t[kk]), 0), n - D);
 #pragma unroll
-        for (int d = 0; d < D; ++d) vv[u][d] = NT ? ntload(vals + p0 + (kk * D + d) * 64 + lane) : vals[p0 + (kk * D + d) * 64 + lane];
-      }
-      A xv[U][D];
+          for (int d = 0; d < D; ++d)
+            vv[u][d] = kNt ? ntload(vals + p0 + (kk * D + d) * 64 + lane) : vals[p0 + (kk * D + d) * 64 + lane];
+        }
+        A xv[U][D];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int d = 0; d < D; ++d) xv[u][d] = x[cc[u] + d];
+          for (int d = 0; d < D; ++d) xv[u][d] = x[cc[u] + d];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = k0 + u;  // mask bits stop below wd <= 32
-        const bool on = kF32 ? k < wd : (k < 32 && ((m >> (k & 31)) & 1u));
+        for (int u = 0; u < U; ++u) {
+          const int k = k0 + u;  // mask bits stop below wd <= 32
+          const bool on = kF32 ? k < wd : (k < 32 && ((m >> (k & 31)) & 1u));
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
+          for (int d = 0; d < D; ++d) {
 #pragma clang fp contract(off)
-          const A tv = sum + (A)vv[u][d] * xv[u][d];
-          sum = on ? tv : sum;
+            const A tv = sum + (A)vv[u][d] * xv[u][d];
+            sum = on ? tv : sum;
+          }
         }
       }
+      if (r < n) epi.row(r, sum, pre, acc);
     }
-    if (r < n) epi.row(r, sum, pre, acc);
+  };
+  if constexpr (NT) {
+    walk(std::false_type{}, 0, nt_from);
+    walk(std::true_type{}, nt_from, nslice);
+  } else {
+    walk(std::false_type{}, 0, nslice);
   }
   if constexpr (NV > 0) block_publish<A, NV>(acc, epi.partials, epi.stride);
 }

@@ -174,19 +174,21 @@ __global__ __launch_bounds__(BS) void k_spmv_sellb(const int *__restrict__ rows,
 template <typename V, typename A, int BS, int U, int D, bool NT>
 __global__ __launch_bounds__(BS) void k_spmv_sho(int n, const int *__restrict__ off, const int *__restrict__ tab,
                                                  const unsigned *__restrict__ mask, const V *__restrict__ vals,
-                                                 const A *__restrict__ x, int nslice, A *y) {
+                                                 const A *__restrict__ x, int nslice, int nt_from, A *y) {
   EpiStore<A> epi{y};
-  spmv_sho_body<V, A, BS, U, D, NT>(n, off, tab, mask, vals, x, nslice, epi);
+  spmv_sho_body<V, A, BS, U, D, NT>(n, off, tab, mask, vals, x, nslice, nt_from, epi);
 }
 
 template <typename V, typename A, int U, int D>
 static void launch_spmv_sho(ehip_system *s, const V *sv, const A *x, A *y) {
-  if (values_nt((int64_t)sizeof(V) * s->sho_nnz))
+  const int64_t bytes = (int64_t)sizeof(V) * s->sho_nnz;
+  const int nt_from = sho_nt_from(s->nslice, bytes, s->n, (int)sizeof(A));
+  if (sho_streamed(bytes))
     hipLaunchKernelGGL((k_spmv_sho<V, A, 256, U, D, true>), dim3(sell_grid(s)), dim3(256), 0, s->stream, s->n,
-                       s->sho_off, s->sho_tab, s->sho_mask, sv, x, s->nslice, y);
+                       s->sho_off, s->sho_tab, s->sho_mask, sv, x, s->nslice, nt_from, y);
   else
     hipLaunchKernelGGL((k_spmv_sho<V, A, 256, U, D, false>), dim3(sell_grid(s)), dim3(256), 0, s->stream, s->n,
-                       s->sho_off, s->sho_tab, s->sho_mask, sv, x, s->nslice, y);
+                       s->sho_off, s->sho_tab, s->sho_mask, sv, x, s->nslice, nt_from, y);
 }
 
 template <typename V, typename A>

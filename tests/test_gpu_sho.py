@@ -288,3 +288,38 @@ def test_two_dof_node_blocks(ehip):
         out.append((x, rep["iters"], rep["outer_iters"]))
     assert out[0][1:] == out[1][1:] and np.array_equal(out[0][0], out[1][0])
     s.free()
+
+
+@pytest.mark.parametrize("precision,mode", [(_lib.FP64, _lib.FAST), (_lib.MIXED_IR, _lib.FAST)])
+def test_cached_and_streamed_value_loads_same_bits(ehip, monkeypatch, precision, mode):
+    """A values copy larger than the Infinity Cache is read with cached loads
+    up to a budget of slices and streamed after it.  The budget changes only
+    the kind of load, never a sum's order: forced here on a 48^3 system
+    (24 MB of FP64 values) to none, a third and all of its slices
+    (EHIP_MALL_CACHED_MB 0, 8, unset), products and whole solves (fresh
+    systems, so no graph recorded under another budget is replayed) give
+    the same bits, the FP64 products the reference loop's."""
+    w = W.poisson_hex8(48, sigma=1.0)
+    u = np.random.default_rng(5).standard_normal(w["mesh"].nnodes)
+    got = []
+    for mb in (None, "8", "0"):
+        if mb is None:
+            monkeypatch.delenv("EHIP_MALL_CACHED_MB", raising=False)
+        else:
+            monkeypatch.setenv("EHIP_MALL_CACHED_MB", mb)
+        s, _ = W.build_system(w)
+        assert s.spmv_config(0)["format"] == "sell_shared"
+        _, rows, cols, _ = s.pattern()
+        vals, rhs = s.values()
+        y64 = s.spmv(u)
+        assert np.array_equal(y64, oracle_spmv(rows, cols, vals, u[:s.n]))
+        x, rep = s.solve(w["kw"], precision=precision, mode=mode, true_residual_stop=True)
+        assert rep["info"] == 1 and rep["res_true"] <= 1e-10, rep
+        y32 = s.spmv(u, precision=1) if precision == _lib.MIXED_IR else None
+        got.append((y64, y32, x, rep["iters"], rep["outer_iters"]))
+        s.free()
+    for y64, y32, x, it, up in got[1:]:
+        assert np.array_equal(y64, got[0][0]) and np.array_equal(x, got[0][2])
+        assert (it, up) == got[0][3:]
+        if y32 is not None:
+            assert np.array_equal(y32, got[0][1])
