@@ -232,6 +232,8 @@ struct ehip_system {
   int *ilu_rows = nullptr, *ilu_cols = nullptr, *ilu_diag = nullptr, *ilu_amap = nullptr;
   double *ilu = nullptr;
   int *ilu_lord = nullptr, *ilu_uord = nullptr;
+  double *ilu_y = nullptr;             // n: the forward sweep's result
+  int *ilu_sync = nullptr;             // the sweeps' two tickets and error flag
   std::vector<int> ilu_loff, ilu_uoff;
   hipGraphExec_t ilu_graph[2] = {};
   const double *ilu_graph_src[2] = {};
@@ -347,6 +349,7 @@ int assembly_check(ehip_system *s);
 int ilu_setup(ehip_system *s, int order);
 int ilu_factor(ehip_system *s);
 int ilu_apply(ehip_system *s, const double *src, double *dst);
+int ilu_check(ehip_system *s);  // a sweep gave up waiting: EHIP_EHIP
 void ilu_release(ehip_system *s);
 
 // ehip_solve's body.  events = false times the solve with host clock

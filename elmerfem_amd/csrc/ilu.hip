@@ -132,6 +132,128 @@ __global__ __launch_bounds__(kIluBlock) void k_lu_bwd(const int *__restrict__ ro
   dst[i] = ilu[di] * lu_part(dst[i], di + 1, rows[i + 1], cols, ilu, dst);
 }
 
+// ---------------------------------------------------------------- dependency-driven sweeps
+// One launch per triangular solve instead of one per level.  Every output
+// entry starts as kPending, a signalling-NaN bit pattern no arithmetic
+// produces (a NaN result is the quiet canonical one); a row's lane reloads
+// the entries its part names (agent-scope atomic loads: the writer may sit
+// on another XCD) until none is pending, subtracts them in the reference's
+// order exactly as lu_part does -- the same bits as the level launches --
+// and publishes its value with one agent-scope atomic store, the value
+// being its own ready flag.  Rows are taken in level order by workgroups
+// numbered in the order they start (an atomic ticket), so a row waits only
+// for rows of workgroups already running or done, and a lane finishes its
+// row inside the wait loop, so lanes of one wave never wait for each
+// other's exit.  Every wait is bounded: past kSpinLimit rounds, or once any
+// lane has given up, a lane sets the error flag (sync[2]) and publishes a
+// NaN, and the solve reports the failure instead of hanging.
+constexpr unsigned long long kPending = 0x7ff4c0ffee5eed00ull;
+constexpr int kSpinLimit = 1 << 20;
+constexpr int kSweepGroups = 256;
+
+__device__ __forceinline__ unsigned long long ld_agent(const double *p) {
+  return __hip_atomic_load(reinterpret_cast<unsigned long long *>(const_cast<double *>(p)), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_lu_pending(int n, double *y, double *z, int *sync) {
+  const double p = __longlong_as_double((long long)kPending);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    y[i] = p;
+    z[i] = p;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 2) sync[threadIdx.x] = 0;  // the two sweeps' tickets
+}
+
+// One row of a sweep.  FWD: dst(i) = src(i) - sum over the lower part of
+// L(i,j) dst(j); else dst(i) = U(i,i)^-1 (src(i) - sum over the upper part
+// of U(i,j) dst(j)).  The lane first polls only the part's nearest entry
+// (column i - 1 or i + 1 as a rule, the dependency finished last), then
+// loads kLuU entries at a time; all of it in one loop, publishing the row
+// inside it.  The loop ends for the whole wave at once (when every lane is
+// done): with a per-lane exit the compiler may move the publishing store
+// after the loop, where a lane publishes only once every lane of its wave is
+// done -- and a lane waiting for another lane of its wave would never be.
+template <bool FWD>
+__device__ __forceinline__ void sweep_row(int i, const int *__restrict__ rows, const int *__restrict__ cols,
+                                          const int *__restrict__ diag, const double *__restrict__ ilu,
+                                          const double *__restrict__ src, double *dst, int *sync) {
+#pragma clang fp contract(off)
+  const int j0 = FWD ? rows[i] : diag[i] + 1;
+  const int len = (FWD ? diag[i] : rows[i + 1]) - j0;
+  const int jn = FWD ? j0 + len - 1 : j0;
+  double s = src[i];
+  int k0 = 0, spins = 0;
+  bool near = len == 0, done = false;
+  for (;;) {
+    if (__all(done)) break;  // wave-uniform exit
+    if (done) continue;
+    bool ready = true;
+    if (!near) ready = near = ld_agent(dst + cols[jn]) != kPending;
+    if (near && k0 < len) {
+      double a[kLuU];
+      int c[kLuU];
+#pragma unroll
+      for (int u = 0; u < kLuU; ++u) {
+        const int j = j0 + min(k0 + u, len - 1);  // clamped entries are dependencies too
+        a[u] = ilu[j];
+        c[u] = cols[j];
+      }
+      unsigned long long xb[kLuU];
+#pragma unroll
+      for (int u = 0; u < kLuU; ++u) xb[u] = ld_agent(dst + c[u]);
+#pragma unroll
+      for (int u = 0; u < kLuU; ++u) ready = ready && xb[u] != kPending;
+      if (ready) {
+#pragma unroll
+        for (int u = 0; u < kLuU; ++u) {
+          const double v = s - a[u] * __longlong_as_double((long long)xb[u]);
+          s = k0 + u < len ? v : s;
+        }
+        k0 += kLuU;
+      }
+    }
+    bool fail = false;
+    if (!ready) {
+      if (++spins > kSpinLimit || __hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        fail = true;
+      else
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if ((near && k0 >= len) || fail) {
+      if (fail) {
+        __hip_atomic_store(&sync[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s = __longlong_as_double(0x7ff8000000000000ll);
+      } else if (!FWD) {
+        s = ilu[diag[i]] * s;
+      }
+      __hip_atomic_store(reinterpret_cast<unsigned long long *>(dst + i), (unsigned long long)__double_as_longlong(s),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      done = true;
+    }
+  }
+}
+
+// A sweep over all rows in level order (`order`): each workgroup takes the
+// next kIluBlock rows by ticket until none is left, so the grid bounds the
+// rows in flight.
+template <bool FWD>
+__global__ __launch_bounds__(kIluBlock) void k_lu_sweep(const int *__restrict__ rows, const int *__restrict__ cols,
+                                                      const int *__restrict__ diag, const double *__restrict__ ilu,
+                                                      const double *__restrict__ src, double *dst,
+                                                      const int *__restrict__ order, int n, int *sync) {
+  __shared__ int ticket;
+  for (;;) {
+    __syncthreads();  // every lane has read the previous ticket
+    if (threadIdx.x == 0) ticket = atomicAdd(&sync[FWD ? 0 : 1], 1);
+    __syncthreads();
+    const int t0 = ticket * kIluBlock;
+    if (t0 >= n) return;
+    const int t = t0 + (int)threadIdx.x;
+    if (t < n) sweep_row<FWD>(order[t], rows, cols, diag, ilu, src, dst, sync);
+  }
+}
+
 // ---------------------------------------------------------------- host: pattern and levels
 
 namespace {
@@ -213,6 +335,10 @@ void ilu_release(ehip_system *s) {
   dfree(s, s->ilu, sizeof(double) * (size_t)s->ilu_nnz);
   dfree(s, s->ilu_lord, sizeof(int) * (size_t)s->n);
   dfree(s, s->ilu_uord, sizeof(int) * (size_t)s->n);
+  dfree(s, s->ilu_y, sizeof(double) * (size_t)s->n);
+  dfree(s, s->ilu_sync, sizeof(int) * 4);
+  s->ilu_y = nullptr;
+  s->ilu_sync = nullptr;
   for (int k = 0; k < 2; ++k)
     if (s->ilu_graph[k]) {
       (void)hipGraphExecDestroy(s->ilu_graph[k]);
@@ -253,10 +379,13 @@ int ilu_setup(ehip_system *s, int order) {
   s->ilu_nnz = (int64_t)K.size();
   if ((rc = dmalloc(s, (void **)&s->ilu, sizeof(double) * (size_t)s->ilu_nnz)) ||
       (rc = dmalloc(s, (void **)&s->ilu_lord, sizeof(int) * (size_t)n)) ||
-      (rc = dmalloc(s, (void **)&s->ilu_uord, sizeof(int) * (size_t)n)))
+      (rc = dmalloc(s, (void **)&s->ilu_uord, sizeof(int) * (size_t)n)) ||
+      (rc = dmalloc(s, (void **)&s->ilu_y, sizeof(double) * (size_t)n)) ||
+      (rc = dmalloc(s, (void **)&s->ilu_sync, sizeof(int) * 4)))
     return rc;
   EHIP_HIP_TRY(hipMemcpy(s->ilu_lord, lord.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   EHIP_HIP_TRY(hipMemcpy(s->ilu_uord, uord.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+  EHIP_HIP_TRY(hipMemset(s->ilu_sync, 0, sizeof(int) * 4));
   if (order > 0) {
     // the matrix's nonzeros inside the ILU rows (a subset of them, both sorted)
     std::vector<int> amap((size_t)s->nnz);
@@ -311,7 +440,30 @@ int ilu_factor(ehip_system *s) {
   return EHIP_OK;
 }
 
+// The LU solve as two dependency-driven sweeps (the default), or as one
+// launch per level (EHIP_ILU_SWEEP=levels); both give the same bits.
+static bool level_launches() {
+  const char *v = std::getenv("EHIP_ILU_SWEEP");
+  return v && v[0] == 'l';
+}
+
+// Workgroups of a sweep (EHIP_ILU_SWEEP_WG overrides).
+static int sweep_groups(int g) {
+  const char *v = std::getenv("EHIP_ILU_SWEEP_WG");
+  const int w = v && *v ? std::atoi(v) : kSweepGroups;
+  return std::max(1, std::min(g, w));
+}
+
 static void ilu_sweeps(ehip_system *s, const double *src, double *dst) {
+  if (!level_launches()) {
+    const int n = s->n, g = (n + kIluBlock - 1) / kIluBlock, w = sweep_groups(g);
+    launch_k(k_lu_pending, dim3(std::min(2048, g)), dim3(kIluBlock), s->stream, n, s->ilu_y, dst, s->ilu_sync);
+    launch_k(k_lu_sweep<true>, dim3(w), dim3(kIluBlock), s->stream, s->ilu_rows, s->ilu_cols, s->ilu_diag, s->ilu,
+             src, s->ilu_y, s->ilu_lord, n, s->ilu_sync);
+    launch_k(k_lu_sweep<false>, dim3(w), dim3(kIluBlock), s->stream, s->ilu_rows, s->ilu_cols, s->ilu_diag, s->ilu,
+             (const double *)s->ilu_y, dst, s->ilu_uord, n, s->ilu_sync);
+    return;
+  }
   const std::vector<int> &lo = s->ilu_loff, &uo = s->ilu_uoff;
   for (size_t l = 0; l + 1 < lo.size(); ++l) {
     const int cnt = lo[l + 1] - lo[l];
@@ -358,6 +510,19 @@ int ilu_apply(ehip_system *s, const double *src, double *dst) {
     s->ilu_graph_dst[slot] = dst;
   }
   EHIP_HIP_TRY(hipGraphLaunch(s->ilu_graph[slot], s->stream));
+  return EHIP_OK;
+}
+
+int ilu_check(ehip_system *s) {
+  if (!s->ilu_sync) return EHIP_OK;
+  int flag = 0;
+  EHIP_HIP_TRY(hipMemcpyAsync(&flag, s->ilu_sync + 2, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
+  if (flag) {
+    EHIP_HIP_TRY(hipMemset(s->ilu_sync + 2, 0, sizeof(int)));
+    set_error("ILU solve: a row waited past the spin limit for the rows it depends on");
+    return EHIP_EHIP;
+  }
   return EHIP_OK;
 }
 
@@ -410,7 +575,7 @@ extern "C" int ehip_lu_solve(ehip_system *s, const double *b, double *x) {
   if ((rc = ilu_apply(s, src, dst))) return rc;
   EHIP_HIP_TRY(hipMemcpyAsync(x, dst, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
   EHIP_HIP_TRY(hipStreamSynchronize(s->stream));
-  return EHIP_OK;
+  return ilu_check(s);
 }
 
 extern "C" int ehip_ilu_info(const ehip_system *s, int *order, int64_t *ilu_nnz, int *levels_forward,

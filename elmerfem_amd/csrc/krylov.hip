@@ -2231,7 +2231,7 @@ static int ilu_krylov(ehip_system *s, SolveCtx &c, int order) {
   }
   EHIP_HIP_TRY(hipGetLastError());
   EHIP_HIP_TRY(take_launch_error());
-  return EHIP_OK;
+  return ilu_check(s);
 }
 
 // Time marks of a solve: HIP events on the system's stream, or, for callers
