@@ -129,6 +129,11 @@ extern "C" int ehip_solve_csr_noalloc(int n, const int *rows, const int *cols, c
     set_error("ehip_solve_csr_noalloc: only FP64 solves without history or profiling");
     return EHIP_EUNSUP;
   }
+  if (p->precond != EHIP_PRECOND_NONE && p->precond != EHIP_PRECOND_DIAGONAL) {
+    // ILU(k) factors and level lists are allocated per system
+    set_error("ehip_solve_csr_noalloc: only no preconditioner or the diagonal (ILU allocates its factors)");
+    return EHIP_EUNSUP;
+  }
   int dev = 0;
   EHIP_HIP_TRY(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) {

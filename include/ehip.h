@@ -239,8 +239,9 @@ int ehip_solve_csr(int n, const int *rows, const int *cols, const int *diag, con
  * creates no stream or event; it runs on the null stream of the current
  * device and times the solve with the host clock.
  * Limits: n <= EHIP_NOALLOC_MAX_N and nnz <= EHIP_NOALLOC_MAX_NNZ (else
- * EHIP_ECAP); precision EHIP_FP64 with record_history = 0 and
- * profile_every = 0 (else EHIP_EUNSUP).  The SpMV is the CSR stream layout,
+ * EHIP_ECAP); precision EHIP_FP64 with record_history = 0,
+ * profile_every = 0 and precond EHIP_PRECOND_NONE or _DIAGONAL (else
+ * EHIP_EUNSUP).  The SpMV is the CSR stream layout,
  * whose FP64 row sums equal every other layout's.
  * All calls on one GPU share its array and leave their data in it: calls
  * that overlap on one GPU (several host threads) race, and the caller must

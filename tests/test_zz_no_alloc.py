@@ -96,6 +96,11 @@ def test_noalloc_limits_and_options(ehip):
     n = 64
     rc, _, _ = _call(n, *_identity(n), kw.to_params(_lib.MIXED_IR, _lib.FAST))
     assert rc == _lib.EUNSUP
+    # ILU(k) would allocate its factors: refused before any device work
+    kw_ilu = from_solver_section({"Linear System Iterative Method": "CG", "Linear System Preconditioning": "ILU0",
+                                  "Linear System Convergence Tolerance": 1e-10, "Linear System Max Iterations": 100})
+    rc, _, _ = _call(n, *_identity(n), kw_ilu.to_params(_lib.FP64, _lib.FAST))
+    assert rc == _lib.EUNSUP
     # within the limits the identity system solves at once
     rc, x, rep = _call(n, *_identity(n), kw.to_params(_lib.FP64, _lib.FAST))
     assert rc == 0 and rep.info == 1 and np.allclose(x, 1.0, rtol=1e-12)
