@@ -181,24 +181,33 @@ __device__ __forceinline__ void sweep_row(int i, const int *__restrict__ rows, c
 #pragma clang fp contract(off)
   const int j0 = FWD ? rows[i] : diag[i] + 1;
   const int len = (FWD ? diag[i] : rows[i + 1]) - j0;
-  const int jn = FWD ? j0 + len - 1 : j0;
   double s = src[i];
   int k0 = 0, spins = 0;
   bool near = len == 0, done = false;
+  // the part's factors and columns do not change during the sweep: the
+  // current chunk's are loaded before the wait, so a dependency that turns
+  // ready costs one more trip (its values), not two
+  double a[kLuU];
+  int c[kLuU];
+  auto load_chunk = [&](int k) {
+#pragma unroll
+    for (int u = 0; u < kLuU; ++u) {
+      const int j = j0 + min(k + u, len - 1);  // clamped entries are dependencies too
+      a[u] = ilu[j];
+      c[u] = cols[j];
+    }
+  };
+  int cn = 0;
+  if (len > 0) {
+    load_chunk(0);
+    cn = cols[FWD ? j0 + len - 1 : j0];
+  }
   for (;;) {
     if (__all(done)) break;  // wave-uniform exit
     if (done) continue;
     bool ready = true;
-    if (!near) ready = near = ld_agent(dst + cols[jn]) != kPending;
+    if (!near) ready = near = ld_agent(dst + cn) != kPending;
     if (near && k0 < len) {
-      double a[kLuU];
-      int c[kLuU];
-#pragma unroll
-      for (int u = 0; u < kLuU; ++u) {
-        const int j = j0 + min(k0 + u, len - 1);  // clamped entries are dependencies too
-        a[u] = ilu[j];
-        c[u] = cols[j];
-      }
       unsigned long long xb[kLuU];
 #pragma unroll
       for (int u = 0; u < kLuU; ++u) xb[u] = ld_agent(dst + c[u]);
@@ -211,6 +220,7 @@ __device__ __forceinline__ void sweep_row(int i, const int *__restrict__ rows, c
           s = k0 + u < len ? v : s;
         }
         k0 += kLuU;
+        if (k0 < len) load_chunk(k0);
       }
     }
     bool fail = false;
